@@ -1,0 +1,255 @@
+/*
+ * plato_hip.h -- C-ABI of libplato_hip.so, the MI355X (gfx950) dual-mode
+ * push/pull edge engine.
+ *
+ * Conventions
+ *  - Every function returns an int status (PE_OK = 0).  Nothing throws or
+ *    aborts across this boundary; pe_ctx_last_error() holds the message of the
+ *    last failure on a context.
+ *  - Host pointers are borrowed for the duration of the call only.  Device
+ *    objects (graphs, vectors, bitmaps, batches, servers) are owned by their
+ *    handle and freed by the matching *_destroy.
+ *  - Work is issued on the context's stream; functions that return results to
+ *    host memory synchronize that stream before returning.
+ *  - A context is driven by one host thread at a time.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).  The reference is a header-only C++
+ * template engine whose operators are host lambdas; here the operators are a
+ * closed set selected by the entry point (gather-sum, weighted gather-sum,
+ * restart-apply, BFS push/pull claim, PPR push-add).
+ */
+#ifndef PLATO_HIP_H
+#define PLATO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define PE_OK 0
+#define PE_ERR_INVALID 1   /* bad argument */
+#define PE_ERR_HIP 2       /* HIP runtime error */
+#define PE_ERR_OOM 3       /* device or host allocation failed */
+#define PE_ERR_STATE 4     /* object in the wrong state for the call */
+#define PE_ERR_RANGE 5     /* index or size out of range */
+#define PE_ERR_INTERNAL 6  /* unexpected internal failure */
+
+/* ---- dtypes of engine state vectors -------------------------------------- */
+#define PE_F32 0
+#define PE_F64 1
+#define PE_U32 2
+
+/* ---- per-kernel timers (pe_ctx_kernel_time) ------------------------------- */
+#define PE_KT_PULL_TILE 0   /* pull gather + fused apply (merge-path tiles) */
+#define PE_KT_PULL_FIXUP 1  /* rows crossing tile boundaries */
+#define PE_KT_BFS_PUSH 2    /* top-down BFS step */
+#define PE_KT_BFS_PULL 3    /* bottom-up BFS step */
+#define PE_KT_PPR_BLOCK 4   /* batched PPR k-column gather */
+#define PE_KT_COUNT 8
+
+typedef struct pe_ctx_s pe_ctx_t;
+typedef struct pe_graph_s pe_graph_t;
+typedef struct pe_vec_s pe_vec_t;
+typedef struct pe_bitmap_s pe_bitmap_t;
+
+/* ===========================================================================
+ * Context.  Replaces cluster_info_t (plato/graph/base.hpp:158-233) for one
+ * GPU: device, stream, allocation ledger.
+ * ========================================================================= */
+int pe_ctx_create(int device, pe_ctx_t** out);
+int pe_ctx_destroy(pe_ctx_t* ctx);
+const char* pe_ctx_last_error(const pe_ctx_t* ctx);
+int pe_ctx_sync(pe_ctx_t* ctx);
+/* free/total device memory and the bytes this context holds */
+int pe_ctx_mem_info(pe_ctx_t* ctx, uint64_t* free_bytes, uint64_t* total_bytes,
+                    uint64_t* ctx_bytes);
+/* event timer on the context stream (stop synchronizes) */
+int pe_ctx_timer_start(pe_ctx_t* ctx);
+int pe_ctx_timer_stop(pe_ctx_t* ctx, double* ms);
+/* per-kernel timing: when enabled, launches of the kernels named by PE_KT_*
+ * are bracketed by events; pe_ctx_kernel_time drains them (synchronizes) and
+ * returns the accumulated milliseconds and launch count, then resets. */
+int pe_ctx_set_kernel_timing(pe_ctx_t* ctx, int enable);
+int pe_ctx_kernel_time(pe_ctx_t* ctx, int which, double* total_ms, uint64_t* launches);
+/* multiprocessor count of the device */
+int pe_ctx_num_cus(pe_ctx_t* ctx, int* cus);
+
+/* ===========================================================================
+ * Graph.  Replaces dcsc_t (plato/graph/structure/dcsc.hpp:58-185) and
+ * bcsr_t (plato/graph/structure/bcsr.hpp:67-208) with a device CSC over all V
+ * ids (uint64 col_ptr[V+1], uint32 row_idx[nnz], ascending source id inside a
+ * column) and an optional CSR, built on the device, and the factories
+ * create_dcsc_seqs_from_path / create_dualmode_seq_from_path
+ * (plato/graph/structure.hpp:473-555, 987-1086) for in-memory edge lists.
+ * ========================================================================= */
+#define PE_GRAPH_SYMMETRIZE 1u /* add v->u for every u->v (is_directed_ = false) */
+#define PE_GRAPH_BUILD_CSR 2u  /* also build out-edges (needed by push mode) */
+#define PE_GRAPH_WEIGHTED 4u   /* keep per-edge weights, normalised per source */
+
+typedef struct {
+  uint32_t V;             /* id space [0, V) */
+  uint32_t max_v_i;       /* largest id with an edge (graph_info_t::max_v_i_) */
+  uint32_t vertices;      /* ids with at least one edge (graph_info_t::vertices_) */
+  uint32_t max_in_degree;
+  uint64_t edges;         /* input edges (graph_info_t::edges_) */
+  uint64_t nnz;           /* CSC entries: edges, or 2*edges when symmetrized */
+  int32_t is_directed;    /* graph_info_t::is_directed_ */
+  int32_t has_csr;
+  int32_t has_weights;
+  uint32_t tile_items;    /* merge-path items per pull tile */
+  uint64_t num_tiles;
+  uint64_t num_fixup_rows;
+  uint64_t device_bytes;
+} pe_graph_info_t;
+
+/* Build from a host edge list (src[i] -> dst[i], optional weights w[i]).
+ * Ids must be < V.  Self-loops and duplicates are kept (dcsc.hpp:350-394). */
+int pe_graph_create(pe_ctx_t* ctx, const uint32_t* src, const uint32_t* dst, const float* w,
+                    uint64_t num_edges, uint32_t V, uint32_t flags, pe_graph_t** out);
+/* Build a seeded RMAT graph directly on the device: 2^scale ids,
+ * edge_factor * 2^scale edges, quadrant probabilities a, b, c (d = 1-a-b-c).
+ * Counter-based: edge e depends only on (seed, e); weights (PE_GRAPH_WEIGHTED)
+ * are uniform in (0, 1] on a 2^-24 grid. */
+int pe_graph_create_rmat(pe_ctx_t* ctx, uint32_t scale, uint32_t edge_factor, uint64_t seed,
+                         double a, double b, double c, uint32_t flags, pe_graph_t** out);
+/* Generate RMAT edges [e0, e0+n) into host arrays (same generator). */
+int pe_rmat_edges(pe_ctx_t* ctx, uint32_t scale, uint64_t seed, double a, double b, double c,
+                  uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst, float* w);
+int pe_graph_destroy(pe_graph_t* g);
+int pe_graph_get_info(const pe_graph_t* g, pe_graph_info_t* info);
+/* Copy the CSC (col_ptr[V+1], row_idx[nnz], optional normalised weights[nnz]). */
+int pe_graph_download_csc(pe_graph_t* g, uint64_t* col_ptr, uint32_t* row_idx, float* wn);
+int pe_graph_download_csr(pe_graph_t* g, uint64_t* row_ptr, uint32_t* col_idx);
+/* Re-tile the pull gather (merge-path items per tile: 1024, 2048 or 4096). */
+int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items);
+
+/* ===========================================================================
+ * Engine state.  pe_vec replaces dense_state_t
+ * (plato/graph/state/dense_state.hpp:43-175; engine alloc_v_state,
+ * plato/engine/dualmode.hpp:231-236); pe_bitmap replaces bitmap_t
+ * (plato/util/bitmap.hpp:43-350; engine alloc_v_subset, dualmode.hpp:238-242),
+ * same word layout: (bits>>6)+1 uint64 words, bit i of word i>>6 is i&63.
+ * ========================================================================= */
+int pe_vec_create(pe_ctx_t* ctx, int dtype, uint64_t n, pe_vec_t** out);
+int pe_vec_destroy(pe_vec_t* v);
+int pe_vec_fill(pe_vec_t* v, double value);                         /* dense_state.hpp:241-254 */
+int pe_vec_upload(pe_vec_t* v, const void* host, uint64_t n);        /* raw, same dtype */
+int pe_vec_download(pe_vec_t* v, void* host, uint64_t n);            /* raw, same dtype */
+int pe_vec_download_f64(pe_vec_t* v, double* host, uint64_t n);      /* converted */
+int pe_vec_set(pe_vec_t* v, uint64_t i, double value);               /* operator[] store */
+/* sum over all entries in FP64 (dense_state_t::foreach reduction) */
+int pe_vec_sum(pe_vec_t* v, double* out);
+
+int pe_bitmap_create(pe_ctx_t* ctx, uint64_t bits, pe_bitmap_t** out);
+int pe_bitmap_destroy(pe_bitmap_t* b);
+int pe_bitmap_clear(pe_bitmap_t* b);                                 /* bitmap.hpp clear */
+int pe_bitmap_fill(pe_bitmap_t* b);                                  /* bitmap.hpp:168-179 */
+int pe_bitmap_set_bit(pe_bitmap_t* b, uint64_t i);                   /* bitmap.hpp:219-222 */
+int pe_bitmap_clr_bit(pe_bitmap_t* b, uint64_t i);
+int pe_bitmap_get_bit(pe_bitmap_t* b, uint64_t i, int* out);
+int pe_bitmap_count(pe_bitmap_t* b, uint64_t* out);                  /* bitmap.hpp:229-238 */
+int pe_bitmap_msb(pe_bitmap_t* b, uint64_t* out);                    /* msb, 0 when empty */
+int pe_bitmap_copy_from(pe_bitmap_t* dst, const pe_bitmap_t* src);
+int pe_bitmap_upload(pe_bitmap_t* b, const uint64_t* words, uint64_t nwords);
+int pe_bitmap_download(pe_bitmap_t* b, uint64_t* words, uint64_t nwords);
+/* ascending list of set bits (bitmap.hpp to_vector); *count gets the number */
+int pe_bitmap_to_vector(pe_bitmap_t* b, uint32_t* ids, uint64_t cap, uint64_t* count);
+
+/* ===========================================================================
+ * Engine operators (plato/engine/dualmode.hpp, plato/graph/message_passing.hpp)
+ * ========================================================================= */
+/* out-degrees counted from the in-edges: generate_dense_out_degrees_fg
+ * (plato/graph/structure.hpp:292-346) */
+int pe_graph_out_degrees(pe_graph_t* g, uint32_t* od_host);
+int pe_graph_in_degrees(pe_graph_t* g, uint32_t* id_host);
+/* sum of out-degrees over the active set: dualmode.hpp:264-277 */
+int pe_active_edges(pe_graph_t* g, pe_bitmap_t* actives, uint64_t* out);
+/* push/pull decision of foreach_edges: push iff (double)A/(double)E' < threshold
+ * (dualmode.hpp:261-279); *is_push = 1 for push */
+int pe_choose_mode(pe_graph_t* g, pe_bitmap_t* actives, double threshold, int* is_push,
+                   uint64_t* active_edges);
+
+/* Pull-mode aggregate_message (message_passing.hpp:88-144) with the gather-sum
+ * signal and the write_add slot of pagerank.hpp:104-116: y[v] = sum of x[u]
+ * over in-edges u->v (times the normalised weight when weighted != 0, as
+ * weighted_pagerank.hpp:115-128).  x, y: PE_F32 or PE_F64, length V. */
+int pe_pull_sum(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted);
+
+/* Restart-apply of the rank algorithms (the dense_state_t::foreach lambdas of
+ * pagerank.hpp:118-140, weighted_pagerank.hpp:130-148,
+ * personalized_pagerank.hpp:101-125, trustrank.hpp:226-250). */
+typedef struct {
+  double damping;        /* d */
+  double base;           /* restart added to every vertex: 1-d for PageRank, else 0 */
+  double seed_mass;      /* restart added to seed vertices: (1-d)/|S| */
+  const pe_bitmap_t* seeds; /* seed set or NULL */
+  int32_t divide_by_od;  /* state holds rank/od (PageRank, PPR) vs rank (weighted) */
+  int32_t final_iter;    /* last iteration: store the undivided rank, delta 0 */
+} pe_apply_params_t;
+/* y holds gathered sums on entry and the new state on exit; x is the current
+ * state; *delta gets the reference's delta (FP64). */
+int pe_rank_apply(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, const pe_apply_params_t* p,
+                  double* delta);
+/* Fused pull iteration: pe_pull_sum + pe_rank_apply in one pass. */
+int pe_rank_iterate(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted,
+                    const pe_apply_params_t* p, double* delta);
+
+/* ===========================================================================
+ * Algorithms (plato/algo/...).  Whole runs on the device; results returned as
+ * FP64 host arrays like the reference's dense_state_t<double>.
+ * ========================================================================= */
+typedef struct {
+  uint32_t iterations;   /* pagerank_opts_t::iteration_ */
+  double damping;        /* pagerank_opts_t::damping_ */
+  double eps;            /* pagerank_opts_t::eps_ (0: run all iterations) */
+  int32_t precision;     /* 32: FP32 state, compensated sums; 64: FP64 state */
+  int32_t weighted;      /* weighted_pagerank semantics */
+  double l1_norm_tol;    /* >0: also stop when delta/sum(rank) < tol (then one final iteration) */
+} pe_rank_opts_t;
+
+typedef struct {
+  uint32_t iterations_run;
+  double last_delta;
+  double seconds;        /* device time of the iterations */
+  double gteps;          /* (iterations * nnz) / seconds / 1e9 */
+} pe_rank_stats_t;
+
+/* plato::algo::pagerank (plato/algo/pagerank/pagerank.hpp:60-149) and
+ * weighted_pagerank (plato/algo/pagerank/weighted_pagerank.hpp:60-157).
+ * out: host FP64 [V]; deltas (optional): host FP64 [iterations]. */
+int pe_pagerank(pe_graph_t* g, const pe_rank_opts_t* opts, double* out, double* deltas,
+                pe_rank_stats_t* stats);
+/* plato::algo::personalized_pagerank, pull form
+ * (plato/algo/ppr/personalized_pagerank.hpp:40-134) for nseeds == 1, and the
+ * TrustRank restart over a seed set (plato/algo/trustrank/trustrank.hpp:197-250)
+ * for nseeds > 1 (duplicates count once). */
+int pe_ppr(pe_graph_t* g, const uint32_t* seeds, uint32_t nseeds, const pe_rank_opts_t* opts,
+           double* out, double* deltas, pe_rank_stats_t* stats);
+/* push/pull PPR (personalized_pagerank.hpp:137-275); modes (optional) gets
+ * 1 = push, 0 = pull per executed iteration.  Needs PE_GRAPH_BUILD_CSR. */
+int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, double* out,
+                    double* deltas, uint8_t* modes, pe_rank_stats_t* stats);
+
+typedef struct {
+  uint32_t epochs;
+  uint32_t visited;       /* |visited| (the reference's return value) */
+  uint64_t reached_edges; /* sum of od over reached vertices */
+  double seconds;
+} pe_bfs_stats_t;
+/* plato::algo::breadth_first_search (plato/algo/bfs/bfs.hpp:55-155) with the
+ * engine switch (dualmode.hpp:261-279).  levels: host int32 [V] (root 0,
+ * unreached -1).  modes / frontier / active_edges (optional, max_epochs long)
+ * get the per-epoch decision (1 push, 0 pull), |F| and sum of od over F.
+ * Needs PE_GRAPH_BUILD_CSR for directed graphs. */
+int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint8_t* modes,
+           uint64_t* frontier, uint64_t* active_edges, uint32_t max_epochs,
+           pe_bfs_stats_t* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLATO_HIP_H */

@@ -1,0 +1,443 @@
+/*
+ * oracle.c -- CPU restatement (FP64) of the reference's dual-mode engine
+ * clients.  TEST INFRASTRUCTURE ONLY: loaded by tests/, smoke() and the
+ * cpu_baseline leg of bench.py; never by the product library.
+ *
+ * See oracle.h for the contract of each function, the reference lines it
+ * restates and the test that pins it.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- synthetic graph ------------------------------------------------------ */
+
+uint64_t orc_mix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void orc_rmat_thresholds(double a, double b, double c, uint32_t out[3]) {
+  const double p[3] = {a, a + b, a + b + c};
+  for (int i = 0; i < 3; ++i) {
+    double t = floor(p[i] * 4294967296.0);
+    if (t < 0) t = 0;
+    if (t > 4294967295.0) t = 4294967295.0;
+    out[i] = (uint32_t)t;
+  }
+}
+
+/* draw(e, j) = mix64(mix64(seed) ^ (e*32 + j)); level l uses the low (l even)
+ * or high (l odd) half of draw(e, l/2); level l decides bit (scale-1-l).
+ * Quadrant: u < t0 -> (0,0); u < t1 -> dst bit; u < t2 -> src bit; else both. */
+void orc_rmat_edges(uint32_t scale, uint64_t seed, const uint32_t thr[3],
+                    uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst) {
+  const uint64_t sk = orc_mix64(seed);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t e = e0 + i;
+    uint32_t s = 0, d = 0;
+    uint64_t word = 0;
+    for (uint32_t l = 0; l < scale; ++l) {
+      if ((l & 1u) == 0) word = orc_mix64(sk ^ (e * 32u + (l >> 1)));
+      const uint32_t u = (l & 1u) ? (uint32_t)(word >> 32) : (uint32_t)word;
+      const uint32_t bit = 1u << (scale - 1u - l);
+      if (u >= thr[0]) {
+        if (u < thr[1]) d |= bit;
+        else if (u < thr[2]) s |= bit;
+        else { s |= bit; d |= bit; }
+      }
+    }
+    src[i] = s;
+    dst[i] = d;
+  }
+}
+
+float orc_rmat_weight(uint64_t seed, uint64_t e) {
+  const uint64_t r = orc_mix64(orc_mix64(seed) ^ (e * 32u + 31u));
+  return (float)((r >> 40) + 1u) * (1.0f / 16777216.0f);
+}
+
+/* ---- structures ----------------------------------------------------------- */
+
+/* entry i of the (possibly symmetrized) entry list */
+static inline void entry_at(uint64_t E, const uint32_t* src, const uint32_t* dst,
+                            uint64_t i, uint32_t* s, uint32_t* d) {
+  if (i < E) { *s = src[i]; *d = dst[i]; }
+  else { *s = dst[i - E]; *d = src[i - E]; }
+}
+
+/* Stable LSD counting sort of entries by (major, minor): first by minor, then
+ * by major.  major_is_dst selects CSC (group by dst) or CSR (group by src). */
+static int build_grouped(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                         const float* w, int symmetrize, int major_is_dst,
+                         uint64_t* ptr, uint32_t* idx, float* w_out) {
+  const uint64_t nnz = E * (symmetrize ? 2u : 1u);
+  uint64_t* cnt = (uint64_t*)calloc((size_t)V + 1, sizeof(uint64_t));
+  uint64_t* perm = (uint64_t*)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+  uint64_t* perm2 = (uint64_t*)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+  if (!cnt || !perm || !perm2) { free(cnt); free(perm); free(perm2); return -1; }
+  uint32_t s, d;
+  /* pass 1: by minor key */
+  for (uint64_t i = 0; i < nnz; ++i) {
+    entry_at(E, src, dst, i, &s, &d);
+    cnt[(major_is_dst ? s : d) + 1]++;
+  }
+  for (uint32_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
+  for (uint64_t i = 0; i < nnz; ++i) {
+    entry_at(E, src, dst, i, &s, &d);
+    perm[cnt[major_is_dst ? s : d]++] = i;
+  }
+  /* pass 2: by major key, stable */
+  memset(cnt, 0, sizeof(uint64_t) * ((size_t)V + 1));
+  for (uint64_t i = 0; i < nnz; ++i) {
+    entry_at(E, src, dst, i, &s, &d);
+    cnt[(major_is_dst ? d : s) + 1]++;
+  }
+  for (uint32_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
+  memcpy(ptr, cnt, sizeof(uint64_t) * ((size_t)V + 1));
+  for (uint64_t k = 0; k < nnz; ++k) {
+    const uint64_t i = perm[k];
+    entry_at(E, src, dst, i, &s, &d);
+    perm2[cnt[major_is_dst ? d : s]++] = i;
+  }
+  for (uint64_t k = 0; k < nnz; ++k) {
+    const uint64_t i = perm2[k];
+    entry_at(E, src, dst, i, &s, &d);
+    idx[k] = major_is_dst ? s : d;
+    if (w && w_out) w_out[k] = w[i < E ? i : i - E];
+  }
+  free(cnt); free(perm); free(perm2);
+  return 0;
+}
+
+int orc_build_csc(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                  const float* w, int symmetrize,
+                  uint64_t* col_ptr, uint32_t* row_idx, float* w_out) {
+  for (uint64_t i = 0; i < E; ++i)
+    if (src[i] >= V || dst[i] >= V) return -2;
+  return build_grouped(V, E, src, dst, w, symmetrize, 1, col_ptr, row_idx, w_out);
+}
+
+int orc_build_csr(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                  int symmetrize, uint64_t* row_ptr, uint32_t* col_idx) {
+  for (uint64_t i = 0; i < E; ++i)
+    if (src[i] >= V || dst[i] >= V) return -2;
+  return build_grouped(V, E, src, dst, NULL, symmetrize, 0, row_ptr, col_idx, NULL);
+}
+
+void orc_out_degrees(uint32_t V, uint64_t nnz, const uint32_t* row_idx, uint32_t* od) {
+  memset(od, 0, sizeof(uint32_t) * V);
+  for (uint64_t e = 0; e < nnz; ++e) od[row_idx[e]]++;
+}
+
+/* ---- rank algorithms ------------------------------------------------------ */
+
+#define ORC_CHUNK 4096u
+
+/* Deterministic parallel sum: fixed 4096-vertex chunks, summed in chunk order. */
+static double chunked_sum(const double* part, uint32_t nchunks) {
+  double s = 0.0;
+  for (uint32_t i = 0; i < nchunks; ++i) s += part[i];
+  return s;
+}
+
+double orc_pagerank_step(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                         const uint32_t* od, double damping, int final_iter,
+                         const double* x, double* y, uint32_t v_lo, uint32_t v_hi,
+                         int nthreads) {
+  (void)V;
+  if (v_hi <= v_lo) return 0.0;
+  const uint32_t nchunks = (v_hi - v_lo + ORC_CHUNK - 1) / ORC_CHUNK;
+  double* part = (double*)calloc(nchunks, sizeof(double));
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (uint32_t c = 0; c < nchunks; ++c) {
+    const uint32_t lo = v_lo + c * ORC_CHUNK;
+    const uint32_t hi = (lo + ORC_CHUNK < v_hi) ? lo + ORC_CHUNK : v_hi;
+    double dl = 0.0;
+    for (uint32_t v = lo; v < hi; ++v) {
+      /* signal: rank_sum over the column in column order (pagerank.hpp:105-111) */
+      double sum = 0.0;
+      for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) sum += x[row_idx[e]];
+      /* slot + apply (pagerank.hpp:112-135) */
+      double r = 1.0 - damping + damping * sum;
+      if (final_iter) {
+        y[v] = r;
+      } else if (od[v] > 0) {
+        r = r / od[v];
+        y[v] = r;
+        dl += fabs(r - x[v]) * od[v];
+      } else {
+        y[v] = r;
+        dl += fabs(r - x[v]);
+      }
+    }
+    part[c] = dl;
+  }
+  const double delta = chunked_sum(part, nchunks);
+  free(part);
+  return final_iter ? 0.0 : delta;
+}
+
+int orc_pagerank(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                 const uint32_t* od, uint32_t iters, double damping, double eps,
+                 double* out, double* deltas, int nthreads) {
+  double* x = (double*)malloc(sizeof(double) * (V ? V : 1));
+  double* y = (double*)malloc(sizeof(double) * (V ? V : 1));
+  if (!x || !y) { free(x); free(y); return -1; }
+  /* init (pagerank.hpp:90-98) */
+  for (uint32_t v = 0; v < V; ++v) x[v] = od[v] > 0 ? 1.0 / od[v] : 1.0;
+  int executed = 0;
+  for (uint32_t t = 0; t < iters; ++t) {
+    const int final_iter = (iters - 1 == t);
+    const double delta = orc_pagerank_step(V, col_ptr, row_idx, od, damping, final_iter,
+                                           x, y, 0, V, nthreads);
+    if (deltas) deltas[executed] = delta;
+    ++executed;
+    /* early stop jumps to one more, final, iteration (pagerank.hpp:137-139) */
+    if (!final_iter && eps > 0.0 && delta < eps) t = iters - 2;
+    double* tmp = x; x = y; y = tmp;
+  }
+  memcpy(out, x, sizeof(double) * V);
+  free(x); free(y);
+  return executed;
+}
+
+int orc_weighted_pagerank(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                          const float* w, uint32_t iters, double damping, double eps,
+                          double* out, double* deltas) {
+  double* W = (double*)calloc(V ? V : 1, sizeof(double));
+  double* x = (double*)malloc(sizeof(double) * (V ? V : 1));
+  double* y = (double*)malloc(sizeof(double) * (V ? V : 1));
+  if (!W || !x || !y) { free(W); free(x); free(y); return -1; }
+  /* sum of out-weights per source, from the in-edges (weighted_pagerank.hpp:86-99) */
+  for (uint32_t v = 0; v < V; ++v)
+    for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) W[row_idx[e]] += (double)w[e];
+  for (uint32_t v = 0; v < V; ++v) x[v] = 1.0;  /* (104-109) */
+  int executed = 0;
+  for (uint32_t t = 0; t < iters; ++t) {
+    const int final_iter = (iters - 1 == t);
+    double delta = 0.0;
+    for (uint32_t v = 0; v < V; ++v) {
+      double sum = 0.0;
+      for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
+        const uint32_t u = row_idx[e];
+        const double beta = ((double)w[e] * 1.0) / W[u];  /* (119-120) */
+        sum += x[u] * beta;
+      }
+      const double r = 1.0 - damping + damping * sum;
+      y[v] = r;
+      if (!final_iter) delta += fabs(r - x[v]);  /* (138-143) */
+    }
+    if (deltas) deltas[executed] = final_iter ? 0.0 : delta;
+    ++executed;
+    if (!final_iter && eps > 0.0 && delta < eps) t = iters - 2;
+    double* tmp = x; x = y; y = tmp;
+  }
+  memcpy(out, x, sizeof(double) * V);
+  free(W); free(x); free(y);
+  return executed;
+}
+
+/* seed membership as a byte map (a seed bitmap: duplicates count once) */
+static uint8_t* seed_map(uint32_t V, const uint32_t* seeds, uint32_t nseeds, uint32_t* count) {
+  uint8_t* m = (uint8_t*)calloc(V ? V : 1, 1);
+  uint32_t c = 0;
+  for (uint32_t i = 0; i < nseeds; ++i)
+    if (seeds[i] < V && !m[seeds[i]]) { m[seeds[i]] = 1; ++c; }
+  *count = c;
+  return m;
+}
+
+/* apply of PPR / TrustRank (personalized_pagerank.hpp:101-120, trustrank.hpp:226-245) */
+static double ppr_apply(uint32_t V, const uint32_t* od, const uint8_t* seed, double restart,
+                        double damping, int final_iter, const double* x, double* y) {
+  double delta = 0.0;
+  for (uint32_t v = 0; v < V; ++v) {
+    double r = (seed[v] ? restart : 0.0) + damping * y[v];
+    if (final_iter) { y[v] = r; continue; }
+    if (od[v] > 0) {
+      r = r / od[v];
+      y[v] = r;
+      delta += fabs(r - x[v]) * od[v];
+    } else {
+      y[v] = r;
+      delta += fabs(r - x[v]);
+    }
+  }
+  return final_iter ? 0.0 : delta;
+}
+
+int orc_ppr(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+            const uint32_t* od, const uint32_t* seeds, uint32_t nseeds,
+            uint32_t iters, double damping, double eps, double* out, double* deltas) {
+  uint32_t ns = 0;
+  uint8_t* seed = seed_map(V, seeds, nseeds, &ns);
+  double* x = (double*)malloc(sizeof(double) * (V ? V : 1));
+  double* y = (double*)malloc(sizeof(double) * (V ? V : 1));
+  if (!seed || !x || !y || ns == 0) { free(seed); free(x); free(y); return -1; }
+  /* seed set: x(v) = [v in S]/|S|/od(v), restart (1-d)/|S| (trustrank.hpp:197-206,
+   * 230,238).  With one seed init_dist is exactly 1.0, which is the single-source
+   * form x(src) = 1/od(src), restart 1-d (personalized_pagerank.hpp:71-81,105). */
+  const double init_dist = 1.0 / ns;
+  for (uint32_t v = 0; v < V; ++v) {
+    x[v] = seed[v] ? init_dist : 0.0;
+    if (od[v] > 0) x[v] = x[v] / od[v];
+  }
+  const double restart = (1.0 - damping) * init_dist;
+  int executed = 0;
+  for (uint32_t t = 0; t < iters; ++t) {
+    const int final_iter = (iters - 1 == t);
+    for (uint32_t v = 0; v < V; ++v) {
+      double sum = 0.0;
+      for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) sum += x[row_idx[e]];
+      y[v] = sum;
+    }
+    const double delta = ppr_apply(V, od, seed, restart, damping, final_iter, x, y);
+    if (deltas) deltas[executed] = delta;
+    ++executed;
+    if (!final_iter && eps > 0.0 && delta < eps) t = iters - 2;
+    double* tmp = x; x = y; y = tmp;
+  }
+  memcpy(out, x, sizeof(double) * V);
+  free(seed); free(x); free(y);
+  return executed;
+}
+
+int orc_ppr_pushpull(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                     const uint64_t* row_ptr, const uint32_t* col_idx,
+                     const uint32_t* od, uint64_t edges_prime, uint32_t src,
+                     uint32_t iters, double damping, double eps,
+                     double* out, double* deltas, uint8_t* modes) {
+  if (src >= V) return -1;
+  uint8_t* seed = (uint8_t*)calloc(V ? V : 1, 1);
+  uint8_t* active = (uint8_t*)calloc(V ? V : 1, 1);
+  double* x = (double*)calloc(V ? V : 1, sizeof(double));
+  double* y = (double*)malloc(sizeof(double) * (V ? V : 1));
+  if (!seed || !active || !x || !y) { free(seed); free(active); free(x); free(y); return -1; }
+  seed[src] = 1;
+  x[src] = 1.0;
+  if (od[src] > 0) x[src] = x[src] / od[src];
+  active[src] = 1;
+  int do_pull = 0, executed = 0;
+  for (uint32_t t = 0; t < iters; ++t) {
+    const int final_iter = (iters - 1 == t);
+    if (!do_pull) {  /* (188-205) */
+      uint64_t ae = 0;
+      for (uint32_t v = 0; v < V; ++v) if (active[v]) ae += od[v];
+      do_pull = (ae > edges_prime / 20u);
+    }
+    if (do_pull) {
+      for (uint32_t v = 0; v < V; ++v) {
+        double sum = 0.0;
+        for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) sum += x[row_idx[e]];
+        y[v] = sum;
+      }
+    } else {  /* push over the accumulated active set (222-239) */
+      memset(y, 0, sizeof(double) * V);
+      /* snapshot: vertices activated in this epoch carry x = 0 and add nothing */
+      uint8_t* was = (uint8_t*)malloc(V ? V : 1);
+      memcpy(was, active, V);
+      for (uint32_t v = 0; v < V; ++v) {
+        if (!was[v]) continue;
+        for (uint64_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
+          const uint32_t w = col_idx[e];
+          active[w] = 1;
+          y[w] += x[v];
+        }
+      }
+      free(was);
+    }
+    if (modes) modes[executed] = (uint8_t)(do_pull ? 0 : 1);
+    const double delta = ppr_apply(V, od, seed, 1.0 - damping, damping, final_iter, x, y);
+    if (deltas) deltas[executed] = delta;
+    ++executed;
+    if (!final_iter && eps > 0.0 && delta < eps) t = iters - 2;
+    double* tmp = x; x = y; y = tmp;
+  }
+  memcpy(out, x, sizeof(double) * V);
+  free(seed); free(active); free(x); free(y);
+  return executed;
+}
+
+/* ---- BFS -------------------------------------------------------------------- */
+
+int orc_bfs(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+            const uint64_t* row_ptr, const uint32_t* col_idx, const uint32_t* od,
+            uint64_t edges_prime, double threshold, uint32_t root,
+            int32_t* levels, uint8_t* modes, uint64_t* frontier, uint64_t* active_edges,
+            uint32_t max_epochs, uint32_t* visited) {
+  if (root >= V) return -1;
+  uint8_t* cur = (uint8_t*)calloc(V ? V : 1, 1);
+  uint8_t* nxt = (uint8_t*)calloc(V ? V : 1, 1);
+  if (!cur || !nxt) { free(cur); free(nxt); return -1; }
+  for (uint32_t v = 0; v < V; ++v) levels[v] = -1;
+  levels[root] = 0;
+  cur[root] = 1;
+  uint64_t nactive = 1, nvisited = 1;
+  int epoch = 0;
+  while (nactive != 0) {
+    uint64_t A = 0;  /* dualmode.hpp:264-276 */
+    for (uint32_t v = 0; v < V; ++v) if (cur[v]) A += od[v];
+    const int push = ((double)A / (double)edges_prime) < threshold;  /* (279) */
+    memset(nxt, 0, V);
+    uint64_t claimed = 0;
+    if (push) {  /* bfs.hpp:98-118 */
+      for (uint32_t v = 0; v < V; ++v) {
+        if (!cur[v]) continue;
+        for (uint64_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
+          const uint32_t w = col_idx[e];
+          if (levels[w] < 0) { levels[w] = epoch + 1; nxt[w] = 1; ++claimed; }
+        }
+      }
+    } else {  /* bfs.hpp:119-136: unvisited columns take their first active in-neighbour */
+      for (uint32_t v = 0; v < V; ++v) {
+        if (levels[v] >= 0) continue;
+        for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
+          if (cur[row_idx[e]]) { levels[v] = epoch + 1; nxt[v] = 1; ++claimed; break; }
+        }
+      }
+    }
+    if ((uint32_t)epoch < max_epochs) {
+      if (modes) modes[epoch] = (uint8_t)push;
+      if (frontier) frontier[epoch] = nactive;
+      if (active_edges) active_edges[epoch] = A;
+    }
+    nvisited += claimed;
+    nactive = claimed;
+    uint8_t* t = cur; cur = nxt; nxt = t;
+    ++epoch;
+  }
+  if (visited) *visited = (uint32_t)nvisited;
+  free(cur); free(nxt);
+  return epoch;
+}
+
+/* ---- top-k ----------------------------------------------------------------- */
+
+static const double* g_rank;
+static int topk_cmp(const void* a, const void* b) {
+  const uint32_t i = *(const uint32_t*)a, j = *(const uint32_t*)b;
+  if (g_rank[i] > g_rank[j]) return -1;
+  if (g_rank[i] < g_rank[j]) return 1;
+  return (i < j) ? -1 : (i > j);
+}
+
+void orc_topk(uint32_t V, const double* r, uint32_t k, uint32_t* ids, double* vals) {
+  uint32_t* p = (uint32_t*)malloc(sizeof(uint32_t) * (V ? V : 1));
+  for (uint32_t v = 0; v < V; ++v) p[v] = v;
+  g_rank = r;
+  qsort(p, V, sizeof(uint32_t), topk_cmp);
+  if (k > V) k = V;
+  for (uint32_t i = 0; i < k; ++i) { ids[i] = p[i]; vals[i] = r[p[i]]; }
+  free(p);
+}

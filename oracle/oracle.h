@@ -1,0 +1,129 @@
+/*
+ * oracle.h -- CPU restatement (FP64, plain C) of the reference algorithms on
+ * the dual-mode edge-processing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code.  It is the checker used by tests/, by
+ * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.
+ *
+ * Every function cites the reference file:line it restates.  Each one is
+ * pinned by an independent check in tests/test_oracle.py (reference fixtures,
+ * closed forms, linear solves, library BFS, invariants); the pin for each
+ * function is named next to its declaration.
+ */
+#ifndef PLATO_ORACLE_H
+#define PLATO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- synthetic graph ------------------------------------------------------ */
+
+/* 64-bit mixing function used by the counter-based RMAT generator
+ * (SplitMix64 finalizer, Steele/Lea/Flood 2014).  Pinned: published test
+ * vector in tests/test_oracle.py. */
+uint64_t orc_mix64(uint64_t x);
+
+/* Integer quadrant thresholds floor(p * 2^32) for p = a, a+b, a+b+c. */
+void orc_rmat_thresholds(double a, double b, double c, uint32_t out[3]);
+
+/* Generate edges [e0, e0+n) of the RMAT graph keyed by (seed, edge id).
+ * Pinned: quadrant frequencies / bit marginals (statistical) and the numpy
+ * twin in oracle/rmat.py. */
+void orc_rmat_edges(uint32_t scale, uint64_t seed, const uint32_t thr[3],
+                    uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst);
+
+/* Edge weight in (0, 1] of edge e (24-bit uniform grid). */
+float orc_rmat_weight(uint64_t seed, uint64_t e);
+
+/* ---- structures ----------------------------------------------------------- */
+
+/* In-edge CSC of an edge list: entries are the E input edges, followed (when
+ * symmetrize != 0) by their E reversals, as Plato adds reverse entries for
+ * undirected input (dcsc.hpp:379-384, bcsr.hpp:411-426).  Columns are sorted by
+ * source id, stable in entry order.  col_ptr has V+1 entries; row_idx and
+ * w_out (optional) have nnz = E * (symmetrize ? 2 : 1) entries.
+ * Pinned: reference DCSC fixtures (dcsc.cc:81-191) and message_passing.cc:48-88. */
+int orc_build_csc(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                  const float* w, int symmetrize,
+                  uint64_t* col_ptr, uint32_t* row_idx, float* w_out);
+
+/* Out-edge CSR: the same entries grouped by source, sorted by destination. */
+int orc_build_csr(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                  int symmetrize, uint64_t* row_ptr, uint32_t* col_idx);
+
+/* od(u) = number of in-edge entries whose source is u (structure.hpp:292-338).
+ * Pinned: structure.cc:137-218 known degrees (19/59/19). */
+void orc_out_degrees(uint32_t V, uint64_t nnz, const uint32_t* row_idx, uint32_t* od);
+
+/* ---- rank algorithms (all FP64) ------------------------------------------ */
+
+/* plato::algo::pagerank (pagerank.hpp:60-149).  Writes the returned state (the
+ * undivided rank after the final iteration) to out[V]; deltas[i] (optional,
+ * length iters) receives the delta of the i-th executed iteration (0 for the
+ * final one).  Returns the number of executed iterations.
+ * Pinned: chain closed form, linear solve of the fixed point. */
+int orc_pagerank(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                 const uint32_t* od, uint32_t iters, double damping, double eps,
+                 double* out, double* deltas, int nthreads);
+
+/* One Plato PageRank iteration over columns [v_lo, v_hi): reads x (state =
+ * rank/od), writes y.  Returns the delta contribution of the range.  Used as
+ * the timed CPU baseline and by the property tests. */
+double orc_pagerank_step(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                         const uint32_t* od, double damping, int final_iter,
+                         const double* x, double* y, uint32_t v_lo, uint32_t v_hi,
+                         int nthreads);
+
+/* plato::algo::weighted_pagerank (weighted_pagerank.hpp:60-157); w is aligned
+ * with row_idx.  Pinned: chain (equals unweighted), linear solve. */
+int orc_weighted_pagerank(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                          const float* w, uint32_t iters, double damping, double eps,
+                          double* out, double* deltas);
+
+/* Personalized PageRank with restart mass spread evenly over a seed set:
+ * nseeds == 1 is plato::algo::personalized_pagerank, pull form
+ * (personalized_pagerank.hpp:40-134); nseeds > 1 is the TrustRank restart
+ * (trustrank.hpp:197-250).  Duplicate seeds count once (a seed bitmap).
+ * Pinned: chain closed form, linear solve, linearity in the seed set. */
+int orc_ppr(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+            const uint32_t* od, const uint32_t* seeds, uint32_t nseeds,
+            uint32_t iters, double damping, double eps, double* out, double* deltas);
+
+/* Push/pull PPR (personalized_pagerank.hpp:137-275): sticky switch
+ * active_edges > E'/20 (integer division); push scatters x(v) of the
+ * accumulated active set over out-edges.  modes[i] = 1 push, 0 pull.
+ * Pinned: equals orc_ppr bit for bit (x = 0 outside the active set). */
+int orc_ppr_pushpull(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                     const uint64_t* row_ptr, const uint32_t* col_idx,
+                     const uint32_t* od, uint64_t edges_prime, uint32_t src,
+                     uint32_t iters, double damping, double eps,
+                     double* out, double* deltas, uint8_t* modes);
+
+/* ---- BFS ------------------------------------------------------------------- */
+
+/* plato::algo::breadth_first_search (bfs.hpp:55-155) with the engine switch
+ * (dualmode.hpp:261-279): push iff (double)A/(double)E' < threshold, where A is
+ * the sum of od over the frontier.  Writes levels (root 0, unreached -1), and
+ * per epoch: mode (1 push / 0 pull), frontier size |F|, active edges A.
+ * Returns the number of epochs; *visited gets |visited|.
+ * Pinned: scipy.sparse.csgraph shortest paths, binary tree closed form. */
+int orc_bfs(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+            const uint64_t* row_ptr, const uint32_t* col_idx, const uint32_t* od,
+            uint64_t edges_prime, double threshold, uint32_t root,
+            int32_t* levels, uint8_t* modes, uint64_t* frontier, uint64_t* active_edges,
+            uint32_t max_epochs, uint32_t* visited);
+
+/* ---- top-k ----------------------------------------------------------------- */
+
+/* Top k of r by (rank desc, id asc) (trustrank.hpp:109-112). */
+void orc_topk(uint32_t V, const double* r, uint32_t k, uint32_t* ids, double* vals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,142 @@
+"""ctypes binding of libplato_hip.so (declarations in include/plato_hip.h).
+
+The shared library is built in-tree by ``python -m plato_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libplato_hip.so")
+
+PE_OK = 0
+PE_ERR = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "bad state",
+          5: "out of range", 6: "internal error"}
+PE_F32, PE_F64, PE_U32 = 0, 1, 2
+PE_KT_PULL_TILE, PE_KT_PULL_FIXUP, PE_KT_BFS_PUSH, PE_KT_BFS_PULL, PE_KT_PPR_BLOCK = 0, 1, 2, 3, 4
+PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED = 1, 2, 4
+
+
+class PlatoHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{PE_ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("V", C.c_uint32), ("max_v_i", C.c_uint32), ("vertices", C.c_uint32),
+                ("max_in_degree", C.c_uint32), ("edges", C.c_uint64), ("nnz", C.c_uint64),
+                ("is_directed", C.c_int32), ("has_csr", C.c_int32), ("has_weights", C.c_int32),
+                ("tile_items", C.c_uint32), ("num_tiles", C.c_uint64),
+                ("num_fixup_rows", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
+class ApplyParams(C.Structure):
+    _fields_ = [("damping", C.c_double), ("base", C.c_double), ("seed_mass", C.c_double),
+                ("seeds", C.c_void_p), ("divide_by_od", C.c_int32), ("final_iter", C.c_int32)]
+
+
+class RankOpts(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("damping", C.c_double), ("eps", C.c_double),
+                ("precision", C.c_int32), ("weighted", C.c_int32), ("l1_norm_tol", C.c_double)]
+
+
+class RankStats(C.Structure):
+    _fields_ = [("iterations_run", C.c_uint32), ("last_delta", C.c_double),
+                ("seconds", C.c_double), ("gteps", C.c_double)]
+
+
+class BfsStats(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("visited", C.c_uint32), ("reached_edges", C.c_uint64),
+                ("seconds", C.c_double)]
+
+
+_vp = C.c_void_p
+_u32, _u64, _i32, _dbl = C.c_uint32, C.c_uint64, C.c_int, C.c_double
+_P = C.POINTER
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "pe_ctx_create": [_i32, _P(_vp)],
+    "pe_ctx_destroy": [_vp],
+    "pe_ctx_last_error": [_vp],
+    "pe_ctx_sync": [_vp],
+    "pe_ctx_mem_info": [_vp, _P(_u64), _P(_u64), _P(_u64)],
+    "pe_ctx_timer_start": [_vp],
+    "pe_ctx_timer_stop": [_vp, _P(_dbl)],
+    "pe_ctx_set_kernel_timing": [_vp, _i32],
+    "pe_ctx_kernel_time": [_vp, _i32, _P(_dbl), _P(_u64)],
+    "pe_ctx_num_cus": [_vp, _P(_i32)],
+    "pe_graph_create": [_vp, _vp, _vp, _vp, _u64, _u32, _u32, _P(_vp)],
+    "pe_graph_create_rmat": [_vp, _u32, _u32, _u64, _dbl, _dbl, _dbl, _u32, _P(_vp)],
+    "pe_rmat_edges": [_vp, _u32, _u64, _dbl, _dbl, _dbl, _u64, _u64, _vp, _vp, _vp],
+    "pe_graph_destroy": [_vp],
+    "pe_graph_get_info": [_vp, _P(GraphInfo)],
+    "pe_graph_download_csc": [_vp, _vp, _vp, _vp],
+    "pe_graph_download_csr": [_vp, _vp, _vp],
+    "pe_graph_set_tile": [_vp, _u32],
+    "pe_vec_create": [_vp, _i32, _u64, _P(_vp)],
+    "pe_vec_destroy": [_vp],
+    "pe_vec_fill": [_vp, _dbl],
+    "pe_vec_upload": [_vp, _vp, _u64],
+    "pe_vec_download": [_vp, _vp, _u64],
+    "pe_vec_download_f64": [_vp, _vp, _u64],
+    "pe_vec_set": [_vp, _u64, _dbl],
+    "pe_vec_sum": [_vp, _P(_dbl)],
+    "pe_bitmap_create": [_vp, _u64, _P(_vp)],
+    "pe_bitmap_destroy": [_vp],
+    "pe_bitmap_clear": [_vp],
+    "pe_bitmap_fill": [_vp],
+    "pe_bitmap_set_bit": [_vp, _u64],
+    "pe_bitmap_clr_bit": [_vp, _u64],
+    "pe_bitmap_get_bit": [_vp, _u64, _P(_i32)],
+    "pe_bitmap_count": [_vp, _P(_u64)],
+    "pe_bitmap_msb": [_vp, _P(_u64)],
+    "pe_bitmap_copy_from": [_vp, _vp],
+    "pe_bitmap_upload": [_vp, _vp, _u64],
+    "pe_bitmap_download": [_vp, _vp, _u64],
+    "pe_bitmap_to_vector": [_vp, _vp, _u64, _P(_u64)],
+    "pe_graph_out_degrees": [_vp, _vp],
+    "pe_graph_in_degrees": [_vp, _vp],
+    "pe_active_edges": [_vp, _vp, _P(_u64)],
+    "pe_choose_mode": [_vp, _vp, _dbl, _P(_i32), _P(_u64)],
+    "pe_pull_sum": [_vp, _vp, _vp, _i32],
+    "pe_rank_apply": [_vp, _vp, _vp, _P(ApplyParams), _P(_dbl)],
+    "pe_rank_iterate": [_vp, _vp, _vp, _i32, _P(ApplyParams), _P(_dbl)],
+    "pe_pagerank": [_vp, _P(RankOpts), _vp, _vp, _P(RankStats)],
+    "pe_ppr": [_vp, _vp, _u32, _P(RankOpts), _vp, _vp, _P(RankStats)],
+    "pe_ppr_pushpull": [_vp, _u32, _P(RankOpts), _vp, _vp, _vp, _P(RankStats)],
+    "pe_bfs": [_vp, _u32, _dbl, _vp, _vp, _vp, _vp, _u32, _P(BfsStats)],
+}
+_RESTYPE = {"pe_ctx_last_error": C.c_char_p}
+
+_lib = None
+
+
+def load():
+    """Load libplato_hip.so; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m plato_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, C.c_int)
+    _lib = L
+    return L
+
+
+def check(rc, ctx=None):
+    if rc != PE_OK:
+        msg = ""
+        if ctx:
+            raw = load().pe_ctx_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise PlatoHipError(rc, msg)

@@ -1,0 +1,329 @@
+// Direction-switching BFS on the dual-mode engine.
+//
+// Replaces plato::algo::breadth_first_search (plato/algo/bfs/bfs.hpp:55-155)
+// with the engine's push/pull choice (plato/engine/dualmode.hpp:261-279):
+// push iff (double)A / (double)E' < threshold, A = sum of od over the frontier.
+//
+//  push (top-down): the frontier queue is walked one wave per vertex over its
+//       out-edges; an unvisited neighbour is claimed with an integer atomicMin
+//       on its level (the claim that the reference does with cas on parent,
+//       bfs.hpp:104-116), and appended to the next queue with one
+//       wave-aggregated atomic (ballot + mbcnt prefix).
+//  pull (bottom-up): one lane per vertex, 64 consecutive vertices per wave, so
+//       a wave owns one 64-bit word of the next-frontier bitmap and writes it
+//       with a ballot, no atomics; an unvisited vertex scans its in-edges for
+//       the first neighbour in the current frontier bitmap and stops there
+//       (bfs.hpp:119-128).  Vertices with more than CAP in-edges that found
+//       nothing in their first CAP continue in a second kernel, one wave each.
+// Both steps produce the next frontier as a queue and as a bitmap, and count
+// its size and active edges on the device, so the next decision needs no scan.
+#include <algorithm>
+
+#include "pe_internal.hpp"
+
+namespace pe {
+
+static constexpr uint32_t UNVIS = 0xFFFFFFFFu;
+static constexpr int PULL_CAP = 32;
+
+struct bfs_counters {
+  unsigned long long next_count;
+  unsigned long long next_active;
+  unsigned long long heavy_count;
+  unsigned long long pad;
+};
+
+__device__ __forceinline__ unsigned wave_prefix(uint64_t mask) {
+  // lanes below this one that are set in mask
+  return __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1ull));
+}
+
+__global__ void k_bfs_init(uint32_t* __restrict__ level, uint32_t V, uint32_t root) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    level[v] = (v == root) ? 0u : UNVIS;
+}
+
+__global__ void k_bfs_seed(uint32_t* __restrict__ queue, uint64_t* __restrict__ bitmap, uint32_t root,
+                           bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+  queue[0] = root;
+  bitmap[root >> 6] |= 1ull << (root & 63);
+  cnt->next_count = 1;
+  cnt->next_active = od[root];
+  cnt->heavy_count = 0;
+}
+
+__global__ void k_bfs_reset(bfs_counters* __restrict__ cnt) {
+  cnt->next_count = 0;
+  cnt->next_active = 0;
+  cnt->heavy_count = 0;
+}
+
+__global__ void k_zero_words(uint64_t* __restrict__ w, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    w[i] = 0ull;
+}
+
+// wave-aggregated append of the lanes in `mask`; returns this lane's slot
+__device__ __forceinline__ unsigned long long wave_append(uint64_t mask,
+                                                          unsigned long long* counter) {
+  unsigned long long base = 0;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  return base + wave_prefix(mask);
+}
+
+__global__ __launch_bounds__(256) void k_bfs_push(
+    const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
+    const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
+    uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
+    bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long active = 0;
+  for (uint64_t q = wave_id; q < n; q += nwaves) {
+    const uint32_t v = queue[q];
+    const uint64_t b = ptr[v], e = ptr[v + 1];
+    for (uint64_t base = b; base < e; base += 64) {
+      const uint64_t i = base + lane;
+      bool claimed = false;
+      uint32_t w = 0;
+      if (i < e) {
+        w = idx[i];
+        if (level[w] == UNVIS) claimed = (atomicMin(&level[w], L) == UNVIS);
+      }
+      const uint64_t m = __ballot(claimed);
+      if (m) {
+        const unsigned long long pos = wave_append(m, &cnt->next_count);
+        if (claimed) {
+          next_queue[pos] = w;
+          atomicOr((unsigned long long*)&next_bits[w >> 6], 1ull << (w & 63));
+          active += od[w];
+        }
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
+  if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
+}
+
+__global__ __launch_bounds__(256) void k_bfs_pull(
+    const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
+    uint64_t nwords, uint32_t* __restrict__ level, const uint64_t* __restrict__ cur_bits, uint32_t L,
+    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ next_queue,
+    uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long active = 0;
+  for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
+       word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint64_t v = word * 64 + lane;
+    bool claimed = false, is_heavy = false;
+    if (v < V && level[v] == UNVIS) {
+      const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+      const uint64_t lim = (e - b > PULL_CAP) ? b + PULL_CAP : e;
+      for (uint64_t i = b; i < lim; ++i) {
+        const uint32_t u = row_idx[i];
+        if ((cur_bits[u >> 6] >> (u & 63)) & 1ull) { claimed = true; break; }
+      }
+      is_heavy = !claimed && e > lim;
+    }
+    const uint64_t m = __ballot(claimed);
+    if (lane == 0) next_bits[word] = m;
+    if (m) {
+      const unsigned long long pos = wave_append(m, &cnt->next_count);
+      if (claimed) {
+        level[v] = L;
+        next_queue[pos] = (uint32_t)v;
+        active += od[v];
+      }
+    }
+    const uint64_t hm = __ballot(is_heavy);
+    if (hm) {
+      const unsigned long long pos = wave_append(hm, &cnt->heavy_count);
+      if (is_heavy) heavy[pos] = (uint32_t)v;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
+  if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
+}
+
+// second pull pass: one wave per heavy vertex, 64 in-edges per step, stop at the first hit
+__global__ __launch_bounds__(256) void k_bfs_pull_heavy(
+    const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
+    uint32_t* __restrict__ level, const uint64_t* __restrict__ cur_bits, uint32_t L,
+    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ next_queue,
+    const uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt,
+    const uint32_t* __restrict__ od) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const unsigned long long n = cnt->heavy_count;
+  unsigned long long active = 0;
+  for (uint64_t h = wave_id; h < n; h += nwaves) {
+    const uint32_t v = heavy[h];
+    const uint64_t b = col_ptr[v] + PULL_CAP, e = col_ptr[v + 1];
+    bool found = false;
+    for (uint64_t base = b; base < e && !found; base += 64) {
+      const uint64_t i = base + lane;
+      bool hit = false;
+      if (i < e) {
+        const uint32_t u = row_idx[i];
+        hit = (cur_bits[u >> 6] >> (u & 63)) & 1ull;
+      }
+      found = __ballot(hit) != 0ull;
+    }
+    if (found && lane == 0) {
+      level[v] = L;
+      atomicOr((unsigned long long*)&next_bits[v >> 6], 1ull << (v & 63));
+      const unsigned long long pos = atomicAdd(&cnt->next_count, 1ull);
+      next_queue[pos] = v;
+      active += od[v];
+    }
+  }
+  if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
+}
+
+__global__ void k_reached_edges(const uint32_t* __restrict__ level, const uint32_t* __restrict__ od,
+                                uint32_t V, unsigned long long* __restrict__ out,
+                                unsigned int* __restrict__ visited) {
+  unsigned long long acc = 0;
+  unsigned int cntv = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    if (level[v] != UNVIS) { acc += od[v]; ++cntv; }
+  for (int off = 32; off > 0; off >>= 1) {
+    acc += __shfl_down(acc, off, 64);
+    cntv += __shfl_down(cntv, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (acc) atomicAdd(out, acc);
+    if (cntv) atomicAdd(visited, cntv);
+  }
+}
+
+// Device state of one BFS engine, reused across roots.
+struct bfs_state {
+  pe_graph_s* g;
+  uint64_t nwords;
+  dbuf<uint32_t> level, q0, q1, heavy;
+  dbuf<uint64_t> b0, b1;
+  dbuf<bfs_counters> cnt;
+  dbuf<unsigned long long> red;
+  bfs_counters* hcnt;  // pinned host mirror
+
+  explicit bfs_state(pe_graph_s* graph) : g(graph) {
+    pe_ctx_s* ctx = g->ctx;
+    const uint32_t V = std::max<uint32_t>(1, g->V);
+    nwords = ((uint64_t)g->V >> 6) + 1;
+    level.alloc(ctx, V);
+    q0.alloc(ctx, V);
+    q1.alloc(ctx, V);
+    heavy.alloc(ctx, V);
+    b0.alloc(ctx, nwords);
+    b1.alloc(ctx, nwords);
+    cnt.alloc(ctx, 1);
+    red.alloc(ctx, 2);
+    hcnt = static_cast<bfs_counters*>(ctx->host_pinned);
+  }
+
+  int run(uint32_t root, double threshold, uint8_t* modes, uint64_t* frontier, uint64_t* actv,
+          uint32_t max_epochs, double* seconds) {
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const uint64_t Ep = g->directed ? g->edges : g->edges * 2;  // dualmode.hpp:261-262
+    const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
+    const unsigned pull_grid = grid_for(nwords * 64, 256, 1u << 20);
+    const unsigned heavy_grid = persistent_grid(ctx, 1u << 20, 4);
+    PE_HIP(hipEventRecord(ctx->t0, s));
+    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, g->V, root);
+    k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(b0.p, nwords);
+    k_bfs_seed<<<1, 1, 0, s>>>(q0.p, b0.p, root, cnt.p, g->od.p);
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipMemcpyAsync(hcnt, cnt.p, sizeof(bfs_counters), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    uint64_t n = hcnt->next_count, A = hcnt->next_active;
+    uint32_t* qc = q0.p;
+    uint32_t* qn = q1.p;
+    uint64_t* bc = b0.p;
+    uint64_t* bn = b1.p;
+    int epoch = 0;
+    while (n != 0) {
+      const bool push = ((double)A / (double)Ep) < threshold;  // dualmode.hpp:279
+      if ((uint32_t)epoch < max_epochs) {
+        if (modes) modes[epoch] = push ? 1 : 0;
+        if (frontier) frontier[epoch] = n;
+        if (actv) actv[epoch] = A;
+      }
+      const uint32_t L = (uint32_t)epoch + 1;
+      k_bfs_reset<<<1, 1, 0, s>>>(cnt.p);
+      if (push) {
+        PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
+        k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(bn, nwords);
+        ktime_scope kt(ctx, PE_KT_BFS_PUSH);
+        const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
+        k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
+                                        g->od.p);
+      } else {
+        ktime_scope kt(ctx, PE_KT_BFS_PULL);
+        k_bfs_pull<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,
+                                             qn, heavy.p, cnt.p, g->od.p);
+        k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, qn,
+                                                    heavy.p, cnt.p, g->od.p);
+      }
+      PE_HIP(hipGetLastError());
+      PE_HIP(hipMemcpyAsync(hcnt, cnt.p, sizeof(bfs_counters), hipMemcpyDeviceToHost, s));
+      PE_HIP(hipStreamSynchronize(s));
+      n = hcnt->next_count;
+      A = hcnt->next_active;
+      std::swap(qc, qn);
+      std::swap(bc, bn);
+      ++epoch;
+      PE_REQUIRE(epoch <= (int)g->V + 1, PE_ERR_INTERNAL, "BFS did not terminate");
+    }
+    PE_HIP(hipEventRecord(ctx->t1, s));
+    PE_HIP(hipEventSynchronize(ctx->t1));
+    float ms = 0.f;
+    PE_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+    if (seconds) *seconds = ms * 1e-3;
+    return epoch;
+  }
+};
+
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" {
+
+int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint8_t* modes,
+           uint64_t* frontier, uint64_t* active_edges, uint32_t max_epochs, pe_bfs_stats_t* stats) {
+  if (!g) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(root < g->V, PE_ERR_RANGE, "root >= V");
+    bfs_state st(g);
+    double secs = 0.0;
+    const int epochs = st.run(root, threshold, modes, frontier, active_edges, max_epochs, &secs);
+    hipStream_t s = g->ctx->stream;
+    if (levels)
+      PE_HIP(hipMemcpyAsync(levels, st.level.p, (uint64_t)g->V * 4, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipMemsetAsync(st.red.p, 0, 16, s));
+    k_reached_edges<<<grid_for(g->V, 256, 4096), 256, 0, s>>>(
+        st.level.p, g->od.p, g->V, st.red.p, reinterpret_cast<unsigned int*>(st.red.p + 1));
+    PE_HIP(hipGetLastError());
+    unsigned long long r[2] = {0, 0};
+    PE_HIP(hipMemcpyAsync(r, st.red.p, 16, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      stats->epochs = (uint32_t)epochs;
+      stats->visited = (uint32_t)(r[1] & 0xFFFFFFFFull);
+      stats->reached_edges = r[0];
+      stats->seconds = secs;
+    }
+  });
+}
+
+}  // extern "C"

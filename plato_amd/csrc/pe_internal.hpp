@@ -1,0 +1,213 @@
+// Internal declarations shared by the HIP translation units of libplato_hip.so.
+// Not part of the C-ABI (include/plato_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "plato_hip.h"
+
+namespace pe {
+
+// ---- errors ------------------------------------------------------------------
+// C-ABI functions never throw: internal code throws pe::error, the boundary
+// catches it and turns it into a status code plus the context's last-error text.
+struct error : std::runtime_error {
+  int code;
+  error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess) {
+    int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PE_ERR_OOM : PE_ERR_HIP;
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    throw error(code, buf);
+  }
+}
+#define PE_HIP(x) ::pe::hip_check((x), #x, __FILE__, __LINE__)
+#define PE_REQUIRE(cond, code, msg) \
+  do { if (!(cond)) ::pe::fail((code), (msg)); } while (0)
+
+// ---- context -------------------------------------------------------------------
+struct kernel_timer {
+  std::vector<hipEvent_t> begin, end;  // recorded pairs, drained by collect()
+  double total_ms = 0.0;
+  uint64_t launches = 0;
+};
+
+}  // namespace pe
+
+struct pe_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  uint64_t allocated = 0;       // bytes currently held by this context
+  uint64_t peak_allocated = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  int timing = 0;               // per-kernel event timing on/off
+  pe::kernel_timer timers[PE_KT_COUNT];
+  int num_cus = 0;
+  // device scalars used by reductions; pinned host mirror for cheap readback
+  void* host_pinned = nullptr;  // 4 KiB
+};
+
+namespace pe {
+
+// Allocation through the context keeps a ledger (admission control reads it).
+void* dmalloc(pe_ctx_s* ctx, size_t bytes);
+void dfree(pe_ctx_s* ctx, void* p, size_t bytes);
+
+template <typename T>
+struct dbuf {
+  pe_ctx_s* ctx = nullptr;
+  T* p = nullptr;
+  size_t n = 0;
+  dbuf() = default;
+  dbuf(pe_ctx_s* c, size_t count) { alloc(c, count); }
+  void alloc(pe_ctx_s* c, size_t count) {
+    release();
+    ctx = c;
+    n = count;
+    p = count ? static_cast<T*>(dmalloc(c, count * sizeof(T))) : nullptr;
+  }
+  void release() {
+    if (p) dfree(ctx, p, n * sizeof(T));
+    p = nullptr;
+    n = 0;
+  }
+  ~dbuf() { release(); }
+  dbuf(const dbuf&) = delete;
+  dbuf& operator=(const dbuf&) = delete;
+  dbuf(dbuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  dbuf& operator=(dbuf&& o) noexcept {
+    if (this != &o) { release(); ctx = o.ctx; p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Kernel timing: brackets a launch with events on the context stream when
+// timing is enabled.  Used for the per-kernel average launch duration.
+struct ktime_scope {
+  pe_ctx_s* ctx;
+  int which;
+  hipEvent_t e1 = nullptr;
+  ktime_scope(pe_ctx_s* c, int w);
+  ~ktime_scope();
+};
+
+inline unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap = 1u << 30) {
+  uint64_t g = (work + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// Persistent grid: enough resident blocks to cover every CU several times.
+inline unsigned persistent_grid(pe_ctx_s* ctx, uint64_t tiles, unsigned blocks_per_cu) {
+  uint64_t cap = static_cast<uint64_t>(ctx->num_cus > 0 ? ctx->num_cus : 256) * blocks_per_cu;
+  if (tiles < cap) cap = tiles;
+  return cap ? static_cast<unsigned>(cap) : 1u;
+}
+
+}  // namespace pe
+
+// ---- graph ----------------------------------------------------------------------
+struct pe_graph_s {
+  pe_ctx_s* ctx = nullptr;
+  uint32_t V = 0;           // vertex id space [0, V)
+  uint64_t edges = 0;       // input edges (Plato graph_info_t::edges_)
+  uint64_t nnz = 0;         // CSC entries
+  bool directed = true;
+  bool has_weights = false;
+  bool has_csr = false;
+  bool csr_is_csc = false;  // symmetrized graphs share one structure
+  uint32_t max_v_i = 0;
+  uint32_t vertices = 0;
+  uint32_t max_in_degree = 0;
+
+  pe::dbuf<uint64_t> col_ptr;   // [V+1]
+  pe::dbuf<uint32_t> row_idx;   // [nnz], ascending source id inside each column
+  pe::dbuf<float> wn;           // [nnz] weight / sum of the source's out-weights
+  pe::dbuf<uint32_t> od;        // [V] out-degree counted from the in-edges
+  pe::dbuf<uint64_t> row_ptr;   // [V+1] CSR (out-edges)
+  pe::dbuf<uint32_t> col_idx;   // [nnz]
+
+  // merge-path tiling of the pull gather (rows + edges, fixed items per tile)
+  uint32_t tile_items = 0;
+  uint64_t num_tiles = 0;
+  pe::dbuf<uint32_t> tile_row;   // [T+1] first row of each tile
+  pe::dbuf<uint64_t> tile_edge;  // [T+1] first edge of each tile
+  // rows whose edges cross a tile boundary: finished by the fix-up kernel
+  uint64_t num_fix = 0;
+  pe::dbuf<uint32_t> fix_row;    // [F]
+  pe::dbuf<uint32_t> fix_t0;     // [F] first tile holding edges of the row
+  pe::dbuf<uint32_t> fix_t1;     // [F] tile holding the row's end
+  uint64_t num_fix_long = 0;     // fix rows [num_fix - num_fix_long, num_fix) span > 8 tiles
+
+  const uint64_t* csr_ptr() const { return csr_is_csc ? col_ptr.p : row_ptr.p; }
+  const uint32_t* csr_idx() const { return csr_is_csc ? row_idx.p : col_idx.p; }
+  uint64_t device_bytes() const {
+    return col_ptr.bytes() + row_idx.bytes() + wn.bytes() + od.bytes() + row_ptr.bytes() +
+           col_idx.bytes() + tile_row.bytes() + tile_edge.bytes() + fix_row.bytes() +
+           fix_t0.bytes() + fix_t1.bytes();
+  }
+};
+
+namespace pe {
+// graph.hip
+void graph_build_tiles(pe_graph_s* g, uint32_t tile_items);
+// vectors
+struct vec_view {
+  int dtype;
+  uint64_t n;
+  void* p;
+};
+}  // namespace pe
+
+struct pe_vec_s {
+  pe_ctx_s* ctx = nullptr;
+  int dtype = PE_F32;
+  uint64_t n = 0;
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct pe_bitmap_s {
+  pe_ctx_s* ctx = nullptr;
+  uint64_t bits = 0;
+  uint64_t words = 0;  // (bits >> 6) + 1, as bitmap.hpp:43-108
+  uint64_t* p = nullptr;
+};
+
+// ---- C-ABI boundary helper --------------------------------------------------------
+namespace pe {
+template <typename F>
+int guarded(pe_ctx_s* ctx, F&& f) {
+  try {
+    f();
+    return PE_OK;
+  } catch (const error& e) {
+    if (ctx) ctx->last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->last_error = "host allocation failed";
+    return PE_ERR_OOM;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->last_error = e.what();
+    return PE_ERR_INTERNAL;
+  } catch (...) {
+    if (ctx) ctx->last_error = "unknown exception";
+    return PE_ERR_INTERNAL;
+  }
+}
+}  // namespace pe

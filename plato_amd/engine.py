@@ -1,0 +1,366 @@
+"""Python host API over the C-ABI: context, graph, engine state and the
+dual-mode engine operators.
+
+Names follow the reference's C++ API (plato/engine/dualmode.hpp,
+plato/graph/state/dense_state.hpp, plato/util/bitmap.hpp) so tests read like
+the reference's own; every call goes to libplato_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Context:
+    """One GPU, one stream, an allocation ledger (cluster_info_t for one device)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.pe_ctx_create(device, C.byref(h))
+        if rc != L.PE_OK:
+            raise L.PlatoHipError(rc, "pe_ctx_create failed (no GPU?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pe_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(self.lib.pe_ctx_sync(self.h), self.h)
+
+    def mem_info(self):
+        f, t, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(self.lib.pe_ctx_mem_info(self.h, C.byref(f), C.byref(t), C.byref(c)), self.h)
+        return dict(free=f.value, total=t.value, ctx=c.value)
+
+    def num_cus(self):
+        n = C.c_int()
+        check(self.lib.pe_ctx_num_cus(self.h, C.byref(n)), self.h)
+        return n.value
+
+    def set_kernel_timing(self, on: bool):
+        check(self.lib.pe_ctx_set_kernel_timing(self.h, int(on)), self.h)
+
+    def kernel_time(self, which: int):
+        ms, n = C.c_double(), C.c_uint64()
+        check(self.lib.pe_ctx_kernel_time(self.h, which, C.byref(ms), C.byref(n)), self.h)
+        return ms.value, n.value
+
+    def timer_start(self):
+        check(self.lib.pe_ctx_timer_start(self.h), self.h)
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        check(self.lib.pe_ctx_timer_stop(self.h, C.byref(ms)), self.h)
+        return ms.value
+
+    def rmat_edges(self, scale, seed, e0, n, a=0.57, b=0.19, c=0.19, weights=False):
+        src = np.zeros(n, np.uint32)
+        dst = np.zeros(n, np.uint32)
+        w = np.zeros(n, np.float32) if weights else None
+        check(self.lib.pe_rmat_edges(self.h, scale, seed, a, b, c, e0, n, _ptr(src), _ptr(dst),
+                                     _ptr(w)), self.h)
+        return (src, dst, w) if weights else (src, dst)
+
+
+@dataclass
+class graph_info_t:
+    """plato/graph/base.hpp:241-257"""
+    is_directed_: bool
+    vertices_: int
+    edges_: int
+    max_v_i_: int
+
+
+class Graph:
+    """Device CSC (+ optional CSR) of one graph (dcsc_t / bcsr_t for one GPU)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = handle
+        self.info = L.GraphInfo()
+        check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), ctx.h)
+
+    @classmethod
+    def from_edges(cls, ctx, src, dst, V, weights=None, directed=True, build_csr=False):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        flags = 0 if directed else L.PE_GRAPH_SYMMETRIZE
+        if build_csr:
+            flags |= L.PE_GRAPH_BUILD_CSR
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, np.float32)
+            flags |= L.PE_GRAPH_WEIGHTED
+        h = C.c_void_p()
+        check(ctx.lib.pe_graph_create(ctx.h, _ptr(src), _ptr(dst), _ptr(w), src.size, V, flags,
+                                      C.byref(h)), ctx.h)
+        return cls(ctx, h)
+
+    @classmethod
+    def rmat(cls, ctx, scale, edge_factor=16, seed=1, a=0.57, b=0.19, c=0.19, directed=True,
+             build_csr=False, weighted=False):
+        flags = 0 if directed else L.PE_GRAPH_SYMMETRIZE
+        if build_csr:
+            flags |= L.PE_GRAPH_BUILD_CSR
+        if weighted:
+            flags |= L.PE_GRAPH_WEIGHTED
+        h = C.c_void_p()
+        check(ctx.lib.pe_graph_create_rmat(ctx.h, scale, edge_factor, seed, a, b, c, flags,
+                                           C.byref(h)), ctx.h)
+        return cls(ctx, h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pe_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def V(self):
+        return self.info.V
+
+    @property
+    def nnz(self):
+        return self.info.nnz
+
+    @property
+    def edges_prime(self):
+        """E' of dualmode.hpp:261-262"""
+        return self.info.edges if self.info.is_directed else 2 * self.info.edges
+
+    def graph_info(self) -> graph_info_t:
+        return graph_info_t(bool(self.info.is_directed), self.info.vertices, self.info.edges,
+                            self.info.max_v_i)
+
+    def set_tile(self, items):
+        check(self.lib.pe_graph_set_tile(self.h, items), self.ctx.h)
+        check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
+
+    def download_csc(self, weights=False):
+        cp = np.zeros(self.V + 1, np.uint64)
+        ri = np.zeros(max(self.nnz, 1), np.uint32)
+        wn = np.zeros(max(self.nnz, 1), np.float32) if weights else None
+        check(self.lib.pe_graph_download_csc(self.h, _ptr(cp), _ptr(ri), _ptr(wn)), self.ctx.h)
+        ri = ri[: self.nnz]
+        return (cp, ri, wn[: self.nnz]) if weights else (cp, ri)
+
+    def download_csr(self):
+        rp = np.zeros(self.V + 1, np.uint64)
+        ci = np.zeros(max(self.nnz, 1), np.uint32)
+        check(self.lib.pe_graph_download_csr(self.h, _ptr(rp), _ptr(ci)), self.ctx.h)
+        return rp, ci[: self.nnz]
+
+    def out_degrees(self):
+        od = np.zeros(self.V, np.uint32)
+        check(self.lib.pe_graph_out_degrees(self.h, _ptr(od)), self.ctx.h)
+        return od
+
+    def in_degrees(self):
+        d = np.zeros(self.V, np.uint32)
+        check(self.lib.pe_graph_in_degrees(self.h, _ptr(d)), self.ctx.h)
+        return d
+
+
+class DenseState:
+    """dense_state_t on the device (plato/graph/state/dense_state.hpp:43-175)."""
+
+    _DT = {np.float32: L.PE_F32, np.float64: L.PE_F64, np.uint32: L.PE_U32}
+
+    def __init__(self, ctx: Context, n: int, dtype=np.float32):
+        self.ctx, self.lib, self.n = ctx, ctx.lib, int(n)
+        self.dtype = np.dtype(dtype).type
+        h = C.c_void_p()
+        check(self.lib.pe_vec_create(ctx.h, self._DT[self.dtype], self.n, C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pe_vec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fill(self, value):
+        check(self.lib.pe_vec_fill(self.h, float(value)), self.ctx.h)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        check(self.lib.pe_vec_upload(self.h, _ptr(arr), arr.size), self.ctx.h)
+
+    def numpy(self):
+        out = np.zeros(self.n, self.dtype)
+        check(self.lib.pe_vec_download(self.h, _ptr(out), self.n), self.ctx.h)
+        return out
+
+    def to_f64(self):
+        out = np.zeros(self.n, np.float64)
+        check(self.lib.pe_vec_download_f64(self.h, _ptr(out), self.n), self.ctx.h)
+        return out
+
+    def __setitem__(self, i, value):
+        check(self.lib.pe_vec_set(self.h, int(i), float(value)), self.ctx.h)
+
+    def sum(self):
+        s = C.c_double()
+        check(self.lib.pe_vec_sum(self.h, C.byref(s)), self.ctx.h)
+        return s.value
+
+
+class Bitmap:
+    """bitmap_t on the device (plato/util/bitmap.hpp:43-350), same word layout."""
+
+    def __init__(self, ctx: Context, size: int):
+        self.ctx, self.lib, self.size = ctx, ctx.lib, int(size)
+        self.words = (self.size >> 6) + 1
+        h = C.c_void_p()
+        check(self.lib.pe_bitmap_create(ctx.h, self.size, C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pe_bitmap_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        check(self.lib.pe_bitmap_clear(self.h), self.ctx.h)
+
+    def fill(self):
+        check(self.lib.pe_bitmap_fill(self.h), self.ctx.h)
+
+    def set_bit(self, i):
+        check(self.lib.pe_bitmap_set_bit(self.h, int(i)), self.ctx.h)
+
+    def clr_bit(self, i):
+        check(self.lib.pe_bitmap_clr_bit(self.h, int(i)), self.ctx.h)
+
+    def get_bit(self, i):
+        o = C.c_int()
+        check(self.lib.pe_bitmap_get_bit(self.h, int(i), C.byref(o)), self.ctx.h)
+        return o.value
+
+    def count(self):
+        o = C.c_uint64()
+        check(self.lib.pe_bitmap_count(self.h, C.byref(o)), self.ctx.h)
+        return o.value
+
+    def msb(self):
+        o = C.c_uint64()
+        check(self.lib.pe_bitmap_msb(self.h, C.byref(o)), self.ctx.h)
+        return o.value
+
+    def copy_from(self, other):
+        check(self.lib.pe_bitmap_copy_from(self.h, other.h), self.ctx.h)
+
+    def words_numpy(self):
+        w = np.zeros(self.words, np.uint64)
+        check(self.lib.pe_bitmap_download(self.h, _ptr(w), self.words), self.ctx.h)
+        return w
+
+    def upload_words(self, w):
+        w = np.ascontiguousarray(w, np.uint64)
+        check(self.lib.pe_bitmap_upload(self.h, _ptr(w), w.size), self.ctx.h)
+
+    def to_vector(self):
+        n = C.c_uint64()
+        check(self.lib.pe_bitmap_to_vector(self.h, None, 0, C.byref(n)), self.ctx.h)
+        ids = np.zeros(max(n.value, 1), np.uint32)
+        check(self.lib.pe_bitmap_to_vector(self.h, _ptr(ids), n.value, C.byref(n)), self.ctx.h)
+        return ids[: n.value]
+
+
+class dualmode_engine_t:
+    """The engine of plato/engine/dualmode.hpp:75-166 over one device graph.
+
+    The reference takes host lambdas as signal/slot functors; here the operator
+    set is closed and chosen by method: pull gather-sum (aggregate_message with
+    the write_add slot), restart-apply, fused iterate, and the push/pull choice.
+    """
+
+    def __init__(self, graph: Graph, push_threshold: float = 0.05):
+        self.graph = graph
+        self.ctx = graph.ctx
+        self.lib = graph.lib
+        self.push_threshold_ = push_threshold  # dualmode_engine_opts_t (dualmode.hpp:41-43)
+
+    def alloc_v_state(self, dtype=np.float32) -> DenseState:
+        return DenseState(self.ctx, self.graph.V, dtype)
+
+    def alloc_v_subset(self) -> Bitmap:
+        return Bitmap(self.ctx, self.graph.V)
+
+    def active_edges(self, actives: Bitmap) -> int:
+        o = C.c_uint64()
+        check(self.lib.pe_active_edges(self.graph.h, actives.h, C.byref(o)), self.ctx.h)
+        return o.value
+
+    def choose_push(self, actives: Bitmap):
+        p, a = C.c_int(), C.c_uint64()
+        check(self.lib.pe_choose_mode(self.graph.h, actives.h, self.push_threshold_, C.byref(p),
+                                      C.byref(a)), self.ctx.h)
+        return bool(p.value), a.value
+
+    def pull_sum(self, x: DenseState, y: DenseState, weighted=False):
+        check(self.lib.pe_pull_sum(self.graph.h, x.h, y.h, int(weighted)), self.ctx.h)
+
+    @staticmethod
+    def _params(damping, base, seed_mass, seeds, divide_by_od, final_iter):
+        p = L.ApplyParams()
+        p.damping, p.base, p.seed_mass = damping, base, seed_mass
+        p.seeds = seeds.h if seeds is not None else None
+        p.divide_by_od, p.final_iter = int(divide_by_od), int(final_iter)
+        return p
+
+    def rank_apply(self, x, y, damping, base, seed_mass=0.0, seeds=None, divide_by_od=True,
+                   final_iter=False):
+        p = self._params(damping, base, seed_mass, seeds, divide_by_od, final_iter)
+        d = C.c_double()
+        check(self.lib.pe_rank_apply(self.graph.h, x.h, y.h, C.byref(p), C.byref(d)), self.ctx.h)
+        return d.value
+
+    def rank_iterate(self, x, y, damping, base, seed_mass=0.0, seeds=None, divide_by_od=True,
+                     final_iter=False, weighted=False):
+        p = self._params(damping, base, seed_mass, seeds, divide_by_od, final_iter)
+        d = C.c_double()
+        check(self.lib.pe_rank_iterate(self.graph.h, x.h, y.h, int(weighted), C.byref(p),
+                                       C.byref(d)), self.ctx.h)
+        return d.value

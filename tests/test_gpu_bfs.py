@@ -1,0 +1,73 @@
+"""GPU: direction-switching BFS.  Levels are bit-exact to the oracle (integer
+work), and so are the per-epoch push/pull decisions, frontier sizes and active
+edge counts."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_bfs_rmat_bit_exact(ctx, directed):
+    from plato_amd import Graph
+    from plato_amd.algo import bfs_opts_t, breadth_first_search
+    scale = 13
+    print(f"case: bfs rmat s{scale} directed={directed}", flush=True)
+    g = Graph.rmat(ctx, scale, 16, seed=77, directed=directed, build_csr=True)
+    s, d = oracle.rmat_edges(scale, 16, 77)
+    og = oracle.CSC(1 << scale, s, d, symmetrize=not directed)
+    rng = np.random.default_rng(5)
+    cand = np.flatnonzero(og.od > 0)
+    roots = list(rng.choice(cand, 6, replace=False)) + [int(np.argmax(og.od))]
+    for root in roots:
+        ref = oracle.bfs(og, int(root))
+        res = breadth_first_search(g, bfs_opts_t(int(root)))
+        assert np.array_equal(res.levels, ref["levels"]), f"root {root}"
+        assert res.epochs == ref["epochs"]
+        assert np.array_equal(res.modes, ref["modes"])
+        assert np.array_equal(res.frontier, ref["frontier"])
+        assert np.array_equal(res.active_edges, ref["active_edges"])
+        assert res.visited == ref["visited"]
+        assert res.reached_edges == int(og.od[ref["levels"] >= 0].sum())
+
+
+def test_bfs_modes_cover_push_and_pull(ctx):
+    """A low-degree root on a symmetrized RMAT goes push -> pull -> push."""
+    from plato_amd import Graph
+    from plato_amd.algo import bfs_opts_t, breadth_first_search
+    g = Graph.rmat(ctx, 14, 16, seed=2, directed=False)
+    s, d = oracle.rmat_edges(14, 16, 2)
+    og = oracle.CSC(1 << 14, s, d, symmetrize=True)
+    root = int(np.flatnonzero(og.od == 1)[0])
+    ref = oracle.bfs(og, root)
+    res = breadth_first_search(g, bfs_opts_t(root))
+    assert np.array_equal(res.levels, ref["levels"])
+    m = res.modes.tolist()
+    assert 1 in m and 0 in m and m[0] == 1
+
+
+def test_bfs_fixtures(ctx):
+    from plato_amd import Graph
+    from plato_amd.algo import bfs_opts_t, breadth_first_search
+    s, d = oracle.read_edge_csv(golden("complete_binary_tree_10_9.csv"))
+    g = Graph.from_edges(ctx, s, d, 10, build_csr=True)
+    res = breadth_first_search(g, bfs_opts_t(0))
+    assert np.array_equal(res.levels, np.floor(np.log2(np.arange(10) + 1)).astype(np.int32))
+    assert res.visited == 10
+    s, d, _ = oracle.read_edge_csv(golden("graph_10_9.csv"), weighted=True)
+    g = Graph.from_edges(ctx, s, d, 10, build_csr=True)
+    res = breadth_first_search(g, bfs_opts_t(4))
+    assert res.levels.tolist() == [-1] * 4 + list(range(6)) and res.visited == 6
+
+
+def test_bfs_isolated_root(ctx):
+    from plato_amd import Graph
+    from plato_amd.algo import bfs_opts_t, breadth_first_search
+    g = Graph.from_edges(ctx, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32), 8,
+                         build_csr=True)
+    res = breadth_first_search(g, bfs_opts_t(5))
+    assert res.visited == 1 and res.epochs == 1 and res.levels[5] == 0
+    assert (np.delete(res.levels, 5) == -1).all()

@@ -1,0 +1,265 @@
+"""Pins the CPU oracle (oracle/) before anything is compared against it.
+
+Each test checks an oracle function against something that does not come from
+the oracle's own formula: the reference tests' known answers on the reference's
+data files, closed forms, dense linear solves of the fixed point, scipy's BFS,
+published hash vectors, or algebraic invariants.  CPU only.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+
+import oracle
+import oracle.rmat as R
+from conftest import golden
+
+D = 0.85
+
+
+# ---- generator ------------------------------------------------------------------
+
+def test_mix64_published_vector():
+    # SplitMix64 from state 0: first outputs (Vigna's reference implementation)
+    assert oracle.mix64(0) == 0xE220A8397B1DCDAF
+    assert oracle.mix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+    assert oracle.mix64(2 * 0x9E3779B97F4A7C15 % 2**64) == 0x06C45D188009454F
+
+
+def test_rmat_c_matches_numpy_twin():
+    s, d = oracle.rmat_edges(12, 16, 42)
+    s2, d2 = R.edges(12, 16, 42)
+    assert np.array_equal(s, s2) and np.array_equal(d, d2)
+    # an offset window equals the same slice of the full list
+    s3, d3 = oracle.rmat_edges(12, 16, 42, e0=1000, n=500)
+    assert np.array_equal(s3, s[1000:1500]) and np.array_equal(d3, d[1000:1500])
+    w = oracle.rmat_weights(42, 300)
+    assert np.array_equal(w, R.weights(42, 0, 300))
+    assert w.min() > 0 and w.max() <= 1
+
+
+def test_rmat_bit_marginals():
+    # each level: P(src bit) = c + d = 0.24, P(dst bit) = b + d = 0.24, P(both) = d = 0.05
+    scale = 14
+    s, d = oracle.rmat_edges(scale, 16, 7)
+    n = s.size
+    for bit in (0, 5, scale - 1):
+        sb = (s >> bit) & 1
+        db = (d >> bit) & 1
+        tol = 5 * np.sqrt(0.25 / n)
+        assert abs(sb.mean() - 0.24) < tol
+        assert abs(db.mean() - 0.24) < tol
+        assert abs((sb & db).mean() - 0.05) < tol
+
+
+# ---- structures (reference known answers) ----------------------------------------
+
+def _csv_graph(name, directed, weighted=False):
+    if weighted:
+        s, d, w = oracle.read_edge_csv(golden(name), weighted=True)
+    else:
+        s, d = oracle.read_edge_csv(golden(name))
+        w = None
+    V = int(max(s.max(), d.max())) + 1
+    return oracle.CSC(V, s, d, w=w, symmetrize=not directed), s, d
+
+
+def test_dcsc_graph_10_9():
+    # dcsc.cc:138-191: weighted directed chain -> 9 columns, 9 entries
+    g, s, d = _csv_graph("graph_10_9.csv", True, weighted=True)
+    indeg = g.in_degrees()
+    assert g.nnz == 9 and int((indeg > 0).sum()) == 9
+    # message_passing.cc:48-88: one message per in-edge -> sum 9, values[0] = 0, others 1
+    assert int(indeg.sum()) == 9 and indeg[0] == 0 and (indeg[1:] == 1).all()
+    # weights survive the CSV parser's whitespace tolerance (graph_10_9.csv lines 6, 9)
+    assert np.isclose(g.w[g.col_ptr[6]], 102.1) and np.isclose(g.w[g.col_ptr[9]], 1.111)
+
+
+def test_dcsc_undirected_v100():
+    # dcsc.cc:81-136: 100 vertices, 4300 entries, every entry an input edge (either way)
+    g, s, d = _csv_graph("v100_e2150_ua_c3.csv", False)
+    assert g.V == 100 and g.nnz == 4300
+    edges = set(zip(s.tolist(), d.tolist()))
+    for v in range(g.V):
+        for e in range(int(g.col_ptr[v]), int(g.col_ptr[v + 1])):
+            u = int(g.row_idx[e])
+            assert (u, v) in edges or (v, u) in edges
+    # structure.cc:137-151: degrees 19 / 59 / 19
+    assert (g.od[0], g.od[99], g.od[28]) == (19, 59, 19)
+
+
+def test_out_degrees_from_in_graph_directed_v100():
+    # structure.cc:205-218 (generate_dense_out_degrees_fg on the DCSC)
+    g, s, d = _csv_graph("v100_e4300_da_c3.csv", True)
+    assert (g.od[0], g.od[99], g.od[28]) == (19, 59, 19)
+    assert np.array_equal(g.od, np.bincount(s, minlength=100).astype(np.uint32))
+
+
+def test_csc_sorted_and_complete():
+    s, d = oracle.rmat_edges(9, 8, 3)
+    g = oracle.CSC(512, s, d, symmetrize=True)
+    for v in range(g.V):
+        col = g.row_idx[int(g.col_ptr[v]):int(g.col_ptr[v + 1])]
+        assert (np.diff(col.astype(np.int64)) >= 0).all()
+    # multiset of entries = edges + reversals
+    ent = sorted(zip(np.repeat(np.arange(g.V), np.diff(g.col_ptr).astype(np.int64)).tolist(),
+                     g.row_idx.tolist()))
+    exp = sorted(list(zip(d.tolist(), s.tolist())) + list(zip(s.tolist(), d.tolist())))
+    assert ent == exp
+    # symmetrized CSR is the CSC
+    assert np.array_equal(g.row_ptr, g.col_ptr) and np.array_equal(g.col_idx, g.row_idx)
+
+
+# ---- PageRank ------------------------------------------------------------------------
+
+def _dense_M(g, weighted=False):
+    """M[v, u] = sum over entries u->v of (1/od(u) or w/W(u))."""
+    V = g.V
+    M = np.zeros((V, V))
+    cols = np.repeat(np.arange(V), np.diff(g.col_ptr).astype(np.int64))
+    if weighted:
+        W = np.zeros(V)
+        np.add.at(W, g.row_idx, g.w.astype(np.float64))
+        np.add.at(M, (cols, g.row_idx), g.w.astype(np.float64) / W[g.row_idx])
+    else:
+        od = g.od.astype(np.float64)
+        np.add.at(M, (cols, g.row_idx), 1.0 / od[g.row_idx])
+    return M
+
+
+def test_pagerank_chain_closed_form():
+    g, _, _ = _csv_graph("graph_10_9.csv", True)
+    v = np.arange(10)
+    for iters in (3, 20):
+        r, _ = oracle.pagerank(g, iters, D, 0.0)
+        # r_t(v) = 0.15 sum_{i<min(v+1,t)} 0.85^i + [t <= v] 0.85^t
+        exp = np.array([0.15 * sum(D ** i for i in range(min(k + 1, iters))) +
+                        (D ** iters if iters <= k else 0.0) for k in v])
+        np.testing.assert_allclose(r, exp, rtol=1e-14)
+
+
+def test_pagerank_fixed_point_linear_solve():
+    s, d = oracle.rmat_edges(8, 8, 11)
+    g = oracle.CSC(256, s, d)
+    r, deltas = oracle.pagerank(g, 400, D, 0.0)
+    M = _dense_M(g)
+    exact = np.linalg.solve(np.eye(g.V) - D * M, (1 - D) * np.ones(g.V))
+    np.testing.assert_allclose(r, exact, rtol=1e-10)
+    assert deltas[-1] == 0.0 and deltas[0] > deltas[-2]
+
+
+def test_pagerank_eps_jumps_to_one_final_iteration():
+    s, d = oracle.rmat_edges(8, 8, 11)
+    g = oracle.CSC(256, s, d)
+    r, deltas = oracle.pagerank(g, 100, D, 1e-3)
+    k = int(np.argmax(deltas[:-1] < 1e-3))
+    # the first delta below eps is followed by exactly one more (final) iteration
+    assert len(deltas) == k + 2 and (deltas[:k] >= 1e-3).all()
+    # the final state is the undivided rank, i.e. one more step from the stored x
+    r_full, _ = oracle.pagerank(g, k + 2, D, 0.0)
+    np.testing.assert_array_equal(r, r_full)
+
+
+def test_pagerank_step_threads_independent():
+    s, d = oracle.rmat_edges(12, 16, 5)
+    g = oracle.CSC(4096, s, d)
+    x = np.where(g.od > 0, 1.0 / np.maximum(g.od, 1), 1.0)
+    y1, d1 = oracle.pagerank_step(g, x, D, False, nthreads=1)
+    y4, d4 = oracle.pagerank_step(g, x, D, False, nthreads=4)
+    assert np.array_equal(y1, y4) and d1 == d4
+
+
+def test_weighted_pagerank_chain_and_solve():
+    g, _, _ = _csv_graph("graph_10_9.csv", True, weighted=True)
+    r, _ = oracle.weighted_pagerank(g, 20, D, 0.0)
+    gu, _, _ = _csv_graph("graph_10_9.csv", True)
+    ru, _ = oracle.pagerank(gu, 20, D, 0.0)
+    np.testing.assert_allclose(r, ru, rtol=1e-14)   # one out-edge per source: beta = 1
+    s, d = oracle.rmat_edges(8, 8, 13)
+    w = oracle.rmat_weights(13, s.size)
+    g = oracle.CSC(256, s, d, w=w)
+    r, _ = oracle.weighted_pagerank(g, 400, D, 0.0)
+    exact = np.linalg.solve(np.eye(g.V) - D * _dense_M(g, weighted=True), (1 - D) * np.ones(g.V))
+    np.testing.assert_allclose(r, exact, rtol=1e-10)
+
+
+# ---- PPR -----------------------------------------------------------------------------
+
+def test_ppr_chain_closed_form():
+    g, _, _ = _csv_graph("graph_10_9.csv", True)
+    r, _ = oracle.ppr(g, [0], 20, D, 0.0)
+    np.testing.assert_allclose(r, 0.15 * D ** np.arange(10), rtol=1e-14)
+
+
+def test_ppr_linear_solve_and_linearity():
+    s, d = oracle.rmat_edges(8, 8, 17)
+    g = oracle.CSC(256, s, d)
+    M = _dense_M(g)
+    seeds = [3, 40, 40, 200]   # duplicate counts once
+    r, _ = oracle.ppr(g, seeds, 400, D, 0.0)
+    b = np.zeros(g.V)
+    b[[3, 40, 200]] = (1 - D) / 3
+    exact = np.linalg.solve(np.eye(g.V) - D * M, b)
+    np.testing.assert_allclose(r, exact, rtol=1e-9, atol=1e-15)
+    # linear in the seed set, at a finite iteration count too
+    parts = [oracle.ppr(g, [v], 30, D, 0.0)[0] for v in (3, 40, 200)]
+    multi, _ = oracle.ppr(g, [3, 40, 200], 30, D, 0.0)
+    np.testing.assert_allclose(multi, sum(parts) / 3, rtol=1e-12, atol=1e-16)
+
+
+def test_ppr_pushpull_equals_pull_and_switch_is_sticky():
+    s, d = oracle.rmat_edges(10, 16, 19)
+    g = oracle.CSC(1024, s, d)
+    src = int(np.argmax((g.od > 0) & (g.od < 3)))
+    pull, dp = oracle.ppr(g, [src], 30, D, 0.0)
+    pp, dpp, modes = oracle.ppr_pushpull(g, src, 30, D, 0.0)
+    assert np.array_equal(pull, pp) and np.array_equal(dp, dpp)
+    assert modes[0] == 1                      # a low-degree source starts in push
+    first_pull = int(np.argmin(modes))
+    assert modes[first_pull] == 0 and (modes[first_pull:] == 0).all()
+
+
+# ---- BFS -------------------------------------------------------------------------------
+
+def _scipy_levels(g, root):
+    V = g.V
+    cols = np.repeat(np.arange(V), np.diff(g.col_ptr).astype(np.int64))
+    A = sp.csr_matrix((np.ones(g.nnz), (g.row_idx, cols)), shape=(V, V))  # u -> v
+    dist = csg.shortest_path(A, directed=True, unweighted=True, indices=root)
+    return np.where(np.isinf(dist), -1, dist).astype(np.int32)
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_bfs_levels_match_scipy(directed):
+    s, d = oracle.rmat_edges(11, 16, 23)
+    g = oracle.CSC(2048, s, d, symmetrize=not directed)
+    roots = np.flatnonzero(g.od > 0)[[0, 7, 300]]
+    for root in roots:
+        b = oracle.bfs(g, int(root))
+        lv = _scipy_levels(g, int(root))
+        assert np.array_equal(b["levels"], lv)
+        # frontier sizes are the level histogram; visited is the reached count
+        hist = np.bincount(lv[lv >= 0])
+        assert np.array_equal(b["frontier"], hist.astype(np.uint64))
+        assert b["visited"] == int((lv >= 0).sum())
+        # decision per epoch from the level sets alone (dualmode.hpp:279)
+        for ep in range(b["epochs"]):
+            A = int(g.od[lv == ep].sum())
+            assert b["active_edges"][ep] == A
+            assert b["modes"][ep] == int(A / g.edges_prime < 0.05)
+
+
+def test_bfs_tree_and_chain():
+    g, _, _ = _csv_graph("complete_binary_tree_10_9.csv", True)
+    b = oracle.bfs(g, 0)
+    assert np.array_equal(b["levels"], np.floor(np.log2(np.arange(10) + 1)).astype(np.int32))
+    g, _, _ = _csv_graph("graph_10_9.csv", True)
+    b = oracle.bfs(g, 0)
+    assert np.array_equal(b["levels"], np.arange(10, dtype=np.int32)) and b["visited"] == 10
+
+
+def test_topk_tie_break():
+    r = np.array([0.5, 0.9, 0.5, 0.1, 0.9, 0.5])
+    ids, vals = oracle.topk(r, 4)
+    order = np.lexsort((np.arange(r.size), -r))[:4]   # rank desc, id asc
+    assert np.array_equal(ids, order) and np.array_equal(vals, r[order])
