@@ -33,11 +33,14 @@ def log(*a):
 
 
 def cpu_baseline(g, seconds_target=10.0, threads=None):
-    """Time the oracle's pull iteration (FP64, OpenMP) on a slice of the graph."""
+    """Time the oracle's pull iteration (FP64, OpenMP) on a sample of the graph's columns:
+    16 evenly spaced ranges of V/256 columns each (1/16 of the columns)."""
     import oracle
     threads = threads or min(16, os.cpu_count() or 1)
     cp, ri = g.download_csc()
     od = g.out_degrees()
+    if g.info.relabeled:  # out_degrees() is in input order; the CSC is internal
+        od = od[np.argsort(g.permutation())]
 
     class H:  # host CSC view for the oracle binding
         pass
@@ -45,21 +48,33 @@ def cpu_baseline(g, seconds_target=10.0, threads=None):
     h.V, h.col_ptr, h.row_idx, h.od = g.V, cp, ri, od
     x = np.where(od > 0, 1.0 / np.maximum(od, 1), 1.0)
     y = np.zeros(g.V, np.float64)
-    # slice: first columns holding about E/16 edges
-    target = int(cp[-1]) // 16
-    v_hi = int(np.searchsorted(cp, target))
-    v_hi = max(1, min(v_hi, g.V))
-    edges = int(cp[v_hi] - cp[0])
+    width = max(1, g.V // 256)
+    ranges = [(int(k * g.V // 16), int(min(g.V, k * g.V // 16 + width))) for k in range(16)]
+    edges = int(sum(cp[hi] - cp[lo] for lo, hi in ranges))
     done_edges, spent, passes = 0, 0.0, 0
-    while spent < seconds_target and passes < 20:
+    while spent < seconds_target and passes < 50:
         t0 = time.perf_counter()
-        oracle.pagerank_step(h, x, 0.85, False, 0, v_hi, threads, y=y)
+        for lo, hi in ranges:
+            oracle.pagerank_step(h, x, 0.85, False, lo, hi, threads, y=y)
         spent += time.perf_counter() - t0
         done_edges += edges
         passes += 1
     return {"value": done_edges / spent / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"{passes} passes of one PageRank pull iteration (FP64) over columns "
-                      f"[0, {v_hi}) = {edges} in-edges of the same graph, {spent:.1f} s"}
+            "sample": f"{passes} passes of one PageRank pull iteration (FP64, OpenMP) over 16 evenly "
+                      f"spaced ranges of {width} columns ({edges} in-edges) of the same graph, "
+                      f"{spent:.1f} s"}
+
+
+def profiled_traffic(scale):
+    """HBM bytes per launch of k_pull_tiles from the committed rocprofv3 summary of this workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"k_pull_tiles_rmat{scale}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def run_bfs(ctx, scale, roots, seed=11):
@@ -95,6 +110,9 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tile", type=int, default=0, help="merge-path items per tile (0 = default)")
+    ap.add_argument("--no-relabel", action="store_true", help="keep input ids (no degree renumbering)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (reference dispatch for traffic counters)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--bfs-scale", type=int, default=24)
@@ -113,8 +131,17 @@ def main():
     ctx = Context(0)
     log(f"[bench] building RMAT scale {args.scale} ef {args.edge_factor} on the device")
     t0 = time.time()
-    g = Graph.rmat(ctx, args.scale, args.edge_factor, seed=args.seed)
+    g = Graph.rmat(ctx, args.scale, args.edge_factor, seed=args.seed, relabel=not args.no_relabel)
     build_s = time.time() - t0
+    if args.calibrate_copy:
+        from plato_amd import DenseState
+        a = DenseState(ctx, 1 << 28, np.float32)
+        b = DenseState(ctx, 1 << 28, np.float32)
+        a.fill(1.0)
+        b.copy_from(a)
+        ctx.sync()
+        a.close()
+        b.close()
     if args.tile:
         g.set_tile(args.tile)
     V, E = g.V, g.nnz
@@ -164,9 +191,13 @@ def main():
                    "edge_factor": args.edge_factor, "V": V, "E": E, "seed": args.seed,
                    "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "damping": 0.85,
                    "state": "fp32 rank/od, compensated sums, fp64 apply",
-                   "tile_items": g.info.tile_items, "graph_build_s": round(build_s, 2)},
+                   "tile_items": g.info.tile_items, "relabel_by_degree": bool(g.info.relabeled),
+                   "graph_build_s": round(build_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (profiled_traffic(args.scale)
+                                 if (not args.tile and not args.no_relabel) else None),
+                     "traffic_source": f"profiles/k_pull_tiles_rmat{args.scale}.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": "k_pull_tiles", "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_ms": round(tile_avg_s * 1e3, 4),
                      "fixup_avg_ms": round(fix_ms / max(fix_n, 1), 4)},

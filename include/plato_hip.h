@@ -57,7 +57,8 @@ typedef struct pe_bitmap_s pe_bitmap_t;
 
 /* ===========================================================================
  * Context.  Replaces cluster_info_t (plato/graph/base.hpp:158-233) for one
- * GPU: device, stream, allocation ledger.
+ * GPU: device, stream, allocation ledger.  Destroy every graph, vector,
+ * bitmap, batch and server of a context before the context itself.
  * ========================================================================= */
 int pe_ctx_create(int device, pe_ctx_t** out);
 int pe_ctx_destroy(pe_ctx_t* ctx);
@@ -88,6 +89,9 @@ int pe_ctx_num_cus(pe_ctx_t* ctx, int* cus);
 #define PE_GRAPH_SYMMETRIZE 1u /* add v->u for every u->v (is_directed_ = false) */
 #define PE_GRAPH_BUILD_CSR 2u  /* also build out-edges (needed by push mode) */
 #define PE_GRAPH_WEIGHTED 4u   /* keep per-edge weights, normalised per source */
+#define PE_GRAPH_RELABEL_DEGREE 8u /* renumber ids by out-degree (desc) inside the engine;
+                                      algorithm entry points map seeds, roots and results, so
+                                      callers keep input ids; structure downloads are internal */
 
 typedef struct {
   uint32_t V;             /* id space [0, V) */
@@ -99,6 +103,7 @@ typedef struct {
   int32_t is_directed;    /* graph_info_t::is_directed_ */
   int32_t has_csr;
   int32_t has_weights;
+  int32_t relabeled;
   uint32_t tile_items;    /* merge-path items per pull tile */
   uint64_t num_tiles;
   uint64_t num_fixup_rows;
@@ -123,6 +128,8 @@ int pe_graph_get_info(const pe_graph_t* g, pe_graph_info_t* info);
 /* Copy the CSC (col_ptr[V+1], row_idx[nnz], optional normalised weights[nnz]). */
 int pe_graph_download_csc(pe_graph_t* g, uint64_t* col_ptr, uint32_t* row_idx, float* wn);
 int pe_graph_download_csr(pe_graph_t* g, uint64_t* row_ptr, uint32_t* col_idx);
+/* rank_of[input id] = internal id (identity unless PE_GRAPH_RELABEL_DEGREE). */
+int pe_graph_permutation(pe_graph_t* g, uint32_t* rank_of);
 /* Re-tile the pull gather (merge-path items per tile: 1024, 2048 or 4096). */
 int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items);
 
@@ -136,6 +143,7 @@ int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items);
 int pe_vec_create(pe_ctx_t* ctx, int dtype, uint64_t n, pe_vec_t** out);
 int pe_vec_destroy(pe_vec_t* v);
 int pe_vec_fill(pe_vec_t* v, double value);                         /* dense_state.hpp:241-254 */
+int pe_vec_copy(pe_vec_t* dst, const pe_vec_t* src);                 /* same dtype and length */
 int pe_vec_upload(pe_vec_t* v, const void* host, uint64_t n);        /* raw, same dtype */
 int pe_vec_download(pe_vec_t* v, void* host, uint64_t n);            /* raw, same dtype */
 int pe_vec_download_f64(pe_vec_t* v, double* host, uint64_t n);      /* converted */

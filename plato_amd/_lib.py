@@ -9,14 +9,14 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libplato_hip.so")
+LIB_PATH = os.environ.get("PLATO_HIP_LIB") or os.path.join(HERE, "libplato_hip.so")
 
 PE_OK = 0
 PE_ERR = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "bad state",
           5: "out of range", 6: "internal error"}
 PE_F32, PE_F64, PE_U32 = 0, 1, 2
 PE_KT_PULL_TILE, PE_KT_PULL_FIXUP, PE_KT_BFS_PUSH, PE_KT_BFS_PULL, PE_KT_PPR_BLOCK = 0, 1, 2, 3, 4
-PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED = 1, 2, 4
+PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED, PE_GRAPH_RELABEL_DEGREE = 1, 2, 4, 8
 
 
 class PlatoHipError(RuntimeError):
@@ -29,7 +29,7 @@ class GraphInfo(C.Structure):
     _fields_ = [("V", C.c_uint32), ("max_v_i", C.c_uint32), ("vertices", C.c_uint32),
                 ("max_in_degree", C.c_uint32), ("edges", C.c_uint64), ("nnz", C.c_uint64),
                 ("is_directed", C.c_int32), ("has_csr", C.c_int32), ("has_weights", C.c_int32),
-                ("tile_items", C.c_uint32), ("num_tiles", C.c_uint64),
+                ("relabeled", C.c_int32), ("tile_items", C.c_uint32), ("num_tiles", C.c_uint64),
                 ("num_fixup_rows", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
@@ -77,9 +77,11 @@ SIGNATURES = {
     "pe_graph_download_csc": [_vp, _vp, _vp, _vp],
     "pe_graph_download_csr": [_vp, _vp, _vp],
     "pe_graph_set_tile": [_vp, _u32],
+    "pe_graph_permutation": [_vp, _vp],
     "pe_vec_create": [_vp, _i32, _u64, _P(_vp)],
     "pe_vec_destroy": [_vp],
     "pe_vec_fill": [_vp, _dbl],
+    "pe_vec_copy": [_vp, _vp],
     "pe_vec_upload": [_vp, _vp, _u64],
     "pe_vec_download": [_vp, _vp, _u64],
     "pe_vec_download_f64": [_vp, _vp, _u64],

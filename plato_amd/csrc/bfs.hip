@@ -304,12 +304,13 @@ int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint
   if (!g) return PE_ERR_INVALID;
   return guarded(g->ctx, [&] {
     PE_REQUIRE(root < g->V, PE_ERR_RANGE, "root >= V");
+    uint32_t iroot = root;
+    to_internal_ids(g, &root, 1, &iroot);
     bfs_state st(g);
     double secs = 0.0;
-    const int epochs = st.run(root, threshold, modes, frontier, active_edges, max_epochs, &secs);
+    const int epochs = st.run(iroot, threshold, modes, frontier, active_edges, max_epochs, &secs);
     hipStream_t s = g->ctx->stream;
-    if (levels)
-      PE_HIP(hipMemcpyAsync(levels, st.level.p, (uint64_t)g->V * 4, hipMemcpyDeviceToHost, s));
+    if (levels) download_external(g, st.level.p, levels, 4);
     PE_HIP(hipMemsetAsync(st.red.p, 0, 16, s));
     k_reached_edges<<<grid_for(g->V, 256, 4096), 256, 0, s>>>(
         st.level.p, g->od.p, g->V, st.red.p, reinterpret_cast<unsigned int*>(st.red.p + 1));

@@ -84,6 +84,20 @@ __global__ void k_sum_final(const double* __restrict__ part, unsigned n, double*
   if (threadIdx.x == 0) *out = s[0];
 }
 
+// plain streaming copy (also the calibration kernel for traffic counters)
+__global__ __launch_bounds__(256) void k_copy_words(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   uint64_t n16) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_copy_tail(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_bitmap_fill(uint64_t* __restrict__ w, uint64_t words, uint64_t bits) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -337,6 +351,22 @@ int pe_vec_fill(pe_vec_t* v, double value) {
       k_fill<double><<<g, 256, 0, v->ctx->stream>>>((double*)v->p, v->n, value);
     else
       k_fill<uint32_t><<<g, 256, 0, v->ctx->stream>>>((uint32_t*)v->p, v->n, (uint32_t)value);
+    PE_HIP(hipGetLastError());
+  });
+}
+
+int pe_vec_copy(pe_vec_t* dst, const pe_vec_t* src) {
+  if (!dst || !src) return PE_ERR_INVALID;
+  return guarded(dst->ctx, [&] {
+    PE_REQUIRE(dst->dtype == src->dtype && dst->n == src->n, PE_ERR_INVALID, "vectors differ in dtype or length");
+    const uint64_t bytes = src->n * dtype_size(src->dtype);
+    const uint64_t n16 = bytes / 16;
+    hipStream_t s = dst->ctx->stream;
+    if (n16)
+      k_copy_words<<<persistent_grid(dst->ctx, grid_for(n16, 256), 16), 256, 0, s>>>(
+          (const uint4*)src->p, (uint4*)dst->p, n16);
+    if (bytes % 16)
+      k_copy_tail<<<1, 64, 0, s>>>((const uint8_t*)src->p + n16 * 16, (uint8_t*)dst->p + n16 * 16, bytes % 16);
     PE_HIP(hipGetLastError());
   });
 }

@@ -163,14 +163,18 @@ __global__ void k_row_lengths(const uint64_t* __restrict__ ptr, uint32_t V, uint
 
 // ids with an edge: count, max id, max in-degree
 __global__ void k_vertex_stats(const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ od,
-                               uint32_t V, unsigned int* __restrict__ out) {
+                               uint32_t V, const uint32_t* __restrict__ perm, unsigned int* __restrict__ out) {
   unsigned int cnt = 0, mx = 0, mdeg = 0;
   bool any = false;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t indeg = (uint32_t)(col_ptr[v + 1] - col_ptr[v]);
     if (indeg > mdeg) mdeg = indeg;
-    if (indeg || od[v]) { ++cnt; mx = (unsigned)v; any = true; }
+    if (indeg || od[v]) {
+      ++cnt;
+      mx = max(mx, perm ? perm[v] : (unsigned)v);
+      any = true;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     cnt += __shfl_down(cnt, off, 64);
@@ -370,7 +374,8 @@ static void finish_build(pe_graph_s* g, dbuf<uint64_t>& keys, dbuf<float>* w, in
   }
   dbuf<unsigned int> st(ctx, 3);
   PE_HIP(hipMemsetAsync(st.p, 0, 12, s));
-  k_vertex_stats<<<grid_for(g->V, 256, 4096), 256, 0, s>>>(g->col_ptr.p, g->od.p, g->V, st.p);
+  k_vertex_stats<<<grid_for(g->V, 256, 4096), 256, 0, s>>>(g->col_ptr.p, g->od.p, g->V,
+                                                          g->relabeled ? g->perm.p : nullptr, st.p);
   PE_HIP(hipGetLastError());
   unsigned int hs[3];
   PE_HIP(hipMemcpyAsync(hs, st.p, 12, hipMemcpyDeviceToHost, s));
@@ -410,6 +415,159 @@ static void normalise_weights(pe_graph_s* g, dbuf<uint64_t>& csr_keys, dbuf<floa
   PE_HIP(hipStreamSynchronize(s));
 }
 
+// ---- degree relabeling -------------------------------------------------------------
+// New id = rank of the vertex by (out-degree desc, id asc), out-degree counted over
+// the CSC entries (reversals included).  Sources with many out-edges get small,
+// adjacent ids, so the hot part of the gathered vector packs into few cache lines.
+__global__ void k_entry_out_degrees(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                    uint64_t E, int sym, uint32_t* __restrict__ od) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    atomicAdd(&od[src[i]], 1u);
+    if (sym) atomicAdd(&od[dst[i]], 1u);
+  }
+}
+
+__global__ void k_degree_keys(const uint32_t* __restrict__ od, uint32_t V, uint64_t* __restrict__ keys) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    keys[v] = ((uint64_t)(0xFFFFFFFFu - od[v]) << 32) | v;
+}
+
+__global__ void k_perm_from_keys(const uint64_t* __restrict__ keys, uint32_t V, uint32_t* __restrict__ perm,
+                                 uint32_t* __restrict__ rank_of) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t old = (uint32_t)(keys[i] & 0xFFFFFFFFull);
+    perm[i] = old;
+    rank_of[old] = (uint32_t)i;
+  }
+}
+
+__global__ void k_relabel(uint32_t* __restrict__ ids, uint64_t n, const uint32_t* __restrict__ rank_of) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    ids[i] = rank_of[ids[i]];
+}
+
+static void build_relabel(pe_graph_s* g, dbuf<uint32_t>& ds, dbuf<uint32_t>& dd, bool sym) {
+  pe_ctx_s* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint32_t V = g->V;
+  dbuf<uint32_t> od(ctx, V);
+  PE_HIP(hipMemsetAsync(od.p, 0, (uint64_t)V * 4, s));
+  if (g->edges)
+    k_entry_out_degrees<<<grid_for(g->edges, 256, 1u << 20), 256, 0, s>>>(ds.p, dd.p, g->edges, sym, od.p);
+  dbuf<uint64_t> keys(ctx, V);
+  k_degree_keys<<<grid_for(V, 256, 8192), 256, 0, s>>>(od.p, V, keys.p);
+  PE_HIP(hipGetLastError());
+  od.release();
+  sort_keys(ctx, keys, nullptr, V, 64);
+  g->perm.alloc(ctx, V);
+  g->rank_of.alloc(ctx, V);
+  k_perm_from_keys<<<grid_for(V, 256, 8192), 256, 0, s>>>(keys.p, V, g->perm.p, g->rank_of.p);
+  if (g->edges) {
+    k_relabel<<<grid_for(g->edges, 256, 1u << 20), 256, 0, s>>>(ds.p, g->edges, g->rank_of.p);
+    k_relabel<<<grid_for(g->edges, 256, 1u << 20), 256, 0, s>>>(dd.p, g->edges, g->rank_of.p);
+  }
+  PE_HIP(hipGetLastError());
+  g->relabeled = true;
+}
+
+// Common builder over a device edge list.
+static void build_from_edges(pe_graph_s* g, dbuf<uint32_t>& ds, dbuf<uint32_t>& dd, dbuf<float>& dw,
+                             uint32_t flags) {
+  pe_ctx_s* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const bool sym = flags & PE_GRAPH_SYMMETRIZE;
+  const bool weighted = flags & PE_GRAPH_WEIGHTED;
+  const uint64_t E = g->edges;
+  g->directed = !sym;
+  g->nnz = E * (sym ? 2 : 1);
+  g->has_weights = weighted;
+  const int b = bits_for(g->V);
+  if (flags & PE_GRAPH_RELABEL_DEGREE) build_relabel(g, ds, dd, sym);
+  auto make_keys = [&](int csr, dbuf<uint64_t>& keys, dbuf<float>* wk) {
+    keys.alloc(ctx, std::max<uint64_t>(1, g->nnz));
+    if (wk) wk->alloc(ctx, std::max<uint64_t>(1, g->nnz));
+    if (E)
+      k_edge_keys<<<grid_for(E, 256, 1u << 20), 256, 0, s>>>(ds.p, dd.p, weighted ? dw.p : nullptr, E, b, csr,
+                                                             sym, keys.p, wk ? wk->p : nullptr);
+    PE_HIP(hipGetLastError());
+  };
+  {
+    dbuf<uint64_t> keys;
+    dbuf<float> wk;
+    make_keys(0, keys, weighted ? &wk : nullptr);
+    finish_build(g, keys, weighted ? &wk : nullptr, b);
+  }
+  if (weighted) {
+    dbuf<uint64_t> keys;
+    dbuf<float> wk;
+    make_keys(1, keys, &wk);
+    normalise_weights(g, keys, wk, b);
+  }
+  if (sym) {
+    g->has_csr = true;
+    g->csr_is_csc = true;
+  } else if (flags & PE_GRAPH_BUILD_CSR) {
+    dbuf<uint64_t> keys;
+    make_keys(1, keys, nullptr);
+    build_csr_from_keys(g, keys, b);
+    g->has_csr = true;
+  }
+  PE_HIP(hipStreamSynchronize(s));
+}
+
+// ---- id mapping helpers (relabeled graphs) -----------------------------------------
+template <typename T>
+__global__ void k_gather_by_rank(const T* __restrict__ in, const uint32_t* __restrict__ rank_of, uint32_t V,
+                                 T* __restrict__ out) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = in[rank_of[v]];
+}
+
+void to_external_order(pe_graph_s* g, const void* in, void* out, size_t elem) {
+  hipStream_t s = g->ctx->stream;
+  if (elem == 8)
+    k_gather_by_rank<uint64_t><<<grid_for(g->V, 256, 8192), 256, 0, s>>>((const uint64_t*)in, g->rank_of.p, g->V,
+                                                                        (uint64_t*)out);
+  else
+    k_gather_by_rank<uint32_t><<<grid_for(g->V, 256, 8192), 256, 0, s>>>((const uint32_t*)in, g->rank_of.p, g->V,
+                                                                        (uint32_t*)out);
+  PE_HIP(hipGetLastError());
+}
+
+void to_internal_ids(pe_graph_s* g, const uint32_t* host_ids, uint32_t n, uint32_t* host_out) {
+  if (!g->relabeled) {
+    std::memcpy(host_out, host_ids, (size_t)n * 4);
+    return;
+  }
+  hipStream_t s = g->ctx->stream;
+  dbuf<uint32_t> d(g->ctx, std::max<uint32_t>(1, n));
+  PE_HIP(hipMemcpyAsync(d.p, host_ids, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  k_relabel<<<grid_for(n, 256, 4096), 256, 0, s>>>(d.p, n, g->rank_of.p);
+  PE_HIP(hipGetLastError());
+  PE_HIP(hipMemcpyAsync(host_out, d.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  PE_HIP(hipStreamSynchronize(s));
+}
+
+// Download a V-long device array in external (input) id order.
+void download_external(pe_graph_s* g, const void* dev, void* host, size_t elem) {
+  hipStream_t s = g->ctx->stream;
+  const uint64_t bytes = (uint64_t)g->V * elem;
+  if (g->relabeled) {
+    dbuf<uint8_t> tmp(g->ctx, std::max<uint64_t>(1, bytes));
+    to_external_order(g, dev, tmp.p, elem);
+    PE_HIP(hipMemcpyAsync(host, tmp.p, bytes, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+  } else {
+    PE_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+  }
+}
+
 }  // namespace pe
 
 using namespace pe;
@@ -429,43 +587,16 @@ int pe_graph_create_rmat(pe_ctx_t* ctx, uint32_t scale, uint32_t edge_factor, ui
     g->ctx = ctx;
     g->V = 1u << scale;
     g->edges = (uint64_t)edge_factor << scale;
-    const bool sym = flags & PE_GRAPH_SYMMETRIZE;
-    const bool weighted = flags & PE_GRAPH_WEIGHTED;
-    g->directed = !sym;
-    g->nnz = g->edges * (sym ? 2 : 1);
-    g->has_weights = weighted;
-    const int b = bits_for(g->V);
     const rmat_params p = make_rmat(scale, seed, a, b_, c);
-    {
-      dbuf<uint64_t> keys(ctx, std::max<uint64_t>(1, g->nnz));
-      dbuf<float> w;
-      if (weighted) w.alloc(ctx, std::max<uint64_t>(1, g->nnz));
-      if (g->edges)
-        k_rmat_keys<<<grid_for(g->edges, 256, 1u << 20), 256, 0, s>>>(p, g->edges, 0, b, 0, sym, keys.p,
-                                                                     weighted ? w.p : nullptr);
-      PE_HIP(hipGetLastError());
-      finish_build(g, keys, weighted ? &w : nullptr, b);
-    }
-    if (weighted) {
-      dbuf<uint64_t> keys(ctx, std::max<uint64_t>(1, g->nnz));
-      dbuf<float> w(ctx, std::max<uint64_t>(1, g->nnz));
-      if (g->edges)
-        k_rmat_keys<<<grid_for(g->edges, 256, 1u << 20), 256, 0, s>>>(p, g->edges, 0, b, 1, sym, keys.p, w.p);
-      PE_HIP(hipGetLastError());
-      normalise_weights(g, keys, w, b);
-    }
-    if (sym) {
-      g->has_csr = true;
-      g->csr_is_csc = true;
-    } else if (flags & PE_GRAPH_BUILD_CSR) {
-      dbuf<uint64_t> keys(ctx, std::max<uint64_t>(1, g->nnz));
-      if (g->edges)
-        k_rmat_keys<<<grid_for(g->edges, 256, 1u << 20), 256, 0, s>>>(p, g->edges, 0, b, 1, 0, keys.p, nullptr);
-      PE_HIP(hipGetLastError());
-      build_csr_from_keys(g, keys, b);
-      g->has_csr = true;
-    }
-    PE_HIP(hipStreamSynchronize(s));
+    const uint64_t E = g->edges;
+    dbuf<uint32_t> ds(ctx, std::max<uint64_t>(1, E)), dd(ctx, std::max<uint64_t>(1, E));
+    dbuf<float> dw;
+    if (flags & PE_GRAPH_WEIGHTED) dw.alloc(ctx, std::max<uint64_t>(1, E));
+    if (E)
+      k_rmat_edges<<<grid_for(E, 256, 1u << 20), 256, 0, s>>>(p, 0, E, ds.p, dd.p,
+                                                              (flags & PE_GRAPH_WEIGHTED) ? dw.p : nullptr);
+    PE_HIP(hipGetLastError());
+    build_from_edges(g, ds, dd, dw, flags);
     *out = guard.release();
   });
 }
@@ -506,11 +637,6 @@ int pe_graph_create(pe_ctx_t* ctx, const uint32_t* src, const uint32_t* dst, con
     g->ctx = ctx;
     g->V = V;
     g->edges = E;
-    const bool sym = flags & PE_GRAPH_SYMMETRIZE;
-    g->directed = !sym;
-    g->nnz = E * (sym ? 2 : 1);
-    g->has_weights = weighted;
-    const int b = bits_for(V);
     dbuf<uint32_t> ds(ctx, std::max<uint64_t>(1, E)), dd(ctx, std::max<uint64_t>(1, E));
     dbuf<float> dw;
     if (E) {
@@ -521,37 +647,20 @@ int pe_graph_create(pe_ctx_t* ctx, const uint32_t* src, const uint32_t* dst, con
       dw.alloc(ctx, std::max<uint64_t>(1, E));
       if (E) PE_HIP(hipMemcpyAsync(dw.p, w, E * 4, hipMemcpyHostToDevice, s));
     }
-    auto make_keys = [&](int csr, dbuf<uint64_t>& keys, dbuf<float>* wk) {
-      keys.alloc(ctx, std::max<uint64_t>(1, g->nnz));
-      if (wk) wk->alloc(ctx, std::max<uint64_t>(1, g->nnz));
-      if (E)
-        k_edge_keys<<<grid_for(E, 256, 1u << 20), 256, 0, s>>>(ds.p, dd.p, weighted ? dw.p : nullptr, E, b,
-                                                               csr, sym, keys.p, wk ? wk->p : nullptr);
-      PE_HIP(hipGetLastError());
-    };
-    {
-      dbuf<uint64_t> keys;
-      dbuf<float> wk;
-      make_keys(0, keys, weighted ? &wk : nullptr);
-      finish_build(g, keys, weighted ? &wk : nullptr, b);
-    }
-    if (weighted) {
-      dbuf<uint64_t> keys;
-      dbuf<float> wk;
-      make_keys(1, keys, &wk);
-      normalise_weights(g, keys, wk, b);
-    }
-    if (sym) {
-      g->has_csr = true;
-      g->csr_is_csc = true;
-    } else if (flags & PE_GRAPH_BUILD_CSR) {
-      dbuf<uint64_t> keys;
-      make_keys(1, keys, nullptr);
-      build_csr_from_keys(g, keys, b);
-      g->has_csr = true;
-    }
-    PE_HIP(hipStreamSynchronize(s));
+    build_from_edges(g, ds, dd, dw, flags);
     *out = guard.release();
+  });
+}
+
+int pe_graph_permutation(pe_graph_t* g, uint32_t* rank_of) {
+  if (!g || !rank_of) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    if (!g->relabeled) {
+      for (uint32_t v = 0; v < g->V; ++v) rank_of[v] = v;
+      return;
+    }
+    PE_HIP(hipMemcpyAsync(rank_of, g->rank_of.p, (uint64_t)g->V * 4, hipMemcpyDeviceToHost, g->ctx->stream));
+    PE_HIP(hipStreamSynchronize(g->ctx->stream));
   });
 }
 
@@ -574,6 +683,7 @@ int pe_graph_get_info(const pe_graph_t* g, pe_graph_info_t* info) {
   info->is_directed = g->directed ? 1 : 0;
   info->has_csr = g->has_csr ? 1 : 0;
   info->has_weights = g->has_weights ? 1 : 0;
+  info->relabeled = g->relabeled ? 1 : 0;
   info->tile_items = g->tile_items;
   info->num_tiles = g->num_tiles;
   info->num_fixup_rows = g->num_fix;
@@ -613,10 +723,7 @@ int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items) {
 
 int pe_graph_out_degrees(pe_graph_t* g, uint32_t* od_host) {
   if (!g || !od_host) return PE_ERR_INVALID;
-  return guarded(g->ctx, [&] {
-    PE_HIP(hipMemcpyAsync(od_host, g->od.p, (uint64_t)g->V * 4, hipMemcpyDeviceToHost, g->ctx->stream));
-    PE_HIP(hipStreamSynchronize(g->ctx->stream));
-  });
+  return guarded(g->ctx, [&] { download_external(g, g->od.p, od_host, 4); });
 }
 
 int pe_graph_in_degrees(pe_graph_t* g, uint32_t* id_host) {
@@ -625,8 +732,7 @@ int pe_graph_in_degrees(pe_graph_t* g, uint32_t* id_host) {
     dbuf<uint32_t> tmp(g->ctx, std::max<uint32_t>(1, g->V));
     k_row_lengths<<<grid_for(g->V, 256, 65536), 256, 0, g->ctx->stream>>>(g->col_ptr.p, g->V, tmp.p);
     PE_HIP(hipGetLastError());
-    PE_HIP(hipMemcpyAsync(id_host, tmp.p, (uint64_t)g->V * 4, hipMemcpyDeviceToHost, g->ctx->stream));
-    PE_HIP(hipStreamSynchronize(g->ctx->stream));
+    download_external(g, tmp.p, id_host, 4);
   });
 }
 

@@ -154,11 +154,16 @@ struct pe_graph_s {
   pe::dbuf<uint32_t> fix_t1;     // [F] tile holding the row's end
   uint64_t num_fix_long = 0;     // fix rows [num_fix - num_fix_long, num_fix) span > 8 tiles
 
+  // degree relabeling: internal id = rank_of[input id], input id = perm[internal id]
+  bool relabeled = false;
+  pe::dbuf<uint32_t> perm;       // [V]
+  pe::dbuf<uint32_t> rank_of;    // [V]
+
   const uint64_t* csr_ptr() const { return csr_is_csc ? col_ptr.p : row_ptr.p; }
   const uint32_t* csr_idx() const { return csr_is_csc ? row_idx.p : col_idx.p; }
   uint64_t device_bytes() const {
     return col_ptr.bytes() + row_idx.bytes() + wn.bytes() + od.bytes() + row_ptr.bytes() +
-           col_idx.bytes() + tile_row.bytes() + tile_edge.bytes() + fix_row.bytes() +
+           col_idx.bytes() + perm.bytes() + rank_of.bytes() + tile_row.bytes() + tile_edge.bytes() + fix_row.bytes() +
            fix_t0.bytes() + fix_t1.bytes();
   }
 };
@@ -166,6 +171,10 @@ struct pe_graph_s {
 namespace pe {
 // graph.hip
 void graph_build_tiles(pe_graph_s* g, uint32_t tile_items);
+// relabeled graphs: input ids -> internal ids (host arrays), and V-long device
+// arrays downloaded in input id order (elem = 4 or 8 bytes)
+void to_internal_ids(pe_graph_s* g, const uint32_t* host_ids, uint32_t n, uint32_t* host_out);
+void download_external(pe_graph_s* g, const void* dev, void* host, size_t elem);
 // vectors
 struct vec_view {
   int dtype;

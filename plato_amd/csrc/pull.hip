@@ -22,7 +22,34 @@
 #include "pe_internal.hpp"
 #include "pull.hpp"
 
+#ifndef PE_PULL_NT
+#define PE_PULL_NT 0
+#endif
+#ifndef PE_GATHER_NT
+#define PE_GATHER_NT 0
+#endif
+
 namespace pe {
+
+// Streamed structure loads (row_idx, col_ptr) may bypass cache allocation so
+// the gathered vector keeps the caches.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+#if PE_PULL_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_gather(const T* p) {
+#if PE_GATHER_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
 
 // ---- compensated pairs ----------------------------------------------------------
 template <typename T>
@@ -105,7 +132,37 @@ __device__ __forceinline__ void block_reduce2(double& a, double& b, double* sm) 
   }
 }
 
+// Apply with the row's out-degree and previous state already in registers.
+template <typename T>
+__device__ __forceinline__ double apply_vals(const rank_args& a, bool final_iter, uint32_t v,
+                                             double s, T xold, uint32_t d, T* __restrict__ y,
+                                             double& rsum) {
+  if (a.mode == MODE_RAW_SUM) {
+    y[v] = (T)s;
+    return 0.0;
+  }
+  const double restart = a.base + (seed_bit(a.seeds, v) ? a.seed_mass : 0.0);
+  const double r = restart + a.damping * s;
+  rsum += r;
+  if (final_iter) {
+    y[v] = (T)r;
+    return 0.0;
+  }
+  if (a.divide_by_od && d > 0) {
+    const T xn = (T)(r / (double)d);
+    y[v] = xn;
+    return fabs((double)xn - (double)xold) * (double)d;
+  }
+  const T xn = (T)r;
+  y[v] = xn;
+  return fabs((double)xn - (double)xold);
+}
+
 // ---- tile kernel -----------------------------------------------------------------------
+// Persistent: block b handles tiles b, b+G, ...  The next tile's descriptors and
+// row_idx slice are loaded while the current one is summed (software pipeline);
+// each row's out-degree and previous state are loaded with the row ends, so the
+// apply at the end of the tile is one coalesced pass with no dependent loads.
 template <typename T, bool WEIGHTED, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_pull_tiles(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
@@ -118,6 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_tiles(
   constexpr int WAVES = BLOCK / 64;
   __shared__ T s_val[TILE];
   __shared__ uint32_t s_end[TILE];
+  __shared__ double s_sum[TILE];
   __shared__ cpair<T> s_wtot[WAVES];
   __shared__ int s_wflag[WAVES];
   __shared__ double s_red[2 * WAVES];
@@ -135,26 +193,77 @@ __global__ __launch_bounds__(BLOCK) void k_pull_tiles(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double delta = 0.0, rsum = 0.0;
 
-  for (uint64_t t = blockIdx.x; t < T_tiles; t += gridDim.x) {
-    const uint32_t r0 = tile_row[t], r1 = tile_row[t + 1];
-    const uint64_t e0 = tile_edge[t], e1 = tile_edge[t + 1];
-    const int nr = (int)(r1 - r0), ne = (int)(e1 - e0);
-    const int total = nr + ne;
-    // 1. stage gathered values and row ends
+  uint64_t t = blockIdx.x;
+  uint32_t r0 = 0, r1 = 0;
+  uint64_t e0 = 0, e1 = 0;
+  uint32_t u[ITEMS];
+  float w[ITEMS];
+  if (t < T_tiles) {
+    r0 = tile_row[t];
+    r1 = tile_row[t + 1];
+    e0 = tile_edge[t];
+    e1 = tile_edge[t + 1];
+    const int ne = (int)(e1 - e0);
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int i = tid + k * BLOCK;
-      if (i < ne) {
-        const uint32_t u = row_idx[e0 + i];
-        T v = x[u];
-        if (WEIGHTED) v = v * (T)wn[e0 + i];
-        s_val[i] = v;
+      u[k] = i < ne ? ld_stream(row_idx + e0 + i) : 0u;
+      w[k] = (WEIGHTED && i < ne) ? ld_stream(wn + e0 + i) : 1.0f;
+    }
+  }
+
+  for (; t < T_tiles; t += gridDim.x) {
+    const int nr = (int)(r1 - r0), ne = (int)(e1 - e0);
+    const int total = nr + ne;
+    // 1. gathers of this tile, row ends, and the apply operands of its rows
+    T val[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int i = tid + k * BLOCK;
+      val[k] = i < ne ? ld_gather(x + u[k]) : (T)0;
+    }
+    uint32_t rod[ITEMS];
+    T rx[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int i = tid + k * BLOCK;
+      if (i < nr) {
+        s_end[i] = (uint32_t)(ld_stream(col_ptr + r0 + i + 1) - e0);
+        rod[k] = od[r0 + i];
+        rx[k] = x[r0 + i];
+      } else {
+        rod[k] = 0u;
+        rx[k] = (T)0;
       }
+    }
+    const bool continued = col_ptr[r0] < e0;  // first row began in an earlier tile
+    // prefetch the next tile
+    const uint64_t tn = t + gridDim.x;
+    uint32_t nr0 = r0, nr1 = r1;
+    uint64_t ne0 = e0, ne1 = e1;
+    if (tn < T_tiles) {
+      nr0 = tile_row[tn];
+      nr1 = tile_row[tn + 1];
+      ne0 = tile_edge[tn];
+      ne1 = tile_edge[tn + 1];
     }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int i = tid + k * BLOCK;
-      if (i < nr) s_end[i] = (uint32_t)(col_ptr[r0 + i + 1] - e0);
+      if (i < ne) {
+        T v = val[k];
+        if (WEIGHTED) v = v * (T)w[k];
+        s_val[i] = v;
+      }
+    }
+    if (tn < T_tiles) {
+      const int nne = (int)(ne1 - ne0);
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const int i = tid + k * BLOCK;
+        u[k] = i < nne ? ld_stream(row_idx + ne0 + i) : 0u;
+        w[k] = (WEIGHTED && i < nne) ? ld_stream(wn + ne0 + i) : 1.0f;
+      }
     }
     __syncthreads();
     // 2. merge-path walk over this thread's ITEMS items
@@ -177,8 +286,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_tiles(
           head_row = xr;
           head_acc = acc;
         } else {
-          // row started and ended inside this thread: final
-          delta += apply_row<T>(a, final_iter, r0 + xr, (double)acc, x, y, od, rsum);
+          s_sum[xr] = (double)acc;  // row started and ended inside this thread
         }
         acc = (T)0;
         ++xr;
@@ -204,36 +312,34 @@ __global__ __launch_bounds__(BLOCK) void k_pull_tiles(
       s_wflag[wave] = sf;
     }
     __syncthreads();
-    // prefix of the waves before this one
     cpair<T> pv{(T)0, (T)0};
-    int pf = 0;
-    for (int w = 0; w < wave; ++w) {
-      if (s_wflag[w]) {
-        pv = s_wtot[w];
-        pf = 1;
-      } else {
-        pv = cadd(pv, s_wtot[w]);
-      }
+    for (int wv = 0; wv < wave; ++wv) {
+      if (s_wflag[wv]) pv = s_wtot[wv];
+      else pv = cadd(pv, s_wtot[wv]);
     }
     cpair<T> inc = sv;
     if (!sf) inc = cadd(pv, sv);
-    // exclusive value = inclusive of the previous thread
     cpair<T> ex{shfl_up_t(inc.hi, 1), shfl_up_t(inc.lo, 1)};
     if (lane == 0) ex = pv;
     if (tid == 0) ex = cpair<T>{(T)0, (T)0};
-    (void)pf;
     if (has_end) {
       const cpair<T> full = cadd(ex, cpair<T>{head_acc, (T)0});
-      const bool continued = (head_row == 0) && (col_ptr[r0] < e0);
-      if (continued) {
-        tile_head[t] = full;
-      } else {
-        delta += apply_row<T>(a, final_iter, r0 + head_row, (double)full.hi + (double)full.lo, x, y,
-                              od, rsum);
-      }
+      if (head_row == 0 && continued) tile_head[t] = full;
+      else s_sum[head_row] = (double)full.hi + (double)full.lo;
     }
     if (tid == BLOCK - 1) tile_carry[t] = inc;
     __syncthreads();
+    // 4. apply, one coalesced pass over the tile's rows
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int i = tid + k * BLOCK;
+      if (i < nr && !(i == 0 && continued))
+        delta += apply_vals<T>(a, final_iter, r0 + i, s_sum[i], rx[k], rod[k], y, rsum);
+    }
+    r0 = nr0;
+    r1 = nr1;
+    e0 = ne0;
+    e1 = ne1;
   }
   block_reduce2<BLOCK>(delta, rsum, s_red);
   if (threadIdx.x == 0) {

@@ -116,6 +116,17 @@ struct seed_set {
   }
 };
 
+// Device state (FP32 or FP64, internal order) -> host FP64 in input id order.
+static void download_state(pe_graph_s* g, int prec, const void* dev, double* out) {
+  if (prec == 64) {
+    download_external(g, dev, out, 8);
+  } else {
+    std::vector<float> tmp(g->V);
+    download_external(g, dev, tmp.data(), 4);
+    for (uint32_t v = 0; v < g->V; ++v) out[v] = tmp[v];
+  }
+}
+
 // Runs the reference loop on the device (pagerank.hpp:100-146 and variants).
 struct rank_run {
   pe_graph_s* g;
@@ -222,14 +233,7 @@ struct rank_run {
     pe_ctx_s* ctx = g->ctx;
     hipStream_t s = ctx->stream;
     void* res = host_ctl.cur ? b1.p : b0.p;
-    if (prec == 64) {
-      PE_HIP(hipMemcpyAsync(out, res, (uint64_t)g->V * 8, hipMemcpyDeviceToHost, s));
-    } else {
-      std::vector<float> tmp(g->V);
-      PE_HIP(hipMemcpyAsync(tmp.data(), res, (uint64_t)g->V * 4, hipMemcpyDeviceToHost, s));
-      PE_HIP(hipStreamSynchronize(s));
-      for (uint32_t v = 0; v < g->V; ++v) out[v] = tmp[v];
-    }
+    download_state(g, prec, res, out);
     if (host_deltas && host_ctl.executed)
       PE_HIP(hipMemcpyAsync(host_deltas, deltas.p,
                             std::min(host_ctl.executed, host_ctl.cap) * sizeof(double),
@@ -363,8 +367,11 @@ int pe_ppr(pe_graph_t* g, const uint32_t* seeds, uint32_t nseeds, const pe_rank_
   if (!g || !opts || !out || !seeds || nseeds == 0) return PE_ERR_INVALID;
   return guarded(g->ctx, [&] {
     PE_REQUIRE(!opts->weighted, PE_ERR_INVALID, "PPR is unweighted");
+    std::vector<uint32_t> internal(nseeds);
+    for (uint32_t i = 0; i < nseeds; ++i) PE_REQUIRE(seeds[i] < g->V, PE_ERR_RANGE, "seed vertex >= V");
+    to_internal_ids(g, seeds, nseeds, internal.data());
     seed_set ss;
-    ss.build(g->ctx, g->V, seeds, nseeds);
+    ss.build(g->ctx, g->V, internal.data(), nseeds);
     const double init_dist = 1.0 / ss.count;
     rank_run run(g, opts->precision ? opts->precision : 32, false);
     run.init_state(ss.words.p, init_dist, 1);
@@ -388,8 +395,10 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
     PE_REQUIRE(prec == 32 || prec == 64, PE_ERR_INVALID, "precision must be 32 or 64");
     const size_t es = prec == 64 ? 8 : 4;
     const uint32_t V = g->V;
+    uint32_t isrc = src;
+    to_internal_ids(g, &src, 1, &isrc);
     seed_set ss;
-    ss.build(ctx, V, &src, 1);
+    ss.build(ctx, V, &isrc, 1);
     dbuf<uint8_t> b0(ctx, (uint64_t)V * es), b1(ctx, (uint64_t)V * es);
     const uint64_t nwords = ((uint64_t)V >> 6) + 1;
     dbuf<uint64_t> active(ctx, nwords), newly(ctx, nwords);
@@ -445,15 +454,7 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
     PE_HIP(hipEventSynchronize(ctx->t1));
     float ms = 0.f;
     PE_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
-    if (prec == 64) {
-      PE_HIP(hipMemcpyAsync(out, x, (uint64_t)V * 8, hipMemcpyDeviceToHost, s));
-    } else {
-      std::vector<float> tmp(V);
-      PE_HIP(hipMemcpyAsync(tmp.data(), x, (uint64_t)V * 4, hipMemcpyDeviceToHost, s));
-      PE_HIP(hipStreamSynchronize(s));
-      for (uint32_t v = 0; v < V; ++v) out[v] = tmp[v];
-    }
-    PE_HIP(hipStreamSynchronize(s));
+    download_state(g, prec, x, out);
     if (stats) {
       stats->iterations_run = executed;
       stats->last_delta = last;

@@ -8,6 +8,7 @@ the reference's own; every call goes to libplato_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -30,9 +31,16 @@ class Context:
         if rc != L.PE_OK:
             raise L.PlatoHipError(rc, "pe_ctx_create failed (no GPU?)")
         self.h = h
+        # device objects created on this context; destroyed before the context
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(self._children):
+                child.close()
             self.lib.pe_ctx_destroy(self.h)
             self.h = None
 
@@ -104,14 +112,18 @@ class Graph:
         self.h = handle
         self.info = L.GraphInfo()
         check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), ctx.h)
+        ctx._adopt(self)
 
     @classmethod
-    def from_edges(cls, ctx, src, dst, V, weights=None, directed=True, build_csr=False):
+    def from_edges(cls, ctx, src, dst, V, weights=None, directed=True, build_csr=False,
+                   relabel=False):
         src = np.ascontiguousarray(src, np.uint32)
         dst = np.ascontiguousarray(dst, np.uint32)
         flags = 0 if directed else L.PE_GRAPH_SYMMETRIZE
         if build_csr:
             flags |= L.PE_GRAPH_BUILD_CSR
+        if relabel:
+            flags |= L.PE_GRAPH_RELABEL_DEGREE
         w = None
         if weights is not None:
             w = np.ascontiguousarray(weights, np.float32)
@@ -123,8 +135,10 @@ class Graph:
 
     @classmethod
     def rmat(cls, ctx, scale, edge_factor=16, seed=1, a=0.57, b=0.19, c=0.19, directed=True,
-             build_csr=False, weighted=False):
+             build_csr=False, weighted=False, relabel=False):
         flags = 0 if directed else L.PE_GRAPH_SYMMETRIZE
+        if relabel:
+            flags |= L.PE_GRAPH_RELABEL_DEGREE
         if build_csr:
             flags |= L.PE_GRAPH_BUILD_CSR
         if weighted:
@@ -136,7 +150,8 @@ class Graph:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.pe_graph_destroy(self.h)
+            if self.ctx.h:
+                self.lib.pe_graph_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -161,6 +176,12 @@ class Graph:
     def graph_info(self) -> graph_info_t:
         return graph_info_t(bool(self.info.is_directed), self.info.vertices, self.info.edges,
                             self.info.max_v_i)
+
+    def permutation(self):
+        """rank_of[input id] = internal id."""
+        r = np.zeros(self.V, np.uint32)
+        check(self.lib.pe_graph_permutation(self.h, r.ctypes.data), self.ctx.h)
+        return r
 
     def set_tile(self, items):
         check(self.lib.pe_graph_set_tile(self.h, items), self.ctx.h)
@@ -202,10 +223,12 @@ class DenseState:
         h = C.c_void_p()
         check(self.lib.pe_vec_create(ctx.h, self._DT[self.dtype], self.n, C.byref(h)), ctx.h)
         self.h = h
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.pe_vec_destroy(self.h)
+            if self.ctx.h:
+                self.lib.pe_vec_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -216,6 +239,9 @@ class DenseState:
 
     def fill(self, value):
         check(self.lib.pe_vec_fill(self.h, float(value)), self.ctx.h)
+
+    def copy_from(self, other: "DenseState"):
+        check(self.lib.pe_vec_copy(self.h, other.h), self.ctx.h)
 
     def upload(self, arr):
         arr = np.ascontiguousarray(arr, self.dtype)
@@ -249,10 +275,12 @@ class Bitmap:
         h = C.c_void_p()
         check(self.lib.pe_bitmap_create(ctx.h, self.size, C.byref(h)), ctx.h)
         self.h = h
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.pe_bitmap_destroy(self.h)
+            if self.ctx.h:
+                self.lib.pe_bitmap_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -71,3 +71,16 @@ def test_bfs_isolated_root(ctx):
     res = breadth_first_search(g, bfs_opts_t(5))
     assert res.visited == 1 and res.epochs == 1 and res.levels[5] == 0
     assert (np.delete(res.levels, 5) == -1).all()
+
+
+def test_bfs_relabeled_graph(ctx):
+    from plato_amd import Graph
+    from plato_amd.algo import bfs_opts_t, breadth_first_search
+    g = Graph.rmat(ctx, 13, 16, seed=77, directed=False, relabel=True)
+    s, d = oracle.rmat_edges(13, 16, 77)
+    og = oracle.CSC(1 << 13, s, d, symmetrize=True)
+    for root in np.flatnonzero(og.od > 0)[[0, 50, 999]]:
+        ref = oracle.bfs(og, int(root))
+        res = breadth_first_search(g, bfs_opts_t(int(root)))
+        assert np.array_equal(res.levels, ref["levels"])
+        assert np.array_equal(res.modes, ref["modes"]) and res.visited == ref["visited"]

@@ -133,6 +133,14 @@ def test_dense_state_ops(ctx):
     y = DenseState(ctx, 1000, np.float64)
     y.upload(np.arange(1000, dtype=np.float64))
     assert y.sum() == 499500.0 and np.array_equal(y.to_f64(), np.arange(1000.0))
+    z = DenseState(ctx, 1000, np.float64)
+    z.copy_from(y)
+    assert np.array_equal(z.numpy(), np.arange(1000.0))
+    odd = DenseState(ctx, 1001, np.float32)      # length not a multiple of 16 bytes
+    src = DenseState(ctx, 1001, np.float32)
+    src.upload(np.arange(1001, dtype=np.float32))
+    odd.copy_from(src)
+    assert np.array_equal(odd.numpy(), np.arange(1001, dtype=np.float32))
 
 
 def test_active_edges_and_mode_choice(ctx):

@@ -171,3 +171,41 @@ def test_aggregate_message_known_answer(ctx):
     eng.pull_sum(x, y)
     vals = y.numpy()
     assert vals.sum() == 9 and vals[0] == 0 and (vals[1:] == 1).all()
+
+
+def test_relabeled_graph_gives_same_ranks(ctx, s16):
+    """Degree relabeling is internal: results come back in input id order."""
+    from plato_amd import Graph
+    from plato_amd.algo import (multi_seed_ppr, pagerank, pagerank_opts_t,
+                                personalized_pagerank_opts_t, personalized_pagerank_pushpull)
+    _, og = s16
+    g = Graph.rmat(ctx, 16, 16, seed=1, build_csr=True, relabel=True)
+    assert g.info.relabeled == 1
+    rank_of = g.permutation()
+    assert np.array_equal(np.sort(rank_of), np.arange(og.V, dtype=np.uint32))
+    # internal ids ordered by out-degree, descending
+    od_internal = np.zeros(og.V, np.uint32)
+    od_internal[rank_of] = og.od
+    assert (np.diff(od_internal.astype(np.int64)) <= 0).all()
+    # internal CSC is the oracle CSC of the renumbered edge list
+    s, d = oracle.rmat_edges(16, 16, 1)
+    orl = oracle.CSC(og.V, rank_of[s], rank_of[d])
+    cp, ri = g.download_csc()
+    assert np.array_equal(cp, orl.col_ptr) and np.array_equal(ri, orl.row_idx)
+    assert np.array_equal(g.out_degrees(), og.od)
+    gi = g.graph_info()
+    present = (og.od > 0) | (og.in_degrees() > 0)
+    assert gi.max_v_i_ == int(np.flatnonzero(present).max()) and gi.vertices_ == int(present.sum())
+    ref, _ = oracle.pagerank(og, 20, D, 0.0)
+    res = pagerank(g, pagerank_opts_t(20, D, 0.0), precision=64)
+    assert linf_rel(res.rank, ref) <= 1e-12
+    res32 = pagerank(g, pagerank_opts_t(20, D, 0.0), precision=32)
+    assert l1n(res32.rank, ref) <= 2e-7
+    seeds = [5, 900, 31000]
+    refm, _ = oracle.ppr(og, seeds, 40, D, 0.0)
+    rm = multi_seed_ppr(g, seeds, 40, D, 0.0, precision=64)
+    assert float(np.abs(rm.rank - refm).sum()) <= 1e-13
+    src = int(np.flatnonzero((og.od > 0) & (og.od < 4))[3])
+    refp, _, modes = oracle.ppr_pushpull(og, src, 30, D, 0.0)
+    rp = personalized_pagerank_pushpull(g, personalized_pagerank_opts_t(30, D, src, 0.0), precision=64)
+    assert np.array_equal(rp.modes, modes) and float(np.abs(rp.rank - refp).sum()) <= 1e-13
