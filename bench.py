@@ -101,6 +101,90 @@ def run_bfs(ctx, scale, roots, seed=11):
     return out
 
 
+def ppr_seeds(od, rng, n=64):
+    """Half from the top 1% by out-degree, half with 1 <= od <= median (uneven convergence)."""
+    cand = np.flatnonzero(od > 0)
+    order = cand[np.argsort(-od[cand], kind="stable")]
+    hi = order[: max(1, cand.size // 100)]
+    lo = cand[od[cand] <= np.median(od[cand])]
+    return [int(rng.choice(hi)) if i % 2 == 0 else int(rng.choice(lo)) for i in range(n)]
+
+
+def run_batched_ppr(ctx, scale, tol=1e-7, max_iters=200, seed=24):
+    """64 PPR queries as one 64-column block vs the same queries one at a time."""
+    from plato_amd import Graph
+    from plato_amd.algo import (multi_seed_ppr, personalized_pagerank_opts_t,
+                                personalized_pagerank_pushpull)
+    from plato_amd.batch import PprBatch
+    g = Graph.rmat(ctx, scale, 16, seed=seed, relabel=True, build_csr=True)
+    od = g.out_degrees()
+    seeds = ppr_seeds(od, np.random.default_rng(seed))
+    b = PprBatch(g, 0.85, tol, max_iters)
+    # warm-up: one query through the captured iteration graph
+    b.admit(0, [seeds[0]])
+    b.run_until_done([0], chunk=4)
+    b.release(0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for slot, sd in enumerate(seeds):
+        b.admit(slot, [sd])
+    passes = b.run_until_done(range(64), chunk=4)
+    ctx.sync()
+    t_batch = time.perf_counter() - t0
+    _, iters, _ = b.poll()
+    sample = [0, 1, 2, 3, 32, 33, 62, 63]
+    batch_ranks = b.results(sample)
+    b.close()
+    # the plain one-seed GPU engine: pull PPR, same tolerance rule, one query at a time
+    t_single, single_iters, l1_single = 0.0, [], []
+    ref = {}
+    for j, sd in enumerate(seeds):
+        t0 = time.perf_counter()
+        r = multi_seed_ppr(g, [sd], max_iters, 0.85, tol, precision=32)
+        t_single += time.perf_counter() - t0
+        single_iters.append(r.iterations)
+        if j in sample:
+            ref[j] = multi_seed_ppr(g, [sd], 1000, 0.85, 0.0, precision=64, l1_norm_tol=1e-13).rank
+            l1_single.append(float(np.abs(r.rank - ref[j]).sum()))
+    l1_batch = [float(np.abs(batch_ranks[i] - ref[j]).sum()) for i, j in enumerate(sample)]
+    # push/pull single-seed variant on 8 of the seeds
+    t_pp = 0.0
+    for sd in seeds[:8]:
+        t0 = time.perf_counter()
+        personalized_pagerank_pushpull(g, personalized_pagerank_opts_t(max_iters, 0.85, sd, tol))
+        t_pp += time.perf_counter() - t0
+    g.close()
+    return {"scale": scale, "k": 64, "tol": tol, "batched_qps": 64 / t_batch,
+            "batched_seconds": t_batch, "passes": passes, "single_seed_qps": 64 / t_single,
+            "speedup_vs_single": t_single / t_batch, "pushpull_single_qps": 8 / t_pp,
+            "mean_iterations": float(np.mean(iters)), "max_iterations": int(np.max(iters)),
+            "single_mean_iterations": float(np.mean(single_iters)),
+            "max_l1_vs_fp64_batched": max(l1_batch), "max_l1_vs_fp64_single": max(l1_single),
+            "l1_sample_columns": len(sample)}
+
+
+def run_serving(ctx, scale, qps_list, n_queries, seed=25):
+    """Poisson request stream against a resident graph, 1-256 seeds per query, top-100."""
+    from plato_amd import Graph
+    from plato_amd.serve import Server, poisson_trace
+    g = Graph.rmat(ctx, scale, 16, seed=seed, relabel=True)
+    cand = np.flatnonzero(g.out_degrees() > 0)
+    out = []
+    for qps in qps_list:
+        rng = np.random.default_rng(seed + int(qps))
+        arrivals, queries = poisson_trace(rng, n_queries, qps, cand)
+        srv = Server(g, 0.85, 1e-7, 200, topk=100, iters_per_pass=2, max_batches=4,
+                     hbm_reserve_bytes=16 << 30)
+        res = srv.run_trace(arrivals, queries)
+        srv.close()
+        row = {"offered_qps": qps}
+        row.update(res.summary())
+        out.append(row)
+        log(f"[bench] serving qps={qps}: {row}")
+    g.close()
+    return {"scale": scale, "queries_per_point": n_queries, "points": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +202,12 @@ def main():
     ap.add_argument("--bfs-scale", type=int, default=24)
     ap.add_argument("--bfs-roots", type=int, default=64)
     ap.add_argument("--no-bfs", action="store_true")
+    ap.add_argument("--ppr-scale", type=int, default=24)
+    ap.add_argument("--no-ppr", action="store_true")
+    ap.add_argument("--serve-scale", type=int, default=25)
+    ap.add_argument("--serve-qps", default="4,16,64")
+    ap.add_argument("--serve-queries", type=int, default=128)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported",
@@ -222,6 +312,14 @@ def main():
     if not args.no_bfs:
         log(f"[bench] bfs scale {args.bfs_scale}, {args.bfs_roots} roots")
         out["bfs"] = run_bfs(ctx, args.bfs_scale, args.bfs_roots)
+    if not args.no_ppr:
+        log(f"[bench] batched PPR scale {args.ppr_scale}, k = 64")
+        out["ppr_batched"] = run_batched_ppr(ctx, args.ppr_scale)
+    if not args.no_serve:
+        log(f"[bench] serving scale {args.serve_scale}")
+        out["serving"] = run_serving(ctx, args.serve_scale,
+                                     [float(q) for q in args.serve_qps.split(",")],
+                                     args.serve_queries)
     ctx.close()
     print(json.dumps(out), flush=True)
 

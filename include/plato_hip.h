@@ -256,6 +256,106 @@ int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint
            uint64_t* frontier, uint64_t* active_edges, uint32_t max_epochs,
            pe_bfs_stats_t* stats);
 
+/* ===========================================================================
+ * Batched PPR.  k = 64 queries share one FP32 rank block [V][64]; each column is
+ * personalized_pagerank (plato/algo/ppr/personalized_pagerank.hpp:40-134) with
+ * the TrustRank restart over the query's seed set (trustrank.hpp:197-250) and
+ * the reference's per-run stopping rule applied per column: after an iteration
+ * with delta < tol one final iteration stores the undivided rank
+ * (pagerank.hpp:137-139).  Columns retire and are refilled independently:
+ * slot states EMPTY 0, LOADING 1, ACTIVE 2, FINAL 3, DONE 4.  A column's
+ * arithmetic does not depend on the other columns.
+ * ========================================================================= */
+typedef struct pe_ppr_batch_s pe_ppr_batch_t;
+typedef struct {
+  uint32_t k;           /* columns; must be 64 */
+  double damping;       /* 1 - alpha */
+  double tol;           /* per-column delta threshold (the reference's eps_) */
+  uint32_t max_iters;   /* iteration cap per column (the reference's iteration_) */
+  uint32_t task_edges;  /* edges per wave task (0: 256); longer columns are split */
+} pe_ppr_batch_opts_t;
+
+int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_batch_t** out);
+int pe_ppr_batch_destroy(pe_ppr_batch_t* b);
+/* device bytes held by the batch (rank blocks, seed masks, task lists) */
+int pe_ppr_batch_bytes(const pe_ppr_batch_t* b, uint64_t* bytes);
+/* put a query (seed set, input ids) into an EMPTY slot; it starts LOADING */
+int pe_ppr_batch_admit(pe_ppr_batch_t* b, uint32_t slot, const uint32_t* seeds, uint32_t nseeds);
+/* free a slot (any state) */
+int pe_ppr_batch_release(pe_ppr_batch_t* b, uint32_t slot);
+/* enqueue `iterations` passes (captured once as a graph and replayed; async) */
+int pe_ppr_batch_step(pe_ppr_batch_t* b, uint32_t iterations);
+/* slot states, iterations run and last delta per slot (arrays of 64; any may be NULL) */
+int pe_ppr_batch_poll(pe_ppr_batch_t* b, uint32_t* status, uint32_t* iters, double* last_delta);
+/* full rank vectors of DONE slots, input id order: ranks[n][V] (FP64) */
+int pe_ppr_batch_results(pe_ppr_batch_t* b, const uint32_t* slots, uint32_t n, double* ranks);
+/* top-k (rank desc, id asc; trustrank.hpp:109-112) of DONE slots: ids/vals [n][k] */
+int pe_ppr_batch_topk(pe_ppr_batch_t* b, const uint32_t* slots, uint32_t n, uint32_t k, uint32_t* ids,
+                      float* vals);
+
+/* ===========================================================================
+ * PPR serving loop: continuous batching of queries into 64-column blocks
+ * (pe_ppr_batch), with admission by free HBM.  No reference counterpart: the
+ * reference runs one PPR per process launch (example/personalized_pagerank.cc);
+ * each query here has that query's semantics (see pe_ppr_batch).
+ * ========================================================================= */
+typedef struct pe_server_s pe_server_t;
+typedef struct {
+  double damping;            /* 1 - alpha */
+  double tol;                /* per-query delta threshold */
+  uint32_t max_iters;        /* per-query iteration cap */
+  uint32_t topk;             /* vertices returned per query (rank desc, id asc) */
+  uint32_t iters_per_pass;   /* iterations enqueued per block between slot checks */
+  uint32_t max_batches;      /* upper bound on resident 64-column blocks */
+  uint64_t hbm_reserve_bytes;/* free HBM kept back when opening another block */
+} pe_server_opts_t;
+typedef struct {
+  uint64_t id;
+  double arrival_s;          /* seconds since the loop started */
+  double admit_s;            /* when the query got a column slot */
+  double done_s;             /* when its result was read out */
+  uint32_t iterations;
+  uint32_t batch;            /* block and slot that ran it */
+  uint32_t slot;
+} pe_request_stats_t;
+
+int pe_server_create(pe_graph_t* g, const pe_server_opts_t* opts, pe_server_t** out);
+int pe_server_destroy(pe_server_t* s);
+/* queue a query (seed set in input ids) that arrives at arrival_s */
+int pe_server_submit(pe_server_t* s, uint64_t id, const uint32_t* seeds, uint32_t nseeds, double arrival_s);
+/* one pass: admit arrived queries, run iterations on busy blocks, harvest finished columns */
+int pe_server_pump(pe_server_t* s, uint32_t* completed);
+/* move up to max finished queries out: stats[max], topk ids/vals [max][topk] */
+int pe_server_take(pe_server_t* s, uint32_t max, pe_request_stats_t* stats, uint32_t* topk_ids,
+                   float* topk_vals, uint32_t* taken);
+int pe_server_info(pe_server_t* s, uint32_t* batches, uint32_t* in_flight, uint32_t* waiting,
+                   uint64_t* passes);
+/* replay an arrival trace in real time until every query is answered; seeds of
+ * query i are seeds[seed_offsets[i] .. seed_offsets[i+1]); outputs in query order */
+int pe_server_run_trace(pe_server_t* s, uint32_t n, const uint64_t* ids, const double* arrivals,
+                        const uint64_t* seed_offsets, const uint32_t* seeds, pe_request_stats_t* stats,
+                        uint32_t* topk_ids, float* topk_vals, double* wall_s);
+
+/* ===========================================================================
+ * BFS engine step: one epoch of breadth_first_search's 5-argument
+ * foreach_edges (plato/algo/bfs/bfs.hpp:95-138) with the BFS operators: push
+ * claims unvisited out-neighbours of the frontier, pull gives each unvisited
+ * vertex its first in-neighbour found in the frontier.  The state holds the
+ * levels (the reference keeps parents; levels are the order-free part).
+ * Bitmaps are in internal id order; roots and levels use input ids.
+ * ========================================================================= */
+typedef struct pe_bfs_s pe_bfs_t;
+int pe_bfs_create(pe_graph_t* g, pe_bfs_t** out);
+int pe_bfs_destroy(pe_bfs_t* b);
+/* levels = -1 except root = 0; frontier = {root} */
+int pe_bfs_reset(pe_bfs_t* b, uint32_t root, pe_bitmap_t* frontier);
+/* one epoch from `current` into `next` (overwritten); push != 0 selects push.
+ * *activated = newly reached vertices (the reference's slot-return sum),
+ * *next_active_edges = sum of od over them (the next decision's A). */
+int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, uint64_t* activated,
+                uint64_t* next_active_edges);
+int pe_bfs_get_levels(pe_bfs_t* b, int32_t* levels);
+
 #ifdef __cplusplus
 }
 #endif

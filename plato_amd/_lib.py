@@ -53,6 +53,23 @@ class BfsStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class PprBatchOpts(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("damping", C.c_double), ("tol", C.c_double),
+                ("max_iters", C.c_uint32), ("task_edges", C.c_uint32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("damping", C.c_double), ("tol", C.c_double), ("max_iters", C.c_uint32),
+                ("topk", C.c_uint32), ("iters_per_pass", C.c_uint32), ("max_batches", C.c_uint32),
+                ("hbm_reserve_bytes", C.c_uint64)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("arrival_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("iterations", C.c_uint32), ("batch", C.c_uint32),
+                ("slot", C.c_uint32)]
+
+
 _vp = C.c_void_p
 _u32, _u64, _i32, _dbl = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 _P = C.POINTER
@@ -111,6 +128,27 @@ SIGNATURES = {
     "pe_ppr": [_vp, _vp, _u32, _P(RankOpts), _vp, _vp, _P(RankStats)],
     "pe_ppr_pushpull": [_vp, _u32, _P(RankOpts), _vp, _vp, _vp, _P(RankStats)],
     "pe_bfs": [_vp, _u32, _dbl, _vp, _vp, _vp, _vp, _u32, _P(BfsStats)],
+    "pe_bfs_create": [_vp, _P(_vp)],
+    "pe_bfs_destroy": [_vp],
+    "pe_bfs_reset": [_vp, _u32, _vp],
+    "pe_bfs_step": [_vp, _i32, _vp, _vp, _P(_u64), _P(_u64)],
+    "pe_bfs_get_levels": [_vp, _vp],
+    "pe_ppr_batch_create": [_vp, _P(PprBatchOpts), _P(_vp)],
+    "pe_ppr_batch_destroy": [_vp],
+    "pe_ppr_batch_bytes": [_vp, _P(_u64)],
+    "pe_ppr_batch_admit": [_vp, _u32, _vp, _u32],
+    "pe_ppr_batch_release": [_vp, _u32],
+    "pe_ppr_batch_step": [_vp, _u32],
+    "pe_ppr_batch_poll": [_vp, _vp, _vp, _vp],
+    "pe_ppr_batch_results": [_vp, _vp, _u32, _vp],
+    "pe_ppr_batch_topk": [_vp, _vp, _u32, _u32, _vp, _vp],
+    "pe_server_create": [_vp, _P(ServerOpts), _P(_vp)],
+    "pe_server_destroy": [_vp],
+    "pe_server_submit": [_vp, _u64, _vp, _u32, _dbl],
+    "pe_server_pump": [_vp, _P(_u32)],
+    "pe_server_take": [_vp, _u32, _vp, _vp, _vp, _P(_u32)],
+    "pe_server_info": [_vp, _P(_u32), _P(_u32), _P(_u32), _P(_u64)],
+    "pe_server_run_trace": [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _P(_dbl)],
 }
 _RESTYPE = {"pe_ctx_last_error": C.c_char_p}
 

@@ -18,6 +18,7 @@
 // Both steps produce the next frontier as a queue and as a bitmap, and count
 // its size and active edges on the device, so the next decision needs no scan.
 #include <algorithm>
+#include <memory>
 
 #include "pe_internal.hpp"
 
@@ -77,13 +78,15 @@ __device__ __forceinline__ unsigned long long wave_append(uint64_t mask,
 }
 
 __global__ __launch_bounds__(256) void k_bfs_push(
-    const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
+    const uint32_t* __restrict__ queue, uint64_t n_host, const unsigned long long* __restrict__ n_dev,
+    const uint64_t* __restrict__ ptr,
     const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
     uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
     bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t n = n_dev ? (uint64_t)*n_dev : n_host;
   unsigned long long active = 0;
   for (uint64_t q = wave_id; q < n; q += nwaves) {
     const uint32_t v = queue[q];
@@ -187,6 +190,38 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
 }
 
+// Frontier bitmap -> queue: one lane per 64-bit word, a wave covers 64 words;
+// the wave's popcounts are prefix-summed with shuffles, one atomic reserves the
+// wave's slice of the queue, and each lane writes its word's ids.
+__global__ __launch_bounds__(256) void k_frontier_compact(const uint64_t* __restrict__ words, uint64_t nwords,
+                                                          uint32_t V, uint32_t* __restrict__ queue,
+                                                          unsigned long long* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6) * 64; base < nwords;
+       base += ((uint64_t)gridDim.x * blockDim.x)) {
+    const uint64_t wi = base + lane;
+    uint64_t w = wi < nwords ? words[wi] : 0ull;
+    if (wi == nwords - 1 && (V & 63)) w &= (1ull << (V & 63)) - 1ull;  // bits >= V are not vertices
+    if (wi >= (uint64_t)(V >> 6) + ((V & 63) ? 1 : 0)) w = 0ull;
+    const unsigned c = __popcll(w);
+    unsigned incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const unsigned total = __shfl(incl, 63, 64);
+    unsigned long long start = 0;
+    if (lane == 0 && total) start = atomicAdd(count, (unsigned long long)total);
+    start = __shfl(start, 0, 64);
+    uint64_t pos = start + incl - c;
+    while (w) {
+      const int b = __ffsll((unsigned long long)w) - 1;
+      queue[pos++] = (uint32_t)(wi * 64 + b);
+      w &= w - 1;
+    }
+  }
+}
+
 __global__ void k_reached_edges(const uint32_t* __restrict__ level, const uint32_t* __restrict__ od,
                                 uint32_t V, unsigned long long* __restrict__ out,
                                 unsigned int* __restrict__ visited) {
@@ -265,7 +300,7 @@ struct bfs_state {
         k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(bn, nwords);
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
         const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
-        k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
+        k_bfs_push<<<grid, 256, 0, s>>>(qc, n, nullptr, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
                                         g->od.p);
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
@@ -298,6 +333,98 @@ struct bfs_state {
 using namespace pe;
 
 extern "C" {
+
+}  // extern "C"
+
+struct pe_bfs_s {
+  pe_graph_s* g;
+  std::unique_ptr<bfs_state> st;
+  uint32_t epoch = 0;
+};
+
+extern "C" {
+
+int pe_bfs_create(pe_graph_t* g, pe_bfs_t** out) {
+  if (!g || !out) return PE_ERR_INVALID;
+  *out = nullptr;
+  return guarded(g->ctx, [&] {
+    auto b = std::make_unique<pe_bfs_s>();
+    b->g = g;
+    b->st = std::make_unique<bfs_state>(g);
+    *out = b.release();
+  });
+}
+
+int pe_bfs_destroy(pe_bfs_t* b) {
+  if (!b) return PE_ERR_INVALID;
+  (void)hipStreamSynchronize(b->g->ctx->stream);
+  delete b;
+  return PE_OK;
+}
+
+int pe_bfs_reset(pe_bfs_t* b, uint32_t root, pe_bitmap_t* frontier) {
+  if (!b || !frontier) return PE_ERR_INVALID;
+  pe_graph_s* g = b->g;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(root < g->V, PE_ERR_RANGE, "root >= V");
+    PE_REQUIRE(frontier->bits >= g->V, PE_ERR_INVALID, "bitmap smaller than the vertex set");
+    uint32_t iroot = root;
+    to_internal_ids(g, &root, 1, &iroot);
+    hipStream_t s = g->ctx->stream;
+    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(b->st->level.p, g->V, iroot);
+    k_zero_words<<<grid_for(frontier->words, 256, 4096), 256, 0, s>>>(frontier->p, frontier->words);
+    k_bfs_seed<<<1, 1, 0, s>>>(b->st->q0.p, frontier->p, iroot, b->st->cnt.p, g->od.p);
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipStreamSynchronize(s));
+    b->epoch = 0;
+  });
+}
+
+int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, uint64_t* activated,
+                uint64_t* next_active_edges) {
+  if (!b || !current || !next) return PE_ERR_INVALID;
+  pe_graph_s* g = b->g;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(current->bits >= g->V && next->bits >= g->V, PE_ERR_INVALID, "bitmap smaller than the vertex set");
+    PE_REQUIRE(current != next, PE_ERR_INVALID, "current and next must differ");
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    bfs_state& st = *b->st;
+    const uint32_t L = b->epoch + 1;
+    const uint64_t nwords = ((uint64_t)g->V >> 6) + 1;
+    k_bfs_reset<<<1, 1, 0, s>>>(st.cnt.p);
+    if (push) {
+      PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
+      // compact the frontier bitmap into a queue (count in heavy_count, reused as scratch)
+      k_frontier_compact<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(current->p, nwords, g->V, st.q0.p,
+                                                                     &st.cnt.p->heavy_count);
+      k_zero_words<<<grid_for(next->words, 256, 4096), 256, 0, s>>>(next->p, next->words);
+      ktime_scope kt(ctx, PE_KT_BFS_PUSH);
+      k_bfs_push<<<persistent_grid(ctx, 1u << 20, 8), 256, 0, s>>>(
+          st.q0.p, 0, &st.cnt.p->heavy_count, g->csr_ptr(), g->csr_idx(), st.level.p, L, st.q1.p, next->p,
+          st.cnt.p, g->od.p);
+    } else {
+      ktime_scope kt(ctx, PE_KT_BFS_PULL);
+      k_bfs_pull<<<grid_for(nwords * 64, 256, 1u << 20), 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords,
+                                                                       st.level.p, current->p, L, next->p, st.q1.p,
+                                                                       st.heavy.p, st.cnt.p, g->od.p);
+      k_bfs_pull_heavy<<<persistent_grid(ctx, 1u << 20, 4), 256, 0, s>>>(
+          g->col_ptr.p, g->row_idx.p, st.level.p, current->p, L, next->p, st.q1.p, st.heavy.p, st.cnt.p, g->od.p);
+    }
+    PE_HIP(hipGetLastError());
+    bfs_counters h{};
+    PE_HIP(hipMemcpyAsync(&h, st.cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    if (activated) *activated = h.next_count;
+    if (next_active_edges) *next_active_edges = h.next_active;
+    b->epoch += 1;
+  });
+}
+
+int pe_bfs_get_levels(pe_bfs_t* b, int32_t* levels) {
+  if (!b || !levels) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] { download_external(b->g, b->st->level.p, levels, 4); });
+}
 
 int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint8_t* modes,
            uint64_t* frontier, uint64_t* active_edges, uint32_t max_epochs, pe_bfs_stats_t* stats) {

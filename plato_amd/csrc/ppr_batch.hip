@@ -1,0 +1,674 @@
+// Batched Personalized PageRank: k = 64 queries as the 64 columns of one
+// FP32 rank block X[V][64] (row-major, 256 B per vertex).
+//
+// Each query is plato::algo::personalized_pagerank with its restart mass spread
+// over a seed set (personalized_pagerank.hpp:40-134 for one seed, the TrustRank
+// restart of trustrank.hpp:197-250 for several), with the reference's stopping
+// rule per column: after a non-final iteration with delta < tol, one more, final,
+// iteration stores the undivided rank (pagerank.hpp:137-139).
+//
+// Pull gather, one wave per task, lane j = column j: every in-edge u->v reads the
+// 256 B row X[u][0..63] once for all 64 queries.  A task is either a run of
+// consecutive columns (vertices) holding at most C edges, or one chunk of a long
+// column; chunk partials are combined in chunk order by the fix-up kernel.
+// Sums are Kahan-compensated per lane in ascending source order, so a column's
+// arithmetic never depends on the other 63 columns: a query gives the same bits
+// alone or in any mix.
+//
+// Columns move through EMPTY -> LOADING -> ACTIVE -> FINAL -> DONE on the device
+// (k_batch_finish); a slot is refilled by writing its initial column during the
+// same pass that updates the others, so admission costs no extra pass.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "pe_internal.hpp"
+#include "ppr_batch.hpp"
+
+namespace pe {
+
+static constexpr int BK = 64;  // columns per block (one per lane)
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+  // Kahan, "Further remarks on reducing truncation errors", CACM 8(1), 1965
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// TwoSum of two (sum, -compensation) accumulators, in order a then b
+__device__ __forceinline__ void pair_add(float& s, float& c, float s2, float c2) {
+  const float hi = s + s2;
+  const float bp = hi - s;
+  const float lo = (s - (hi - bp)) + (s2 - bp);
+  s = hi;
+  c = c + c2 - lo;
+}
+
+struct lane_cfg {
+  uint32_t status;
+  double init_dist;  // 1/|S|
+  double restart;    // (1-d)/|S|
+};
+
+// Per-lane apply for vertex v given the gathered sum (s - c).
+__device__ __forceinline__ float batch_apply(const lane_cfg& L, double damping, double sum, float xold,
+                                             uint32_t d, bool seed, double& delta) {
+  switch (L.status) {
+    case COL_LOADING: {  // x0 = [v in S] / |S| / od (trustrank.hpp:197-206)
+      double x0 = seed ? L.init_dist : 0.0;
+      if (d > 0) x0 = x0 / (double)d;
+      return (float)x0;
+    }
+    case COL_ACTIVE: {
+      const double r = (seed ? L.restart : 0.0) + damping * sum;
+      if (d > 0) {
+        const float xn = (float)(r / (double)d);
+        delta += fabs((double)xn - (double)xold) * (double)d;
+        return xn;
+      }
+      const float xn = (float)r;
+      delta += fabs((double)xn - (double)xold);
+      return xn;
+    }
+    case COL_FINAL:  // final iteration stores the undivided rank
+      return (float)((seed ? L.restart : 0.0) + damping * sum);
+    case COL_DONE:
+      return xold;
+    default:
+      return 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_tasks(
+    const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
+    const uint32_t* __restrict__ od, const uint64_t* __restrict__ seedmask,
+    const bt_task* __restrict__ tasks, uint64_t ntasks, float* X0, float* X1,
+    const batch_ctl* __restrict__ ctl, const uint32_t* __restrict__ status,
+    const double* __restrict__ init_dist, double damping, float2* __restrict__ partial,
+    double* __restrict__ part_delta) {
+  __shared__ double s_red[4][BK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t cur = ctl->cur;
+  const float* X = cur ? X1 : X0;
+  float* Y = cur ? X0 : X1;
+  lane_cfg L;
+  L.status = status[lane];
+  L.init_dist = init_dist[lane];
+  L.restart = (1.0 - damping) * L.init_dist;
+  double delta = 0.0;
+  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t ti = wave_id; ti < ntasks; ti += nwaves) {
+    const bt_task t = tasks[ti];
+    if (t.chunk == 0xFFFFFFFFu) {
+      for (uint32_t v = t.v0; v < t.v1; ++v) {
+        const uint64_t eb = col_ptr[v], ee = col_ptr[v + 1];
+        float s = 0.f, c = 0.f;
+        uint64_t e = eb;
+        for (; e + 8 <= ee; e += 8) {
+          float vals[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) kahan_add(s, c, vals[i]);
+        }
+        for (; e < ee; ++e) kahan_add(s, c, X[(uint64_t)row_idx[e] * BK + lane]);
+        const uint64_t sm = seedmask[v];
+        const float xold = X[(uint64_t)v * BK + lane];
+        Y[(uint64_t)v * BK + lane] =
+            batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta);
+      }
+    } else {
+      float s = 0.f, c = 0.f;
+      uint64_t e = t.e0;
+      for (; e + 8 <= t.e1; e += 8) {
+        float vals[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) kahan_add(s, c, vals[i]);
+      }
+      for (; e < t.e1; ++e) kahan_add(s, c, X[(uint64_t)row_idx[e] * BK + lane]);
+      partial[(uint64_t)t.chunk * BK + lane] = make_float2(s, c);
+    }
+  }
+  s_red[wave][lane] = delta;
+  __syncthreads();
+  if (threadIdx.x < BK) {
+    double d = 0.0;
+    for (int w = 0; w < 4; ++w) d += s_red[w][threadIdx.x];
+    part_delta[(uint64_t)blockIdx.x * BK + threadIdx.x] = d;
+  }
+}
+
+// Long columns: chunk partials combined in chunk order, then applied.
+__global__ __launch_bounds__(256) void k_batch_fixup(
+    const uint32_t* __restrict__ od, const uint64_t* __restrict__ seedmask,
+    const bt_long* __restrict__ longs, uint64_t nlong, float* X0, float* X1,
+    const batch_ctl* __restrict__ ctl, const uint32_t* __restrict__ status,
+    const double* __restrict__ init_dist, double damping, const float2* __restrict__ partial,
+    double* __restrict__ part_delta) {
+  __shared__ double s_red[4][BK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t cur = ctl->cur;
+  const float* X = cur ? X1 : X0;
+  float* Y = cur ? X0 : X1;
+  lane_cfg L;
+  L.status = status[lane];
+  L.init_dist = init_dist[lane];
+  L.restart = (1.0 - damping) * L.init_dist;
+  double delta = 0.0;
+  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t li = wave_id; li < nlong; li += nwaves) {
+    const bt_long lc = longs[li];
+    float s = 0.f, c = 0.f;
+    uint32_t k = 0;
+    for (; k + 8 <= lc.nchunks; k += 8) {
+      float2 p[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) p[i] = partial[(uint64_t)(lc.c0 + k + i) * BK + lane];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) pair_add(s, c, p[i].x, p[i].y);
+    }
+    for (; k < lc.nchunks; ++k) {
+      const float2 p = partial[(uint64_t)(lc.c0 + k) * BK + lane];
+      pair_add(s, c, p.x, p.y);
+    }
+    const uint32_t v = lc.v;
+    const uint64_t sm = seedmask[v];
+    const float xold = X[(uint64_t)v * BK + lane];
+    Y[(uint64_t)v * BK + lane] =
+        batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta);
+  }
+  s_red[wave][lane] = delta;
+  __syncthreads();
+  if (threadIdx.x < BK) {
+    double d = 0.0;
+    for (int w = 0; w < 4; ++w) d += s_red[w][threadIdx.x];
+    part_delta[(uint64_t)blockIdx.x * BK + threadIdx.x] = d;
+  }
+}
+
+// Per-column delta, state transitions, buffer flip.  One block of 64 threads.
+__global__ void k_batch_finish(const double* __restrict__ part_delta, uint32_t nparts,
+                               batch_ctl* __restrict__ ctl, uint32_t* __restrict__ status,
+                               uint32_t* __restrict__ iters, double* __restrict__ last_delta,
+                               double tol, uint32_t max_iters) {
+  const int j = threadIdx.x;
+  if (j < BK) {
+    double d = 0.0;
+    for (uint32_t b = 0; b < nparts; ++b) d += part_delta[(uint64_t)b * BK + j];
+    const uint32_t st = status[j];
+    if (st == COL_LOADING) {
+      status[j] = COL_ACTIVE;
+      iters[j] = 0;
+    } else if (st == COL_ACTIVE) {
+      const uint32_t it = iters[j] + 1;
+      iters[j] = it;
+      last_delta[j] = d;
+      // delta < tol jumps to one more, final, iteration; the cap makes the
+      // max_iters-th iteration the final one
+      if (d < tol || it + 1 >= max_iters) status[j] = COL_FINAL;
+    } else if (st == COL_FINAL) {
+      iters[j] = iters[j] + 1;
+      status[j] = COL_DONE;
+    }
+  }
+  __syncthreads();
+  if (j == 0) {
+    ctl->cur ^= 1u;
+    ctl->iteration += 1;
+  }
+}
+
+__global__ void k_batch_seed_bits(uint64_t* __restrict__ seedmask, const uint32_t* __restrict__ ids,
+                                  uint32_t n, uint32_t slot, int set) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (set) atomicOr((unsigned long long*)&seedmask[ids[i]], 1ull << slot);
+    else atomicAnd((unsigned long long*)&seedmask[ids[i]], ~(1ull << slot));
+  }
+}
+
+__global__ void k_batch_set_slot(uint32_t* status, double* init_dist, uint32_t* iters, double* last_delta,
+                                 uint32_t slot, uint32_t st, double dist) {
+  status[slot] = st;
+  init_dist[slot] = dist;
+  iters[slot] = 0;
+  last_delta[slot] = 0.0;
+}
+
+// Columns of the current block -> dense [n][V] (internal order).
+__global__ void k_batch_extract(const float* X0, const float* X1, const batch_ctl* __restrict__ ctl,
+                                uint32_t V, const uint32_t* __restrict__ slots, uint32_t n,
+                                float* __restrict__ out) {
+  const float* X = ctl->cur ? X1 : X0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = 0; i < n; ++i) out[(uint64_t)i * V + v] = X[v * BK + slots[i]];
+}
+
+// Top-k support: per-slot histogram of the top 11 bits of the (non-negative)
+// float pattern, then candidates at or above the threshold bin.
+__global__ void k_batch_hist(const float* X0, const float* X1, const batch_ctl* __restrict__ ctl, uint32_t V,
+                             const uint32_t* __restrict__ slots, uint32_t n, unsigned int* __restrict__ hist) {
+  const float* X = ctl->cur ? X1 : X0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = 0; i < n; ++i) {
+      const float f = X[v * BK + slots[i]];
+      if (f > 0.f) atomicAdd(&hist[i * 2048u + (__float_as_uint(f) >> 20)], 1u);
+    }
+}
+
+__global__ void k_batch_candidates(const float* X0, const float* X1, const batch_ctl* __restrict__ ctl,
+                                   uint32_t V, const uint32_t* __restrict__ slots, uint32_t n,
+                                   const uint32_t* __restrict__ thr_bin, uint32_t cap,
+                                   unsigned int* __restrict__ count, uint32_t* __restrict__ cid,
+                                   float* __restrict__ cval) {
+  const float* X = ctl->cur ? X1 : X0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = 0; i < n; ++i) {
+      const float f = X[v * BK + slots[i]];
+      if (f > 0.f && (__float_as_uint(f) >> 20) >= thr_bin[i]) {
+        const unsigned int p = atomicAdd(&count[i], 1u);
+        if (p < cap) {
+          cid[(uint64_t)i * cap + p] = (uint32_t)v;
+          cval[(uint64_t)i * cap + p] = f;
+        }
+      }
+    }
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+// ================================================================================
+// host side
+// ================================================================================
+
+struct pe_ppr_batch_s {
+  pe_graph_s* g = nullptr;
+  pe_ppr_batch_opts_t opts{};
+  dbuf<float> X0, X1;
+  dbuf<uint64_t> seedmask;
+  dbuf<bt_task> tasks;
+  dbuf<bt_long> longs;
+  dbuf<float2> partial;
+  dbuf<double> part_delta;
+  dbuf<batch_ctl> ctl;
+  dbuf<uint32_t> status, iters;
+  dbuf<double> init_dist, last_delta;
+  uint64_t ntasks = 0, nlong = 0, nchunks = 0;
+  unsigned task_grid = 0, fix_grid = 0;
+  std::vector<std::vector<uint32_t>> slot_seeds;  // internal ids per slot
+  std::vector<uint32_t> host_status;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  uint32_t graph_iters = 0;
+
+  ~pe_ppr_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+
+  void build_tasks(uint32_t C) {
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const uint32_t V = g->V;
+    std::vector<uint64_t> cp((uint64_t)V + 1);
+    PE_HIP(hipMemcpyAsync(cp.data(), g->col_ptr.p, ((uint64_t)V + 1) * 8, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    std::vector<bt_task> t;
+    std::vector<bt_long> l;
+    uint64_t chunk = 0;
+    uint32_t v = 0;
+    const uint32_t max_group = 32;
+    while (v < V) {
+      const uint64_t deg = cp[v + 1] - cp[v];
+      if (deg > C) {
+        // sqrt-sized chunks keep both the chunk work and the fix-up short
+        uint64_t cs = std::max<uint64_t>(C, (uint64_t)std::ceil(std::sqrt((double)deg)));
+        cs = (cs + 63) / 64 * 64;
+        const uint32_t n = (uint32_t)((deg + cs - 1) / cs);
+        PE_REQUIRE(chunk + n < 0xFFFFFFFFull, PE_ERR_RANGE, "too many chunks");
+        bt_long lc{v, (uint32_t)chunk, n};
+        l.push_back(lc);
+        for (uint32_t k = 0; k < n; ++k) {
+          bt_task tk{};
+          tk.v0 = v;
+          tk.v1 = v + 1;
+          tk.e0 = cp[v] + (uint64_t)k * cs;
+          tk.e1 = std::min<uint64_t>(cp[v + 1], tk.e0 + cs);
+          tk.chunk = (uint32_t)(chunk + k);
+          t.push_back(tk);
+        }
+        chunk += n;
+        ++v;
+        continue;
+      }
+      bt_task tk{};
+      tk.v0 = v;
+      tk.e0 = cp[v];
+      tk.chunk = 0xFFFFFFFFu;
+      uint64_t edges = 0;
+      while (v < V) {
+        const uint64_t d = cp[v + 1] - cp[v];
+        if (d > C || (v > tk.v0 && (edges + d > C || v - tk.v0 >= max_group))) break;
+        edges += d;
+        ++v;
+      }
+      tk.v1 = v;
+      tk.e1 = cp[v];
+      t.push_back(tk);
+    }
+    ntasks = t.size();
+    nlong = l.size();
+    nchunks = chunk;
+    tasks.alloc(ctx, std::max<uint64_t>(1, ntasks));
+    longs.alloc(ctx, std::max<uint64_t>(1, nlong));
+    partial.alloc(ctx, std::max<uint64_t>(1, nchunks) * BK);
+    if (ntasks) PE_HIP(hipMemcpyAsync(tasks.p, t.data(), ntasks * sizeof(bt_task), hipMemcpyHostToDevice, s));
+    if (nlong) PE_HIP(hipMemcpyAsync(longs.p, l.data(), nlong * sizeof(bt_long), hipMemcpyHostToDevice, s));
+    PE_HIP(hipStreamSynchronize(s));
+  }
+
+  void launch_iteration() {
+    hipStream_t s = g->ctx->stream;
+    {
+      ktime_scope kt(g->ctx, PE_KT_PPR_BLOCK);
+      k_batch_tasks<<<task_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->od.p, seedmask.p, tasks.p, ntasks,
+                                              X0.p, X1.p, ctl.p, status.p, init_dist.p, opts.damping, partial.p,
+                                              part_delta.p);
+    }
+    k_batch_fixup<<<fix_grid, 256, 0, s>>>(g->od.p, seedmask.p, longs.p, nlong, X0.p, X1.p, ctl.p, status.p,
+                                           init_dist.p, opts.damping, partial.p,
+                                           part_delta.p + (uint64_t)task_grid * BK);
+    k_batch_finish<<<1, 64, 0, s>>>(part_delta.p, task_grid + fix_grid, ctl.p, status.p, iters.p,
+                                    last_delta.p, opts.tol, opts.max_iters);
+    PE_HIP(hipGetLastError());
+  }
+
+  void step(uint32_t n) {
+    if (n == 0) return;
+    hipStream_t s = g->ctx->stream;
+    if (g->ctx->timing) {
+      for (uint32_t i = 0; i < n; ++i) launch_iteration();
+      return;
+    }
+    if (!exec || graph_iters != n) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+      PE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (uint32_t i = 0; i < n; ++i) launch_iteration();
+      PE_HIP(hipStreamEndCapture(s, &graph));
+      PE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      graph_iters = n;
+    }
+    PE_HIP(hipGraphLaunch(exec, s));
+  }
+};
+
+namespace {
+
+void check_slot(pe_ppr_batch_s* b, uint32_t slot) {
+  PE_REQUIRE(slot < b->opts.k, PE_ERR_RANGE, "slot out of range");
+}
+
+// Slots given by the caller -> device list (all must be DONE)
+dbuf<uint32_t> device_slots(pe_ppr_batch_s* b, const uint32_t* slots, uint32_t n) {
+  dbuf<uint32_t> d(b->g->ctx, std::max<uint32_t>(1, n));
+  PE_HIP(hipMemcpyAsync(d.p, slots, (size_t)n * 4, hipMemcpyHostToDevice, b->g->ctx->stream));
+  return d;
+}
+
+void refresh_status(pe_ppr_batch_s* b) {
+  hipStream_t s = b->g->ctx->stream;
+  b->host_status.resize(BK);
+  PE_HIP(hipMemcpyAsync(b->host_status.data(), b->status.p, BK * 4, hipMemcpyDeviceToHost, s));
+  PE_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_batch_t** out) {
+  if (!g || !opts || !out) return PE_ERR_INVALID;
+  *out = nullptr;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(opts->k == 64, PE_ERR_INVALID, "k must be 64 (one column per lane of a wave)");
+    PE_REQUIRE(opts->max_iters >= 1, PE_ERR_INVALID, "max_iters must be >= 1");
+    PE_REQUIRE(opts->damping > 0 && opts->damping < 1, PE_ERR_INVALID, "damping must be in (0, 1)");
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    auto b = std::make_unique<pe_ppr_batch_s>();
+    b->g = g;
+    b->opts = *opts;
+    if (!b->opts.task_edges) b->opts.task_edges = 256;
+    const uint64_t cells = (uint64_t)std::max<uint32_t>(1, g->V) * BK;
+    b->X0.alloc(ctx, cells);
+    b->X1.alloc(ctx, cells);
+    b->seedmask.alloc(ctx, std::max<uint32_t>(1, g->V));
+    PE_HIP(hipMemsetAsync(b->X0.p, 0, cells * 4, s));
+    PE_HIP(hipMemsetAsync(b->X1.p, 0, cells * 4, s));
+    PE_HIP(hipMemsetAsync(b->seedmask.p, 0, (uint64_t)std::max<uint32_t>(1, g->V) * 8, s));
+    b->build_tasks(b->opts.task_edges);
+    b->task_grid = persistent_grid(ctx, (b->ntasks + 3) / 4, 8);
+    b->fix_grid = persistent_grid(ctx, std::max<uint64_t>(1, (b->nlong + 3) / 4), 2);
+    b->part_delta.alloc(ctx, (uint64_t)(b->task_grid + b->fix_grid) * BK);
+    b->ctl.alloc(ctx, 1);
+    b->status.alloc(ctx, BK);
+    b->iters.alloc(ctx, BK);
+    b->init_dist.alloc(ctx, BK);
+    b->last_delta.alloc(ctx, BK);
+    PE_HIP(hipMemsetAsync(b->ctl.p, 0, sizeof(batch_ctl), s));
+    PE_HIP(hipMemsetAsync(b->status.p, 0, BK * 4, s));
+    PE_HIP(hipMemsetAsync(b->iters.p, 0, BK * 4, s));
+    PE_HIP(hipMemsetAsync(b->init_dist.p, 0, BK * 8, s));
+    PE_HIP(hipMemsetAsync(b->last_delta.p, 0, BK * 8, s));
+    PE_HIP(hipStreamSynchronize(s));
+    b->slot_seeds.resize(BK);
+    b->host_status.assign(BK, COL_EMPTY);
+    *out = b.release();
+  });
+}
+
+int pe_ppr_batch_destroy(pe_ppr_batch_t* b) {
+  if (!b) return PE_ERR_INVALID;
+  (void)hipStreamSynchronize(b->g->ctx->stream);
+  delete b;
+  return PE_OK;
+}
+
+int pe_ppr_batch_bytes(const pe_ppr_batch_t* b, uint64_t* bytes) {
+  if (!b || !bytes) return PE_ERR_INVALID;
+  *bytes = b->X0.bytes() + b->X1.bytes() + b->seedmask.bytes() + b->tasks.bytes() + b->longs.bytes() +
+           b->partial.bytes() + b->part_delta.bytes();
+  return PE_OK;
+}
+
+int pe_ppr_batch_admit(pe_ppr_batch_t* b, uint32_t slot, const uint32_t* seeds, uint32_t nseeds) {
+  if (!b || !seeds || nseeds == 0) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] {
+    check_slot(b, slot);
+    refresh_status(b);
+    PE_REQUIRE(b->host_status[slot] == COL_EMPTY, PE_ERR_STATE, "slot is not free (release it first)");
+    pe_graph_s* g = b->g;
+    for (uint32_t i = 0; i < nseeds; ++i) PE_REQUIRE(seeds[i] < g->V, PE_ERR_RANGE, "seed vertex >= V");
+    std::vector<uint32_t> ids(nseeds);
+    to_internal_ids(g, seeds, nseeds, ids.data());
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    hipStream_t s = g->ctx->stream;
+    dbuf<uint32_t> d(g->ctx, ids.size());
+    PE_HIP(hipMemcpyAsync(d.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, s));
+    k_batch_seed_bits<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->seedmask.p, d.p, (uint32_t)ids.size(), slot, 1);
+    k_batch_set_slot<<<1, 1, 0, s>>>(b->status.p, b->init_dist.p, b->iters.p, b->last_delta.p, slot, COL_LOADING,
+                                     1.0 / (double)ids.size());
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipStreamSynchronize(s));
+    b->slot_seeds[slot] = std::move(ids);
+    b->host_status[slot] = COL_LOADING;
+  });
+}
+
+int pe_ppr_batch_release(pe_ppr_batch_t* b, uint32_t slot) {
+  if (!b) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] {
+    check_slot(b, slot);
+    hipStream_t s = b->g->ctx->stream;
+    auto& ids = b->slot_seeds[slot];
+    if (!ids.empty()) {
+      dbuf<uint32_t> d(b->g->ctx, ids.size());
+      PE_HIP(hipMemcpyAsync(d.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, s));
+      k_batch_seed_bits<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->seedmask.p, d.p, (uint32_t)ids.size(), slot, 0);
+      ids.clear();
+    }
+    k_batch_set_slot<<<1, 1, 0, s>>>(b->status.p, b->init_dist.p, b->iters.p, b->last_delta.p, slot, COL_EMPTY, 0.0);
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipStreamSynchronize(s));
+    b->host_status[slot] = COL_EMPTY;
+  });
+}
+
+int pe_ppr_batch_step(pe_ppr_batch_t* b, uint32_t iterations) {
+  if (!b) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] { b->step(iterations); });
+}
+
+int pe_ppr_batch_poll(pe_ppr_batch_t* b, uint32_t* status, uint32_t* iters, double* last_delta) {
+  if (!b) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] {
+    hipStream_t s = b->g->ctx->stream;
+    if (status) PE_HIP(hipMemcpyAsync(status, b->status.p, BK * 4, hipMemcpyDeviceToHost, s));
+    if (iters) PE_HIP(hipMemcpyAsync(iters, b->iters.p, BK * 4, hipMemcpyDeviceToHost, s));
+    if (last_delta) PE_HIP(hipMemcpyAsync(last_delta, b->last_delta.p, BK * 8, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    if (status) b->host_status.assign(status, status + BK);
+  });
+}
+
+int pe_ppr_batch_results(pe_ppr_batch_t* b, const uint32_t* slots, uint32_t n, double* ranks) {
+  if (!b || (n && (!slots || !ranks))) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] {
+    if (!n) return;
+    refresh_status(b);
+    for (uint32_t i = 0; i < n; ++i) {
+      check_slot(b, slots[i]);
+      PE_REQUIRE(b->host_status[slots[i]] == COL_DONE, PE_ERR_STATE, "slot has no finished result");
+    }
+    pe_graph_s* g = b->g;
+    hipStream_t s = g->ctx->stream;
+    const uint32_t V = g->V;
+    auto ds = device_slots(b, slots, n);
+    dbuf<float> dense(g->ctx, (uint64_t)n * std::max<uint32_t>(1, V));
+    k_batch_extract<<<grid_for(V, 256, 8192), 256, 0, s>>>(b->X0.p, b->X1.p, b->ctl.p, V, ds.p, n, dense.p);
+    PE_HIP(hipGetLastError());
+    std::vector<float> tmp(V);
+    for (uint32_t i = 0; i < n; ++i) {
+      download_external(g, dense.p + (uint64_t)i * V, tmp.data(), 4);
+      for (uint32_t v = 0; v < V; ++v) ranks[(uint64_t)i * V + v] = tmp[v];
+    }
+  });
+}
+
+int pe_ppr_batch_topk(pe_ppr_batch_t* b, const uint32_t* slots, uint32_t n, uint32_t k, uint32_t* ids,
+                      float* vals) {
+  if (!b || (n && (!slots || !ids || !vals)) || k == 0) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] {
+    if (!n) return;
+    refresh_status(b);
+    for (uint32_t i = 0; i < n; ++i) {
+      check_slot(b, slots[i]);
+      PE_REQUIRE(b->host_status[slots[i]] == COL_DONE, PE_ERR_STATE, "slot has no finished result");
+    }
+    pe_graph_s* g = b->g;
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const uint32_t V = g->V;
+    auto ds = device_slots(b, slots, n);
+    dbuf<unsigned int> hist(ctx, (uint64_t)n * 2048);
+    PE_HIP(hipMemsetAsync(hist.p, 0, (uint64_t)n * 2048 * 4, s));
+    k_batch_hist<<<grid_for(V, 256, 8192), 256, 0, s>>>(b->X0.p, b->X1.p, b->ctl.p, V, ds.p, n, hist.p);
+    std::vector<unsigned int> h((uint64_t)n * 2048);
+    PE_HIP(hipMemcpyAsync(h.data(), hist.p, h.size() * 4, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> thr(n);
+    uint32_t need_cap = k;
+    for (uint32_t i = 0; i < n; ++i) {
+      uint64_t acc = 0;
+      int bin = 2047;
+      for (; bin > 0; --bin) {
+        acc += h[(uint64_t)i * 2048 + bin];
+        if (acc >= k) break;
+      }
+      thr[i] = (uint32_t)std::max(bin, 1);  // bin 0 holds only denormals; zeros are excluded
+      uint64_t cnt = 0;
+      for (int b2 = 2047; b2 >= (int)thr[i]; --b2) cnt += h[(uint64_t)i * 2048 + b2];
+      need_cap = std::max<uint32_t>(need_cap, (uint32_t)std::min<uint64_t>(cnt, 1u << 24));
+    }
+    dbuf<uint32_t> dthr(ctx, n);
+    PE_HIP(hipMemcpyAsync(dthr.p, thr.data(), n * 4, hipMemcpyHostToDevice, s));
+    dbuf<unsigned int> count(ctx, n);
+    PE_HIP(hipMemsetAsync(count.p, 0, n * 4, s));
+    dbuf<uint32_t> cid(ctx, (uint64_t)n * need_cap);
+    dbuf<float> cval(ctx, (uint64_t)n * need_cap);
+    k_batch_candidates<<<grid_for(V, 256, 8192), 256, 0, s>>>(b->X0.p, b->X1.p, b->ctl.p, V, ds.p, n, dthr.p,
+                                                               need_cap, count.p, cid.p, cval.p);
+    PE_HIP(hipGetLastError());
+    std::vector<unsigned int> hc(n);
+    std::vector<uint32_t> hid((uint64_t)n * need_cap);
+    std::vector<float> hval((uint64_t)n * need_cap);
+    PE_HIP(hipMemcpyAsync(hc.data(), count.p, n * 4, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipMemcpyAsync(hid.data(), cid.p, hid.size() * 4, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipMemcpyAsync(hval.data(), cval.p, hval.size() * 4, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    // input ids for the tie-break (rank desc, id asc: trustrank.hpp:109-112)
+    std::vector<uint32_t> perm;
+    if (g->relabeled) {
+      perm.resize(V);
+      PE_HIP(hipMemcpyAsync(perm.data(), g->perm.p, (uint64_t)V * 4, hipMemcpyDeviceToHost, s));
+      PE_HIP(hipStreamSynchronize(s));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t m = std::min<uint32_t>(hc[i], need_cap);
+      std::vector<std::pair<float, uint32_t>> c(m);
+      for (uint32_t q = 0; q < m; ++q) {
+        const uint32_t iv = hid[(uint64_t)i * need_cap + q];
+        c[q] = {hval[(uint64_t)i * need_cap + q], g->relabeled ? perm[iv] : iv};
+      }
+      std::sort(c.begin(), c.end(), [](const std::pair<float, uint32_t>& a, const std::pair<float, uint32_t>& bb) {
+        return a.first > bb.first || (a.first == bb.first && a.second < bb.second);
+      });
+      // fewer positive entries than k: the remaining places go to the smallest
+      // ids with rank 0, as the reference's sort would order them
+      std::vector<uint8_t> taken;
+      uint32_t next_zero = 0;
+      for (uint32_t q = 0; q < k; ++q) {
+        if (q < c.size()) {
+          ids[(uint64_t)i * k + q] = c[q].second;
+          vals[(uint64_t)i * k + q] = c[q].first;
+        } else {
+          if (taken.empty()) {
+            taken.assign(V, 0);
+            for (auto& pr : c) taken[pr.second] = 1;
+          }
+          while (next_zero < V && taken[next_zero]) ++next_zero;
+          ids[(uint64_t)i * k + q] = next_zero < V ? next_zero : 0xFFFFFFFFu;
+          vals[(uint64_t)i * k + q] = 0.0f;
+          ++next_zero;
+        }
+      }
+    }
+  });
+}
+
+}  // extern "C"

@@ -392,3 +392,41 @@ class dualmode_engine_t:
         check(self.lib.pe_rank_iterate(self.graph.h, x.h, y.h, int(weighted), C.byref(p),
                                        C.byref(d)), self.ctx.h)
         return d.value
+
+
+class BfsEngineState:
+    """Device BFS state for engine-level stepping (pe_bfs_*): one epoch of the
+    reference's 5-argument foreach_edges with the BFS operators per step()."""
+
+    def __init__(self, graph: Graph):
+        self.graph, self.ctx, self.lib = graph, graph.ctx, graph.lib
+        h = C.c_void_p()
+        check(self.lib.pe_bfs_create(graph.h, C.byref(h)), self.ctx.h)
+        self.h = h
+        self.ctx._adopt(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                self.lib.pe_bfs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, root, frontier: Bitmap):
+        check(self.lib.pe_bfs_reset(self.h, int(root), frontier.h), self.ctx.h)
+
+    def step(self, push: bool, current: Bitmap, nxt: Bitmap):
+        a, n = C.c_uint64(), C.c_uint64()
+        check(self.lib.pe_bfs_step(self.h, int(push), current.h, nxt.h, C.byref(a), C.byref(n)),
+              self.ctx.h)
+        return a.value, n.value
+
+    def levels(self):
+        lv = np.zeros(self.graph.V, np.int32)
+        check(self.lib.pe_bfs_get_levels(self.h, lv.ctypes.data), self.ctx.h)
+        return lv

@@ -1,0 +1,7 @@
+// Umbrella header, as plato/graph/graph.hpp.
+#pragma once
+
+#include "plato/graph/base.hpp"
+#include "plato/graph/state/dense_state.hpp"
+#include "plato/graph/structure.hpp"
+#include "plato/util/bitmap.hpp"

@@ -1,0 +1,174 @@
+// Graph structures and factories on the device.
+//
+// dcsc_t / bcsr_t (plato/graph/structure/dcsc.hpp:58-185, bcsr.hpp:67-208)
+// become handles onto one device graph (CSC over all ids, optional CSR);
+// sequence_balanced_by_* (plato/graph/partition/sequence.hpp) is the
+// one-partition identity.  The factories keep the names and arguments of
+// create_dcsc_seqs_from_path / create_bcsr_seqd_from_path /
+// create_dualmode_seq_from_path (plato/graph/structure.hpp:473-555, 987-1086)
+// and read Plato's CSV edge format `src,dst[,w]` (parsers.hpp:225-293).
+#pragma once
+
+#include <algorithm>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "plato/graph/base.hpp"
+#include "plato/graph/state/dense_state.hpp"
+
+namespace plato {
+
+enum class edge_format_t { CSV = 0 };
+
+struct sequence_v_view {
+  vid_t start_, end_;
+  vid_t start() const { return start_; }
+  vid_t end() const { return end_; }
+};
+
+// one partition covering every vertex id
+struct sequence_balanced_by_destination_t {
+  vid_t vertices_ = 0;
+  sequence_v_view self_v_view() const { return {0, vertices_}; }
+  int get_partition_id(vid_t) const { return 0; }
+};
+using sequence_balanced_by_source_t = sequence_balanced_by_destination_t;
+
+// Shared owner of one device graph.
+struct device_graph {
+  pe_graph_t* h = nullptr;
+  pe_graph_info_t info{};
+  ~device_graph() {
+    if (h) pe_graph_destroy(h);
+  }
+};
+
+namespace detail {
+template <typename EDATA>
+class graph_facade {
+ public:
+  using partition_t = sequence_balanced_by_destination_t;
+  using edata_t = EDATA;
+  struct adj_unit_list_spec_t {};  // adjacency spans live on the device
+
+  graph_facade() = default;
+  explicit graph_facade(std::shared_ptr<device_graph> g) : g_(std::move(g)) {
+    part_ = std::make_shared<partition_t>();
+    part_->vertices_ = g_->info.V;
+  }
+  std::shared_ptr<partition_t> partitioner() const { return part_; }
+  pe_graph_t* handle() const { return g_->h; }
+  const pe_graph_info_t& info() const { return g_->info; }
+  std::shared_ptr<device_graph> shared() const { return g_; }
+
+ private:
+  std::shared_ptr<device_graph> g_;
+  std::shared_ptr<partition_t> part_;
+};
+}  // namespace detail
+
+template <typename EDATA, typename PART = sequence_balanced_by_destination_t>
+class dcsc_t : public detail::graph_facade<EDATA> {
+  using detail::graph_facade<EDATA>::graph_facade;
+};
+
+template <typename EDATA, typename PART = sequence_balanced_by_source_t>
+class bcsr_t : public detail::graph_facade<EDATA> {
+  using detail::graph_facade<EDATA>::graph_facade;
+};
+
+// decoders: float weights or none
+inline float float_decoder(const std::string& s) { return std::stof(s); }
+template <typename T>
+inline T dummy_decoder(const std::string&) { return T(); }
+
+namespace detail {
+struct edge_list {
+  std::vector<uint32_t> src, dst;
+  std::vector<float> w;
+};
+
+// `src,dst[,w]` per line, whitespace around fields tolerated (parsers.hpp:225-293)
+inline edge_list read_csv(const std::string& path, bool weighted) {
+  edge_list el;
+  std::ifstream in(path);
+  if (!in) fatal("open", -1, path.c_str());
+  std::string line;
+  while (std::getline(in, line)) {
+    std::stringstream ss(line);
+    std::string a, b, c;
+    if (!std::getline(ss, a, ',') || !std::getline(ss, b, ',')) continue;
+    try {
+      const unsigned long s = std::stoul(a), d = std::stoul(b);
+      el.src.push_back((uint32_t)s);
+      el.dst.push_back((uint32_t)d);
+      if (weighted) el.w.push_back(std::getline(ss, c, ',') ? std::stof(c) : 1.0f);
+    } catch (...) {
+      continue;  // malformed line: skipped, as the reference logs and skips
+    }
+  }
+  return el;
+}
+
+inline std::shared_ptr<device_graph> build(graph_info_t* info, const edge_list& el, bool weighted,
+                                           bool build_csr, bool relabel) {
+  uint32_t V = 0;
+  for (size_t i = 0; i < el.src.size(); ++i) V = std::max(V, std::max(el.src[i], el.dst[i]) + 1);
+  if (V == 0) V = 1;
+  uint32_t flags = info->is_directed_ ? 0u : PE_GRAPH_SYMMETRIZE;
+  if (build_csr) flags |= PE_GRAPH_BUILD_CSR;
+  if (weighted) flags |= PE_GRAPH_WEIGHTED;
+  if (relabel) flags |= PE_GRAPH_RELABEL_DEGREE;
+  auto g = std::make_shared<device_graph>();
+  check_status(pe_graph_create(cluster_info_t::get_instance().ctx(), el.src.data(), el.dst.data(),
+                               weighted ? el.w.data() : nullptr, el.src.size(), V, flags, &g->h),
+               "pe_graph_create");
+  check_status(pe_graph_get_info(g->h, &g->info), "pe_graph_get_info");
+  info->vertices_ = g->info.vertices;
+  info->edges_ = g->info.edges;
+  info->max_v_i_ = g->info.max_v_i;
+  return g;
+}
+}  // namespace detail
+
+template <typename EDATA, typename DECODER = void*>
+std::shared_ptr<dcsc_t<EDATA>> create_dcsc_seqs_from_path(graph_info_t* info, const std::string& path,
+                                                          edge_format_t = edge_format_t::CSV,
+                                                          DECODER = nullptr, bool relabel = false) {
+  const bool weighted = !std::is_same<EDATA, empty_t>::value;
+  auto g = detail::build(info, detail::read_csv(path, weighted), weighted, false, relabel);
+  return std::make_shared<dcsc_t<EDATA>>(g);
+}
+
+template <typename EDATA, typename DECODER = void*>
+std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqd_from_path(graph_info_t* info, const std::string& path,
+                                                          edge_format_t = edge_format_t::CSV,
+                                                          DECODER = nullptr) {
+  const bool weighted = !std::is_same<EDATA, empty_t>::value;
+  auto g = detail::build(info, detail::read_csv(path, weighted), weighted, true, false);
+  return std::make_shared<bcsr_t<EDATA>>(g);
+}
+
+template <typename EDATA, typename DECODER = void*>
+std::pair<std::shared_ptr<bcsr_t<EDATA>>, std::shared_ptr<dcsc_t<EDATA>>> create_dualmode_seq_from_path(
+    graph_info_t* info, const std::string& path, edge_format_t = edge_format_t::CSV, DECODER = nullptr,
+    bool relabel = false) {
+  const bool weighted = !std::is_same<EDATA, empty_t>::value;
+  auto g = detail::build(info, detail::read_csv(path, weighted), weighted, true, relabel);
+  return {std::make_shared<bcsr_t<EDATA>>(g), std::make_shared<dcsc_t<EDATA>>(g)};
+}
+
+// out-degrees counted from the in-edges (structure.hpp:292-346)
+template <typename T, typename GRAPH>
+host_state_t<T, typename GRAPH::partition_t> generate_dense_out_degrees_fg(const graph_info_t&, GRAPH& graph,
+                                                                           bool /*is_out_edge*/) {
+  std::vector<uint32_t> od(graph.info().V);
+  check_status(pe_graph_out_degrees(graph.handle(), od.data()), "pe_graph_out_degrees");
+  return host_state_t<T, typename GRAPH::partition_t>(std::vector<T>(od.begin(), od.end()));
+}
+
+}  // namespace plato

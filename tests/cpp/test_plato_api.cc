@@ -1,0 +1,147 @@
+// The host C++ API against the reference tests' known answers, on the device.
+// Each case names the reference test it restates.  Run from the repository
+// root: plato_amd/bin/test_plato_api tests/golden
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "plato/algo/bfs/bfs.hpp"
+#include "plato/algo/pagerank/pagerank.hpp"
+#include "plato/algo/ppr/personalized_pagerank.hpp"
+
+static int failures = 0;
+#define EXPECT(cond)                                                   \
+  do {                                                                 \
+    if (!(cond)) {                                                     \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);  \
+      ++failures;                                                      \
+    }                                                                  \
+  } while (0)
+
+static std::string dir;
+
+// plato/util/test/bitmap.cc:31-50 (MSB) and 52-76 (ORIGIN traversal)
+static void Bitmap_MSB_Traverse() {
+  plato::bitmap_t<> bitmap(40988);
+  bitmap.set_bit(0);
+  EXPECT(bitmap.msb() == 0);
+  bitmap.set_bit(13);
+  EXPECT(bitmap.msb() == 13);
+  bitmap.set_bit(1982);
+  EXPECT(bitmap.msb() == 1982);
+  bitmap.clear();
+  bitmap.set_bit(40987);
+  EXPECT(bitmap.msb() == 40987);
+  bitmap.set_bit(0);
+  bitmap.set_bit(40986);
+  EXPECT(bitmap.msb() == 40987);
+  plato::bitmap_t<> b2(40988);
+  b2.set_bit(0);
+  b2.set_bit(19);
+  b2.set_bit(40987);
+  auto v = b2.to_vector();
+  EXPECT(v.size() == 3 && v[0] == 0 && v[1] == 19 && v[2] == 40987);
+}
+
+// plato/graph/test/message_passing.cc:48-88 through the engine's pull operators
+static void MessagePassing_AggregateMessage() {
+  plato::graph_info_t graph_info(true);
+  auto pdcsc = plato::create_dcsc_seqs_from_path<float>(&graph_info, dir + "/graph_10_9.csv");
+  plato::dualmode_engine_t<plato::dcsc_t<float>, std::nullptr_t> engine(pdcsc, nullptr, graph_info);
+  auto ones = engine.alloc_v_state<float>();
+  auto values = engine.alloc_v_state<float>();
+  ones.fill(1.0f);
+  engine.foreach_edges<float, int>(plato::ops::gather_sum<float>{ones}, plato::ops::write_add<float>{values});
+  EXPECT(values.sum() == 9.0);
+  EXPECT(values[0] == 0.0f);
+  for (plato::vid_t i = 1; i < 10; ++i) EXPECT(values[i] == 1.0f);
+  EXPECT(graph_info.edges_ == 9 && graph_info.max_v_i_ == 9);
+}
+
+// plato/graph/test/structure.cc:137-151 and 205-218
+static void GraphOps_Degrees() {
+  plato::graph_info_t undirected(false);
+  auto g1 = plato::create_dcsc_seqs_from_path<plato::empty_t>(&undirected, dir + "/v100_e2150_ua_c3.csv");
+  auto d1 = plato::generate_dense_out_degrees_fg<uint32_t>(undirected, *g1, false);
+  EXPECT(d1[0] == 19 && d1[99] == 59 && d1[28] == 19);
+  EXPECT(undirected.vertices_ == 100);
+  plato::graph_info_t directed(true);
+  auto g2 = plato::create_dcsc_seqs_from_path<plato::empty_t>(&directed, dir + "/v100_e4300_da_c3.csv");
+  auto d2 = plato::generate_dense_out_degrees_fg<uint32_t>(directed, *g2, false);
+  EXPECT(d2[0] == 19 && d2[99] == 59 && d2[28] == 19);
+}
+
+// chain 0->1->...->9: r(v) = 0.15 * sum_{i<=v} 0.85^i after 20 iterations
+static void PageRank_ChainClosedForm() {
+  plato::graph_info_t graph_info(true);
+  auto pdcsc = plato::create_dcsc_seqs_from_path<plato::empty_t>(&graph_info, dir + "/graph_10_9.csv");
+  plato::algo::pagerank_opts_t opts;
+  opts.iteration_ = 20;
+  opts.eps_ = 0.0;
+  opts.precision_ = 64;
+  auto ranks = plato::algo::pagerank(*pdcsc, graph_info, opts);
+  double expect = 0.0, p = 1.0;
+  for (plato::vid_t v = 0; v < 10; ++v) {
+    expect += 0.15 * p;
+    p *= 0.85;
+    EXPECT(std::fabs(ranks[v] - expect) <= 1e-14 * expect);
+  }
+  auto w = plato::create_dcsc_seqs_from_path<float>(&graph_info, dir + "/graph_10_9.csv");
+  auto wranks = plato::algo::weighted_pagerank(*w, graph_info, opts);
+  for (plato::vid_t v = 0; v < 10; ++v) EXPECT(std::fabs(wranks[v] - ranks[v]) <= 1e-6 * ranks[v]);
+}
+
+// PPR from 0 on the chain: r(v) = 0.15 * 0.85^v (pull and push/pull forms)
+static void PPR_ChainClosedForm() {
+  plato::graph_info_t graph_info(true);
+  auto g = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/graph_10_9.csv");
+  plato::algo::personalized_pagerank_opts_t opts;
+  opts.iteration_ = 20;
+  opts.eps_ = 0.0;
+  opts.src_ = 0;
+  opts.precision_ = 64;
+  auto pull = plato::algo::personalized_pagerank(*g.second, graph_info, opts);
+  auto pp = plato::algo::personalized_pagerank(*g.second, *g.first, graph_info, opts);
+  double expect = 0.15;
+  for (plato::vid_t v = 0; v < 10; ++v) {
+    EXPECT(std::fabs(pull[v] - expect) <= 1e-14 * expect);
+    EXPECT(std::fabs(pp[v] - expect) <= 1e-14 * expect);
+    expect *= 0.85;
+  }
+}
+
+// complete binary tree: visited 10, level(v) = floor(log2(v + 1))
+static void BFS_BinaryTree() {
+  plato::graph_info_t graph_info(true);
+  auto g = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/complete_binary_tree_10_9.csv");
+  plato::algo::bfs_opts_t opts;
+  opts.root_ = 0;
+  const plato::vid_t visited = plato::algo::breadth_first_search(*g.second, *g.first, graph_info, opts);
+  EXPECT(visited == 10);
+  const auto& lv = plato::algo::last_bfs_levels();
+  for (int v = 0; v < 10; ++v) EXPECT(lv[v] == (int)std::floor(std::log2(v + 1)));
+  plato::graph_info_t undirected(false);
+  auto u = plato::create_dualmode_seq_from_path<plato::empty_t>(&undirected, dir + "/v100_e2150_ua_c3.csv");
+  opts.root_ = 0;
+  EXPECT(plato::algo::breadth_first_search(*u.second, *u.first, undirected, opts) == 100);
+}
+
+int main(int argc, char** argv) {
+  dir = argc > 1 ? argv[1] : "tests/golden";
+  struct { const char* name; void (*fn)(); } cases[] = {
+      {"Bitmap.MSB_Traverse", Bitmap_MSB_Traverse},
+      {"MessagePassing.AggregateMessage", MessagePassing_AggregateMessage},
+      {"GraphOps.Degrees", GraphOps_Degrees},
+      {"PageRank.ChainClosedForm", PageRank_ChainClosedForm},
+      {"PPR.ChainClosedForm", PPR_ChainClosedForm},
+      {"BFS.BinaryTree", BFS_BinaryTree},
+  };
+  for (auto& c : cases) {
+    const int before = failures;
+    c.fn();
+    std::printf("[%s] %s\n", failures == before ? "  OK  " : "FAILED", c.name);
+  }
+  std::printf("%d failure(s)\n", failures);
+  return failures ? 1 : 0;
+}

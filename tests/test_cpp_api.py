@@ -1,0 +1,68 @@
+"""Host C++ layer (plato_amd/include/plato/...): it compiles with g++ on the CPU,
+and on the GPU the C++ API test and the example drivers agree with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, ROOT
+
+BIN = os.path.join(ROOT, "plato_amd", "bin")
+
+
+def test_host_programs_build():
+    r = subprocess.run(["make", "-s", "-f", "plato_amd/host/Makefile", f"ROOT={ROOT}"], cwd=ROOT,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "test_plato_api"):
+        assert os.access(os.path.join(BIN, p), os.X_OK)
+
+
+def test_unsupported_functor_is_a_compile_error():
+    src = ('#include "plato/engine/dualmode.hpp"\n'
+           'int main(){ plato::graph_info_t gi(true);\n'
+           ' auto g = std::make_shared<plato::dcsc_t<plato::empty_t>>();\n'
+           ' plato::dualmode_engine_t<plato::dcsc_t<plato::empty_t>, std::nullptr_t> e(g, nullptr, gi);\n'
+           ' e.foreach_edges<double,int>([](int){}, [](int){ return 0; }); }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-",
+                        f"-I{ROOT}/include", f"-I{ROOT}/plato_amd/include"], input=src, text=True,
+                       capture_output=True)
+    assert r.returncode != 0 and "foreach_edges (pull)" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_api_known_answers():
+    r = subprocess.run([os.path.join(BIN, "test_plato_api"), GOLDEN], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("  OK  ") == 6
+
+
+@pytest.mark.gpu
+def test_pagerank_driver_output_matches_oracle(tmp_path):
+    s, d = oracle.read_edge_csv(os.path.join(GOLDEN, "v100_e4300_da_c3.csv"))
+    r = subprocess.run([os.path.join(BIN, "pagerank_simple"), "--input",
+                        os.path.join(GOLDEN, "v100_e4300_da_c3.csv"), "--is_directed=true",
+                        "--iterations", "20", "--eps", "0", "--precision", "64",
+                        "--output", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(tmp_path / "0000_0.csv", delimiter=",")
+    ref, _ = oracle.pagerank(oracle.CSC(100, s, d), 20, 0.85, 0.0)
+    assert np.array_equal(got[:, 0], np.arange(100))
+    np.testing.assert_allclose(got[:, 1], ref, rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_bfs_driver_levels(tmp_path):
+    path = os.path.join(GOLDEN, "v100_e2150_ua_c3.csv")
+    s, d = oracle.read_edge_csv(path)
+    r = subprocess.run([os.path.join(BIN, "bfs_simple"), "--input", path, "--root", "7",
+                        "--output", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(tmp_path / "0000_0.csv", delimiter=",", dtype=np.int64)
+    ref = oracle.bfs(oracle.CSC(100, s, d, symmetrize=True), 7)
+    assert np.array_equal(got[:, 1], ref["levels"])
+    assert f"visited: {ref['visited']}" in r.stdout

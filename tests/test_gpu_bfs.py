@@ -84,3 +84,31 @@ def test_bfs_relabeled_graph(ctx):
         res = breadth_first_search(g, bfs_opts_t(int(root)))
         assert np.array_equal(res.levels, ref["levels"])
         assert np.array_equal(res.modes, ref["modes"]) and res.visited == ref["visited"]
+
+
+def test_engine_step_loop_mirrors_reference(ctx):
+    """bfs.hpp:88-151 driven step by step: choose mode from the active set, one
+    foreach_edges epoch, swap; every decision and frontier equals the oracle's."""
+    from plato_amd import Graph, dualmode_engine_t
+    from plato_amd.engine import BfsEngineState
+    for directed in (False, True):
+        g = Graph.rmat(ctx, 12, 16, seed=31, directed=directed, build_csr=True)
+        s, d = oracle.rmat_edges(12, 16, 31)
+        og = oracle.CSC(1 << 12, s, d, symmetrize=not directed)
+        root = int(np.flatnonzero(og.od == og.od[og.od > 0].min())[0])
+        ref = oracle.bfs(og, root)
+        eng = dualmode_engine_t(g)
+        cur, nxt = eng.alloc_v_subset(), eng.alloc_v_subset()
+        st = BfsEngineState(g)
+        st.reset(root, cur)
+        actives, epoch, modes, visited = 1, 0, [], 1
+        while actives:
+            push, A = eng.choose_push(cur)
+            assert A == ref["active_edges"][epoch] and cur.count() == ref["frontier"][epoch]
+            modes.append(int(push))
+            actives, _ = st.step(push, cur, nxt)
+            visited += actives
+            cur, nxt = nxt, cur
+            epoch += 1
+        assert modes == ref["modes"].tolist() and visited == ref["visited"]
+        assert np.array_equal(st.levels(), ref["levels"])
