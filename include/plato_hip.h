@@ -356,6 +356,26 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
                 uint64_t* next_active_edges);
 int pe_bfs_get_levels(pe_bfs_t* b, int32_t* levels);
 
+/* ===========================================================================
+ * Autotuner (no reference counterpart: the reference has no tunables on this
+ * path beyond dualmode_engine_opts_t::push_threshold_).  Times candidates on
+ * the resident graph and keeps the fastest: pull tile size (applied to the
+ * graph), the 64-column PPR block's task size, and the BFS push/pull ratio.
+ * ========================================================================= */
+#define PE_TUNE_PULL 1u
+#define PE_TUNE_BATCH 2u
+#define PE_TUNE_BFS 4u
+typedef struct {
+  uint32_t tile_items;        /* chosen merge-path items per pull tile */
+  uint32_t batch_task_edges;  /* pe_ppr_batch_opts_t::task_edges */
+  double bfs_push_threshold;  /* threshold for pe_bfs */
+  double pull_ms, batch_ms, bfs_ms;  /* best times seen (per iteration / per search) */
+} pe_tune_result_t;
+int pe_autotune(pe_graph_t* g, uint32_t what, pe_tune_result_t* out);
+/* append / look up a choice keyed by (V, nnz, directed, relabeled) in a text file */
+int pe_autotune_save(pe_graph_t* g, const pe_tune_result_t* r, const char* path);
+int pe_autotune_load(pe_graph_t* g, const char* path, pe_tune_result_t* r, int* found);
+
 #ifdef __cplusplus
 }
 #endif

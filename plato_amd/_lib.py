@@ -70,6 +70,14 @@ class RequestStats(C.Structure):
                 ("slot", C.c_uint32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tile_items", C.c_uint32), ("batch_task_edges", C.c_uint32),
+                ("bfs_push_threshold", C.c_double), ("pull_ms", C.c_double),
+                ("batch_ms", C.c_double), ("bfs_ms", C.c_double)]
+
+
+PE_TUNE_PULL, PE_TUNE_BATCH, PE_TUNE_BFS = 1, 2, 4
+
 _vp = C.c_void_p
 _u32, _u64, _i32, _dbl = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 _P = C.POINTER
@@ -133,6 +141,9 @@ SIGNATURES = {
     "pe_bfs_reset": [_vp, _u32, _vp],
     "pe_bfs_step": [_vp, _i32, _vp, _vp, _P(_u64), _P(_u64)],
     "pe_bfs_get_levels": [_vp, _vp],
+    "pe_autotune": [_vp, _u32, _P(TuneResult)],
+    "pe_autotune_save": [_vp, _P(TuneResult), C.c_char_p],
+    "pe_autotune_load": [_vp, C.c_char_p, _P(TuneResult), _P(_i32)],
     "pe_ppr_batch_create": [_vp, _P(PprBatchOpts), _P(_vp)],
     "pe_ppr_batch_destroy": [_vp],
     "pe_ppr_batch_bytes": [_vp, _P(_u64)],

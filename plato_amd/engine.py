@@ -187,6 +187,16 @@ class Graph:
         check(self.lib.pe_graph_set_tile(self.h, items), self.ctx.h)
         check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
 
+    def autotune(self, pull=True, batch=False, bfs=False):
+        """Time the engine's tunables on this graph (pe_autotune); the pull tile
+        choice is applied to the graph.  Returns the chosen values."""
+        what = (L.PE_TUNE_PULL if pull else 0) | (L.PE_TUNE_BATCH if batch else 0) | \
+            (L.PE_TUNE_BFS if bfs else 0)
+        r = L.TuneResult()
+        check(self.lib.pe_autotune(self.h, what, C.byref(r)), self.ctx.h)
+        check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
+        return {f: getattr(r, f) for f, _ in L.TuneResult._fields_}
+
     def download_csc(self, weights=False):
         cp = np.zeros(self.V + 1, np.uint64)
         ri = np.zeros(max(self.nnz, 1), np.uint32)

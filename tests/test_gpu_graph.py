@@ -170,3 +170,25 @@ def test_errors_are_status_codes(ctx):
         g.download_csr()          # no CSR was built
     with pytest.raises(PlatoHipError):
         g.set_tile(3000)
+
+
+def test_autotune_choices_are_valid_and_keep_results(ctx, tmp_path):
+    from plato_amd import Graph
+    from plato_amd.algo import pagerank, pagerank_opts_t
+    g = Graph.rmat(ctx, 12, 16, seed=8, build_csr=True, relabel=True)
+    before = pagerank(g, pagerank_opts_t(10, 0.85, 0.0), precision=64).rank
+    r = g.autotune(pull=True, batch=True, bfs=True)
+    assert r["tile_items"] in (1024, 2048, 4096) and g.info.tile_items == r["tile_items"]
+    assert r["batch_task_edges"] in (128, 256, 512, 1024)
+    assert r["bfs_push_threshold"] in (0.01, 0.02, 0.05, 0.1, 0.2)
+    assert r["pull_ms"] > 0 and r["batch_ms"] > 0 and r["bfs_ms"] > 0
+    after = pagerank(g, pagerank_opts_t(10, 0.85, 0.0), precision=64).rank
+    np.testing.assert_allclose(after, before, rtol=1e-12)
+    import ctypes as C
+    from plato_amd import _lib as L
+    res = L.TuneResult(**r)
+    path = str(tmp_path / "tune.txt").encode()
+    assert g.lib.pe_autotune_save(g.h, C.byref(res), path) == 0
+    out, found = L.TuneResult(), C.c_int()
+    assert g.lib.pe_autotune_load(g.h, path, C.byref(out), C.byref(found)) == 0
+    assert found.value == 1 and out.tile_items == r["tile_items"]
