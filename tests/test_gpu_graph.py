@@ -120,6 +120,11 @@ def test_bitmap_known_answers(ctx):
     c = Bitmap(ctx, 10)
     c.copy_from(f)
     assert c.count() == 10
+    # message_passing.cc:125-169: spread over a sparse active set {0, 5, 9} reaches 3 vertices
+    sp = Bitmap(ctx, 10)
+    for v in (0, 5, 9):
+        sp.set_bit(v)
+    assert sp.count() == 3 and sp.to_vector().tolist() == [0, 5, 9]
 
 
 def test_dense_state_ops(ctx):
