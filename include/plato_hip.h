@@ -48,6 +48,8 @@ extern "C" {
 #define PE_KT_BFS_PUSH 2    /* top-down BFS step */
 #define PE_KT_BFS_PULL 3    /* bottom-up BFS step */
 #define PE_KT_PPR_BLOCK 4   /* batched PPR k-column gather */
+#define PE_KT_PULL_BIN_SCATTER 5 /* binned pull: contributions into destination-bin runs */
+#define PE_KT_PULL_BIN_GATHER 6  /* binned pull: exact per-bin sums with the apply fused */
 #define PE_KT_COUNT 8
 
 typedef struct pe_ctx_s pe_ctx_t;
@@ -108,6 +110,9 @@ typedef struct {
   uint64_t num_tiles;
   uint64_t num_fixup_rows;
   uint64_t device_bytes;
+  int32_t pull_layout;    /* PE_PULL_MERGE_PATH or PE_PULL_BINNED */
+  uint32_t binned_runs;   /* run records of the binned layout (0 if not built) */
+  uint64_t binned_bytes;  /* device bytes held by the binned layout */
 } pe_graph_info_t;
 
 /* Build from a host edge list (src[i] -> dst[i], optional weights w[i]).
@@ -132,6 +137,28 @@ int pe_graph_download_csr(pe_graph_t* g, uint64_t* row_ptr, uint32_t* col_idx);
 int pe_graph_permutation(pe_graph_t* g, uint32_t* rank_of);
 /* Re-tile the pull gather (merge-path items per tile: 1024, 2048 or 4096). */
 int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items);
+
+/* Pull layout used by the FP32 rank iterations (pe_pagerank, pe_ppr, pe_rank_iterate,
+ * weighted or not) -- the same aggregate_message pass (message_passing.hpp:88-144)
+ * under two decompositions:
+ *   PE_PULL_MERGE_PATH  CSC gather of x[u] per edge, compensated FP32 sums (default);
+ *   PE_PULL_BINNED      partition-centric: contributions are streamed into
+ *                       destination bins, then each bin is summed in LDS as exact
+ *                       128-bit fixed-point integers (order-independent, bit-
+ *                       reproducible).  Builds a second, bin-ordered copy of the
+ *                       edges (4 B per edge, +4 B with weights) and 4 B per edge of
+ *                       scratch on first use.
+ * FP64 iterations and raw sums (pe_pull_sum) always use the merge-path gather.
+ * opts may be NULL (defaults); its fields tune the binned layout. */
+#define PE_PULL_MERGE_PATH 0
+#define PE_PULL_BINNED 1
+typedef struct {
+  uint32_t src_bits;    /* log2 sources per chunk staged in LDS (<= 15; default 15) */
+  uint32_t dst_bits;    /* log2 destinations per bin summed in LDS (<= 13; default 13) */
+  uint32_t run_cap;     /* max entries per scatter run record (default 256) */
+  uint32_t unit_log2;   /* log2 max entries per gather unit; bigger bins are split (default 20) */
+} pe_binned_opts_t;
+int pe_graph_set_pull_layout(pe_graph_t* g, int layout, const pe_binned_opts_t* opts);
 
 /* ===========================================================================
  * Engine state.  pe_vec replaces dense_state_t

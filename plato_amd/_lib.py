@@ -16,6 +16,8 @@ PE_ERR = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "bad sta
           5: "out of range", 6: "internal error"}
 PE_F32, PE_F64, PE_U32 = 0, 1, 2
 PE_KT_PULL_TILE, PE_KT_PULL_FIXUP, PE_KT_BFS_PUSH, PE_KT_BFS_PULL, PE_KT_PPR_BLOCK = 0, 1, 2, 3, 4
+PE_KT_PULL_BIN_SCATTER, PE_KT_PULL_BIN_GATHER = 5, 6
+PE_PULL_MERGE_PATH, PE_PULL_BINNED = 0, 1
 PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED, PE_GRAPH_RELABEL_DEGREE = 1, 2, 4, 8
 
 
@@ -30,7 +32,13 @@ class GraphInfo(C.Structure):
                 ("max_in_degree", C.c_uint32), ("edges", C.c_uint64), ("nnz", C.c_uint64),
                 ("is_directed", C.c_int32), ("has_csr", C.c_int32), ("has_weights", C.c_int32),
                 ("relabeled", C.c_int32), ("tile_items", C.c_uint32), ("num_tiles", C.c_uint64),
-                ("num_fixup_rows", C.c_uint64), ("device_bytes", C.c_uint64)]
+                ("num_fixup_rows", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("pull_layout", C.c_int32), ("binned_runs", C.c_uint32), ("binned_bytes", C.c_uint64)]
+
+
+class BinnedOpts(C.Structure):
+    _fields_ = [("src_bits", C.c_uint32), ("dst_bits", C.c_uint32), ("run_cap", C.c_uint32),
+                ("unit_log2", C.c_uint32)]
 
 
 class ApplyParams(C.Structure):
@@ -102,6 +110,7 @@ SIGNATURES = {
     "pe_graph_download_csc": [_vp, _vp, _vp, _vp],
     "pe_graph_download_csr": [_vp, _vp, _vp],
     "pe_graph_set_tile": [_vp, _u32],
+    "pe_graph_set_pull_layout": [_vp, C.c_int, _vp],
     "pe_graph_permutation": [_vp, _vp],
     "pe_vec_create": [_vp, _i32, _u64, _P(_vp)],
     "pe_vec_destroy": [_vp],

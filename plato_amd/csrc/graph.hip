@@ -283,7 +283,7 @@ static int bits_for(uint32_t V) {
 }
 
 // Radix sort of packed keys (and optional float payload), stable.
-static void sort_keys(pe_ctx_s* ctx, dbuf<uint64_t>& keys, dbuf<float>* vals, uint64_t n,
+void sort_keys(pe_ctx_s* ctx, dbuf<uint64_t>& keys, dbuf<float>* vals, uint64_t n,
                       int end_bit) {
   if (n == 0) return;
   hipStream_t s = ctx->stream;
@@ -688,6 +688,9 @@ int pe_graph_get_info(const pe_graph_t* g, pe_graph_info_t* info) {
   info->num_tiles = g->num_tiles;
   info->num_fixup_rows = g->num_fix;
   info->device_bytes = g->device_bytes();
+  info->pull_layout = g->pull_layout;
+  info->binned_runs = g->pb ? (uint32_t)std::min<uint64_t>(g->pb->nrecords, 0xFFFFFFFFull) : 0;
+  info->binned_bytes = g->pb ? g->pb->bytes() : 0;
   return PE_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -121,6 +122,38 @@ inline unsigned persistent_grid(pe_ctx_s* ctx, uint64_t tiles, unsigned blocks_p
 
 }  // namespace pe
 
+// ---- binned pull layout (pull_binned.hip) -------------------------------------------
+namespace pe {
+struct pb_unit1 {  // scatter work: run records [r0, r1) of one source chunk
+  uint64_t r0, r1;
+  uint32_t chunk, pad;
+};
+struct pb_unit2 {  // gather work: entries [e0, e1) of one bin; slot ~0u = the whole bin
+  uint64_t e0, e1;
+  uint32_t bin, slot;
+};
+struct pb_layout {
+  uint32_t src_bits = 15, dst_bits = 13, run_cap = 2048, unit_log2 = 20;
+  uint32_t nchunks = 0, nbins = 0;
+  uint64_t nrecords = 0;
+  uint32_t n1 = 0, n2 = 0, nsplit = 0;
+  bool weighted = false;
+  dbuf<uint16_t> src_local, dst_local;  // [nnz] bin order: (bin, chunk, dst, src)
+  dbuf<float> w;                        // [nnz] bin order, weighted graphs
+  dbuf<float> values;                   // [nnz] contributions of the current iteration
+  dbuf<uint64_t> rec_start;             // [nrecords] chunk-major run records
+  dbuf<uint32_t> rec_len;
+  dbuf<pb_unit1> units1;
+  dbuf<pb_unit2> units2;
+  dbuf<uint32_t> split_bin;             // [nsplit] bins summed by several units
+  dbuf<unsigned long long> acc;         // [2 * nsplit << dst_bits] lo, hi words
+  uint64_t bytes() const {
+    return src_local.bytes() + dst_local.bytes() + w.bytes() + values.bytes() + rec_start.bytes() +
+           rec_len.bytes() + units1.bytes() + units2.bytes() + split_bin.bytes() + acc.bytes();
+  }
+};
+}  // namespace pe
+
 // ---- graph ----------------------------------------------------------------------
 struct pe_graph_s {
   pe_ctx_s* ctx = nullptr;
@@ -159,18 +192,26 @@ struct pe_graph_s {
   pe::dbuf<uint32_t> perm;       // [V]
   pe::dbuf<uint32_t> rank_of;    // [V]
 
+  // pull layout of the FP32 rank iterations
+  int pull_layout = PE_PULL_MERGE_PATH;
+  std::unique_ptr<pe::pb_layout> pb;
+
   const uint64_t* csr_ptr() const { return csr_is_csc ? col_ptr.p : row_ptr.p; }
   const uint32_t* csr_idx() const { return csr_is_csc ? row_idx.p : col_idx.p; }
   uint64_t device_bytes() const {
     return col_ptr.bytes() + row_idx.bytes() + wn.bytes() + od.bytes() + row_ptr.bytes() +
            col_idx.bytes() + perm.bytes() + rank_of.bytes() + tile_row.bytes() + tile_edge.bytes() + fix_row.bytes() +
-           fix_t0.bytes() + fix_t1.bytes();
+           fix_t0.bytes() + fix_t1.bytes() + (pb ? pb->bytes() : 0);
   }
 };
 
 namespace pe {
 // graph.hip
 void graph_build_tiles(pe_graph_s* g, uint32_t tile_items);
+// stable radix sort of packed keys over bits [0, end_bit), optional float payload
+void sort_keys(pe_ctx_s* ctx, dbuf<uint64_t>& keys, dbuf<float>* vals, uint64_t n, int end_bit);
+// pull_binned.hip: builds g->pb (no-op if built with the same options)
+void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts);
 // relabeled graphs: input ids -> internal ids (host arrays), and V-long device
 // arrays downloaded in input id order (elem = 4 or 8 bytes)
 void to_internal_ids(pe_graph_s* g, const uint32_t* host_ids, uint32_t n, uint32_t* host_out);

@@ -82,6 +82,11 @@ __device__ __forceinline__ float batch_apply(const lane_cfg& L, double damping, 
   }
 }
 
+#ifndef PE_BATCH_UNROLL
+#define PE_BATCH_UNROLL 8
+#endif
+static constexpr int BU = PE_BATCH_UNROLL;  // row gathers in flight per wave
+
 __global__ __launch_bounds__(256) void k_batch_tasks(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
     const uint32_t* __restrict__ od, const uint64_t* __restrict__ seedmask,
@@ -108,12 +113,12 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
         const uint64_t eb = col_ptr[v], ee = col_ptr[v + 1];
         float s = 0.f, c = 0.f;
         uint64_t e = eb;
-        for (; e + 8 <= ee; e += 8) {
-          float vals[8];
+        for (; e + BU <= ee; e += BU) {
+          float vals[BU];
 #pragma unroll
-          for (int i = 0; i < 8; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
+          for (int i = 0; i < BU; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
 #pragma unroll
-          for (int i = 0; i < 8; ++i) kahan_add(s, c, vals[i]);
+          for (int i = 0; i < BU; ++i) kahan_add(s, c, vals[i]);
         }
         for (; e < ee; ++e) kahan_add(s, c, X[(uint64_t)row_idx[e] * BK + lane]);
         const uint64_t sm = seedmask[v];
@@ -124,12 +129,12 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
     } else {
       float s = 0.f, c = 0.f;
       uint64_t e = t.e0;
-      for (; e + 8 <= t.e1; e += 8) {
-        float vals[8];
+      for (; e + BU <= t.e1; e += BU) {
+        float vals[BU];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
+        for (int i = 0; i < BU; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) kahan_add(s, c, vals[i]);
+        for (int i = 0; i < BU; ++i) kahan_add(s, c, vals[i]);
       }
       for (; e < t.e1; ++e) kahan_add(s, c, X[(uint64_t)row_idx[e] * BK + lane]);
       partial[(uint64_t)t.chunk * BK + lane] = make_float2(s, c);

@@ -19,6 +19,8 @@
 //   4. the apply (restart + damping, divide by out-degree, delta) runs in FP64
 //      on the finished sum and stores the new FP32 state.
 // Per-column order is fixed by the tiling, so results are bit-reproducible.
+#include <algorithm>
+
 #include "pe_internal.hpp"
 #include "pull.hpp"
 
@@ -66,70 +68,9 @@ __device__ __forceinline__ cpair<T> cadd(cpair<T> a, cpair<T> b) {
   return cpair<T>{s, e + (a.lo + b.lo)};
 }
 
-// ---- apply --------------------------------------------------------------------------
-__device__ __forceinline__ bool seed_bit(const uint64_t* seeds, uint32_t v) {
-  return seeds && ((seeds[v >> 6] >> (v & 63)) & 1ull);
-}
-
-// Finishes row v from its gathered sum; returns its delta contribution and adds
-// the (undivided) rank to rsum.
-template <typename T>
-__device__ __forceinline__ double apply_row(const rank_args& a, bool final_iter, uint32_t v,
-                                            double s, const T* __restrict__ x,
-                                            T* __restrict__ y, const uint32_t* __restrict__ od,
-                                            double& rsum) {
-  if (a.mode == MODE_RAW_SUM) {
-    y[v] = (T)s;
-    return 0.0;
-  }
-  const double restart = a.base + (seed_bit(a.seeds, v) ? a.seed_mass : 0.0);
-  const double r = restart + a.damping * s;
-  rsum += r;
-  if (final_iter) {
-    y[v] = (T)r;
-    return 0.0;
-  }
-  if (a.divide_by_od) {
-    const uint32_t d = od[v];
-    if (d > 0) {
-      const T xn = (T)(r / (double)d);
-      y[v] = xn;
-      return fabs((double)xn - (double)x[v]) * (double)d;
-    }
-  }
-  const T xn = (T)r;
-  y[v] = xn;
-  return fabs((double)xn - (double)x[v]);
-}
-
 template <typename T>
 __device__ __forceinline__ T shfl_up_t(T v, int off) {
   return __shfl_up(v, off, 64);
-}
-
-// block reduce of two doubles; result valid in thread 0
-template <int BLOCK>
-__device__ __forceinline__ void block_reduce2(double& a, double& b, double* sm) {
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
-    b += __shfl_down(b, off, 64);
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) {
-    sm[wave] = a;
-    sm[BLOCK / 64 + wave] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double ta = 0.0, tb = 0.0;
-    for (int w = 0; w < BLOCK / 64; ++w) {
-      ta += sm[w];
-      tb += sm[BLOCK / 64 + w];
-    }
-    a = ta;
-    b = tb;
-  }
 }
 
 // Apply with the row's out-degree and previous state already in registers.
@@ -561,14 +502,26 @@ void rank_work::init(pe_graph_s* g, int precision) {
   fix_grid = persistent_grid(ctx, std::max<uint64_t>(1, (g->num_fix + 255) / 256), 2);
   tile_carry.alloc(ctx, g->num_tiles * pair_bytes);
   tile_head.alloc(ctx, g->num_tiles * pair_bytes);
-  part_delta.alloc(ctx, tile_grid + 2 * fix_grid);
-  part_rsum.alloc(ctx, tile_grid + 2 * fix_grid);
+  pb_parts = 0;
+  if (precision != 64 && g->pull_layout == PE_PULL_BINNED && g->pb)
+    pb_parts = binned_grid_gather(g) + binned_grid_split(g);
+  const size_t parts = std::max<size_t>(tile_grid + 2 * fix_grid, pb_parts);
+  part_delta.alloc(ctx, parts);
+  part_rsum.alloc(ctx, parts);
   scalars.alloc(ctx, 2);
 }
 
 void rank_iteration(pe_graph_s* g, int precision, void* b0, void* b1, bool weighted,
                     const rank_args& a, iter_ctl* ctl, rank_work& wk, double* deltas,
                     double* rsums) {
+  if (precision != 64 && a.mode == MODE_APPLY && g->pull_layout == PE_PULL_BINNED && g->pb) {
+    PE_REQUIRE(wk.pb_parts > 0, PE_ERR_STATE, "pull layout changed after the iteration was set up");
+    binned_iteration(g, (float*)b0, (float*)b1, weighted, a, ctl, wk.part_delta.p, wk.part_rsum.p);
+    k_iter_finish<<<1, 256, 0, g->ctx->stream>>>(wk.part_delta.p, wk.part_rsum.p, wk.pb_parts, ctl, deltas,
+                                                 rsums);
+    PE_HIP(hipGetLastError());
+    return;
+  }
   if (precision == 64)
     launch_iteration_t<double>(g, (double*)b0, (double*)b1, weighted, a, ctl, wk, deltas, rsums);
   else

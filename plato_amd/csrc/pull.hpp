@@ -33,8 +33,70 @@ struct iter_ctl {
   double last_delta;
 };
 
+// ---- apply --------------------------------------------------------------------------
+__device__ __forceinline__ bool seed_bit(const uint64_t* seeds, uint32_t v) {
+  return seeds && ((seeds[v >> 6] >> (v & 63)) & 1ull);
+}
+
+// Finishes row v from its gathered sum; returns its delta contribution and adds
+// the (undivided) rank to rsum.
+template <typename T>
+__device__ __forceinline__ double apply_row(const rank_args& a, bool final_iter, uint32_t v,
+                                            double s, const T* __restrict__ x,
+                                            T* __restrict__ y, const uint32_t* __restrict__ od,
+                                            double& rsum) {
+  if (a.mode == MODE_RAW_SUM) {
+    y[v] = (T)s;
+    return 0.0;
+  }
+  const double restart = a.base + (seed_bit(a.seeds, v) ? a.seed_mass : 0.0);
+  const double r = restart + a.damping * s;
+  rsum += r;
+  if (final_iter) {
+    y[v] = (T)r;
+    return 0.0;
+  }
+  if (a.divide_by_od) {
+    const uint32_t d = od[v];
+    if (d > 0) {
+      const T xn = (T)(r / (double)d);
+      y[v] = xn;
+      return fabs((double)xn - (double)x[v]) * (double)d;
+    }
+  }
+  const T xn = (T)r;
+  y[v] = xn;
+  return fabs((double)xn - (double)x[v]);
+}
+
+// block reduce of two doubles; result valid in thread 0
+template <int BLOCK>
+__device__ __forceinline__ void block_reduce2(double& a, double& b, double* sm) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) {
+    sm[wave] = a;
+    sm[BLOCK / 64 + wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0.0, tb = 0.0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      ta += sm[w];
+      tb += sm[BLOCK / 64 + w];
+    }
+    a = ta;
+    b = tb;
+  }
+}
+
 struct rank_work {
   unsigned tile_grid = 0, fix_grid = 0;
+  unsigned pb_parts = 0;                // partials of a binned iteration (0: not binned)
   dbuf<uint8_t> tile_carry, tile_head;  // compensated pair per tile
   dbuf<double> part_delta, part_rsum;   // per-block partials
   dbuf<double> scalars;                 // delta, rank sum (engine-level calls)
@@ -47,6 +109,12 @@ struct rank_work {
 void rank_iteration(pe_graph_s* g, int precision, void* b0, void* b1, bool weighted,
                     const rank_args& a, iter_ctl* ctl, rank_work& wk, double* deltas,
                     double* rsums);
+// Binned pull (pull_binned.hip): grids, and one FP32 iteration writing its
+// per-block partials (gather grid, then split-apply grid).
+unsigned binned_grid_gather(pe_graph_s* g);
+unsigned binned_grid_split(pe_graph_s* g);
+void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const rank_args& a, iter_ctl* ctl,
+                      double* part_delta, double* part_rsum);
 void rank_apply_only(pe_graph_s* g, int precision, const void* x, void* y, const rank_args& a,
                      rank_work& wk);
 

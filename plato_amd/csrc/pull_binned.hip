@@ -1,0 +1,644 @@
+// Binned (partition-centric) pull: the aggregate_message pass of the FP32 rank
+// iterations (message_passing.hpp:88-144 driven by pagerank.hpp:104-140 and its
+// weighted / personalized variants) with no random global gathers.
+//
+// The merge-path gather (pull.hip) reads x[u] once per edge at random addresses;
+// on a 2^27-vertex graph those 4-byte gathers are bound by L1/L2 miss handling,
+// not by HBM bandwidth.  Here every random access goes to LDS:
+//   scatter: a block stages one chunk of x (2^src_bits sources) in LDS and writes
+//            the contribution x[u] of every edge leaving the chunk into the run of
+//            its destination bin (runs are contiguous, so reads and writes stream);
+//   gather:  a block streams one bin's contributions (2^dst_bits destinations) and
+//            adds them into LDS accumulators, then applies restart, damping and the
+//            out-degree division to the whole bin (the apply of pull.hip, fused).
+// (Beamer, Asanovic, Patterson, "Reducing PageRank communication via propagation
+// blocking", IPDPS 2017; Lakhotia, Kannan, Prasanna, "Accelerating PageRank using
+// partition-centric processing", USENIX ATC 2018.)
+//
+// Sums are exact.  Each contribution (non-negative, finite, below 2^32) becomes a
+// 128-bit fixed-point integer with 96 fractional bits -- exact for FP32 values down
+// to 2^-73 and for the FP64 products w * x of weighted graphs down to 2^-43 relative
+// to their leading bit; lower bits are truncated, deterministically.  Lanes of a wave
+// pre-add runs of equal destinations, then 64-bit LDS atomics (with the carry taken
+// into the high word) accumulate.  Integer addition is associative, so every sum is
+// independent of order, launch shape and layout parameters: results are bit-
+// reproducible.  The bound on the values holds for every rank iteration here: the
+// sum into one vertex is at most the total rank mass, at most V < 2^32.
+//
+// Layout (bin order): every CSC entry (v <- u) gets the key
+//   bin = v >> dst_bits | chunk = u >> src_bits | v & dmask | u & smask
+// and the entries are radix-sorted by it.  A run is the entries of one (bin, chunk)
+// pair.  The scatter walks run records (runs cut to at most run_cap entries) grouped
+// by chunk; the gather walks bins, split into units of at most 2^unit_log2 entries
+// (units of a split bin meet in global 128-bit accumulators, applied by a last pass).
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "pe_internal.hpp"
+#include "pull.hpp"
+
+namespace pe {
+
+static constexpr int PB_THREADS = 1024;
+static constexpr int PB_MAX_SRC_BITS = 15;  // 128 KiB of FP32 sources in LDS
+static constexpr int PB_MAX_DST_BITS = 13;  // 128 KiB of 128-bit accumulators in LDS
+static constexpr int PB_FRAC = 96;          // fractional bits of the fixed-point sums
+static constexpr uint64_t PB_UNIT1 = 1ull << 20;  // max entries per scatter unit
+static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the run scan
+#ifndef PB_SCATTER_RECORDS
+#define PB_SCATTER_RECORDS 4  // run records per wave at a time in the scatter
+#endif
+#ifndef PB_GATHER_UNROLL
+#define PB_GATHER_UNROLL 2    // 8-entry groups per lane at a time in the gather
+#endif
+
+struct u128 {
+  uint64_t lo, hi;
+};
+
+__device__ __forceinline__ u128 add128(u128 a, u128 b) {
+  u128 r;
+  r.lo = a.lo + b.lo;
+  r.hi = a.hi + b.hi + (r.lo < a.lo ? 1ull : 0ull);
+  return r;
+}
+
+// m * 2^k as a 128-bit integer, bits below 2^0 dropped (branch-free); exact for
+// k + bits(m) <= 128, which values below 2^32 satisfy
+__device__ __forceinline__ u128 shl128(uint64_t m, int k) {
+  const uint64_t l_left = m << (k & 63), l_right = m >> ((-k) & 63);
+  const uint64_t h_left = m << ((k - 64) & 63), h_right = m >> ((64 - k) & 63);
+  u128 r;
+  r.lo = (k >= 0) ? ((k < 64) ? l_left : 0ull) : ((k > -64) ? l_right : 0ull);
+  r.hi = (k >= 64) ? ((k < 128) ? h_left : 0ull) : ((k > 0) ? h_right : 0ull);
+  return r;
+}
+
+// FP32 value -> fixed point (zero, subnormals and negative values give 0)
+__device__ __forceinline__ u128 fixed_f32(float v) {
+  const uint32_t b = __float_as_uint(v);
+  const int e = (int)((b >> 23) & 0xFFu);
+  if (e == 0 || (b >> 31)) return u128{0ull, 0ull};
+  const uint64_t m = (uint64_t)((b & 0x7FFFFFu) | 0x800000u);
+  return shl128(m, e - 150 + PB_FRAC);
+}
+
+// FP64 value -> fixed point
+__device__ __forceinline__ u128 fixed_f64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const int e = (int)((b >> 52) & 0x7FFull);
+  if (e == 0 || (b >> 63)) return u128{0ull, 0ull};
+  const uint64_t m = (b & ((1ull << 52) - 1ull)) | (1ull << 52);
+  return shl128(m, e - 1075 + PB_FRAC);
+}
+
+__device__ __forceinline__ double from_fixed(uint64_t lo, uint64_t hi) {
+  return ldexp((double)hi, 64 - PB_FRAC) + ldexp((double)lo, -PB_FRAC);
+}
+
+// 128-bit add into two 64-bit words (LDS or global): the low word's carry goes high
+__device__ __forceinline__ void atomic_add128(unsigned long long* lo, unsigned long long* hi, u128 v) {
+  if (v.lo) {
+    const unsigned long long old = atomicAdd(lo, (unsigned long long)v.lo);
+    if (old + v.lo < old) v.hi += 1ull;
+  }
+  if (v.hi) atomicAdd(hi, (unsigned long long)v.hi);
+}
+
+// ---- scatter ----------------------------------------------------------------------------
+// RB run records per wave at a time (records hold at most run_cap entries; each pass
+// covers 256 entries of every record), so each lane keeps 4 * RB loads in flight.
+template <int RB>
+__global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
+    const pb_unit1* __restrict__ units, uint32_t nunits, const uint64_t* __restrict__ rec_start,
+    const uint32_t* __restrict__ rec_len, const uint16_t* __restrict__ src_local, uint32_t V, int src_bits,
+    const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl, float* __restrict__ values) {
+  __shared__ float xs[1 << PB_MAX_SRC_BITS];
+  const float* x = buf0;
+  if (ctl) {
+    if (ctl->done) return;
+    if (ctl->cur) x = buf1;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int WAVES = PB_THREADS / 64;
+  for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+    const pb_unit1 u = units[ui];
+    const uint64_t base = (uint64_t)u.chunk << src_bits;
+    const uint32_t n = (uint32_t)std::min<uint64_t>(1ull << src_bits, (uint64_t)V - base);
+    if ((n & 3u) == 0) {  // base is a multiple of 16 sources, so x + base is 16-byte aligned
+      const float4* x4 = reinterpret_cast<const float4*>(x + base);
+      float4* s4 = reinterpret_cast<float4*>(xs);
+      for (uint32_t i = tid; i < (n >> 2); i += PB_THREADS) s4[i] = x4[i];
+    } else {
+      for (uint32_t i = tid; i < n; i += PB_THREADS) xs[i] = x[base + i];
+    }
+    __syncthreads();
+    for (uint64_t r = u.r0 + (uint64_t)wave * RB; r < u.r1; r += (uint64_t)WAVES * RB) {
+      uint64_t s[RB];
+      uint32_t len[RB], maxlen = 0;
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const bool ok = r + q < u.r1;
+        s[q] = ok ? rec_start[r + q] : 0ull;
+        len[q] = ok ? rec_len[r + q] : 0u;
+        maxlen = len[q] > maxlen ? len[q] : maxlen;
+      }
+      for (uint32_t k0 = 0; k0 < maxlen; k0 += 256) {
+        uint32_t sl[RB][4];
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t k = k0 + j * 64 + lane;
+            sl[q][j] = k < len[q] ? (uint32_t)src_local[s[q] + k] : 0u;
+          }
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t k = k0 + j * 64 + lane;
+            if (k < len[q]) values[s[q] + k] = xs[sl[q][j]];
+          }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- gather + apply -------------------------------------------------------------------------
+template <bool WEIGHTED>
+__device__ __forceinline__ u128 pb_term(const float* values, const float* w, uint64_t i) {
+  return WEIGHTED ? fixed_f64((double)w[i] * (double)values[i]) : fixed_f32(values[i]);
+}
+
+template <bool WEIGHTED, int GU>
+__global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
+    const pb_unit2* __restrict__ units, uint32_t nunits, const float* __restrict__ values,
+    const float* __restrict__ w, const uint16_t* __restrict__ dst_local, uint32_t V, int dst_bits,
+    float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
+    unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum) {
+  __shared__ unsigned long long slo[1 << PB_MAX_DST_BITS];
+  __shared__ unsigned long long shi[1 << PB_MAX_DST_BITS];
+  __shared__ double s_red[2 * PB_THREADS / 64];
+  bool final_iter = a.final_iter;
+  const float* x = buf0;
+  float* y = buf1;
+  if (ctl) {
+    if (ctl->done) return;
+    final_iter = (ctl->t + 1 == ctl->iters);
+    if (ctl->cur) {
+      x = buf1;
+      y = buf0;
+    }
+  }
+  const int tid = threadIdx.x;
+  const uint32_t pd = 1u << dst_bits;
+  double delta = 0.0, rsum = 0.0;
+  for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+    const pb_unit2 u = units[ui];
+    for (uint32_t i = tid; i < pd; i += PB_THREADS) {
+      slo[i] = 0ull;
+      shi[i] = 0ull;
+    }
+    __syncthreads();
+    // Entries in aligned groups of 8 per lane (one 16-byte load of destinations, two of
+    // values); a lane adds runs of equal destinations in registers and issues one LDS
+    // atomic per destination change.  Unaligned head and tail entries go one by one.
+    const uint64_t a0 = (u.e0 + 7) & ~7ull, a1 = u.e1 & ~7ull;
+    const bool grouped = a0 < a1;
+    {
+      const uint64_t h1 = grouped ? a0 : u.e1;
+      for (uint64_t i = u.e0 + tid; i < h1; i += PB_THREADS)
+        atomic_add128(&slo[dst_local[i]], &shi[dst_local[i]], pb_term<WEIGHTED>(values, w, i));
+      if (grouped)
+        for (uint64_t i = a1 + tid; i < u.e1; i += PB_THREADS)
+          atomic_add128(&slo[dst_local[i]], &shi[dst_local[i]], pb_term<WEIGHTED>(values, w, i));
+    }
+    if (grouped) {
+      const uint64_t ng = (a1 - a0) >> 3;
+      const uint4* dq = reinterpret_cast<const uint4*>(dst_local + a0);
+      const float4* vq = reinterpret_cast<const float4*>(values + a0);
+      const float4* wq = WEIGHTED ? reinterpret_cast<const float4*>(w + a0) : nullptr;
+      for (uint64_t g0 = tid; g0 < ng; g0 += (uint64_t)PB_THREADS * GU) {
+        uint4 dd[GU];
+        float4 va[GU], vb[GU], wa[GU], wb[GU];
+#pragma unroll
+        for (int q = 0; q < GU; ++q) {
+          const uint64_t g = g0 + (uint64_t)q * PB_THREADS;
+          if (g < ng) {
+            dd[q] = dq[g];
+            va[q] = vq[2 * g];
+            vb[q] = vq[2 * g + 1];
+            if (WEIGHTED) {
+              wa[q] = wq[2 * g];
+              wb[q] = wq[2 * g + 1];
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < GU; ++q) {
+          if (g0 + (uint64_t)q * PB_THREADS >= ng) break;
+          const uint32_t d[8] = {dd[q].x & 0xFFFFu, dd[q].x >> 16, dd[q].y & 0xFFFFu, dd[q].y >> 16,
+                                 dd[q].z & 0xFFFFu, dd[q].z >> 16, dd[q].w & 0xFFFFu, dd[q].w >> 16};
+          const float xv[8] = {va[q].x, va[q].y, va[q].z, va[q].w, vb[q].x, vb[q].y, vb[q].z, vb[q].w};
+          float wv[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+          if (WEIGHTED) {
+            wv[0] = wa[q].x; wv[1] = wa[q].y; wv[2] = wa[q].z; wv[3] = wa[q].w;
+            wv[4] = wb[q].x; wv[5] = wb[q].y; wv[6] = wb[q].z; wv[7] = wb[q].w;
+          }
+          uint32_t cd = d[0];
+          u128 acc = WEIGHTED ? fixed_f64((double)wv[0] * (double)xv[0]) : fixed_f32(xv[0]);
+#pragma unroll
+          for (int k = 1; k < 8; ++k) {
+            const u128 f = WEIGHTED ? fixed_f64((double)wv[k] * (double)xv[k]) : fixed_f32(xv[k]);
+            if (d[k] == cd) {
+              acc = add128(acc, f);
+            } else {
+              atomic_add128(&slo[cd], &shi[cd], acc);
+              cd = d[k];
+              acc = f;
+            }
+          }
+          atomic_add128(&slo[cd], &shi[cd], acc);
+        }
+      }
+    }
+    __syncthreads();
+    const uint64_t v0 = (uint64_t)u.bin << dst_bits;
+    if (u.slot == 0xFFFFFFFFu) {
+      for (uint32_t i = tid; i < pd; i += PB_THREADS) {
+        const uint64_t v = v0 + i;
+        if (v >= V) break;
+        delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed(slo[i], shi[i]), x, y, od, rsum);
+      }
+    } else {
+      unsigned long long* alo = acc + ((uint64_t)u.slot << (dst_bits + 1));
+      unsigned long long* ahi = alo + pd;
+      for (uint32_t i = tid; i < pd; i += PB_THREADS) {
+        const u128 s{slo[i], shi[i]};
+        if (s.lo | s.hi) atomic_add128(&alo[i], &ahi[i], s);
+      }
+    }
+    __syncthreads();
+  }
+  block_reduce2<PB_THREADS>(delta, rsum, s_red);
+  if (tid == 0) {
+    part_delta[blockIdx.x] = delta;
+    part_rsum[blockIdx.x] = rsum;
+  }
+}
+
+// Bins summed by several units: apply from the global accumulators, then clear them.
+__global__ __launch_bounds__(256) void k_pb_apply_split(
+    const uint32_t* __restrict__ split_bin, uint32_t nsplit, int dst_bits, uint32_t V,
+    unsigned long long* __restrict__ acc, float* buf0, float* buf1, const uint32_t* __restrict__ od,
+    rank_args a, const iter_ctl* __restrict__ ctl, double* __restrict__ part_delta,
+    double* __restrict__ part_rsum) {
+  __shared__ double s_red[8];
+  bool final_iter = a.final_iter;
+  const float* x = buf0;
+  float* y = buf1;
+  if (ctl) {
+    if (ctl->done) return;
+    final_iter = (ctl->t + 1 == ctl->iters);
+    if (ctl->cur) {
+      x = buf1;
+      y = buf0;
+    }
+  }
+  const uint32_t pd = 1u << dst_bits;
+  const uint64_t n = (uint64_t)nsplit << dst_bits;
+  double delta = 0.0, rsum = 0.0;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t slot = j >> dst_bits, i = j & (pd - 1);
+    unsigned long long* alo = acc + (slot << (dst_bits + 1));
+    const uint64_t lo = alo[i], hi = alo[pd + i];
+    alo[i] = 0ull;
+    alo[pd + i] = 0ull;
+    const uint64_t v = ((uint64_t)split_bin[slot] << dst_bits) + i;
+    if (v < V) delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed(lo, hi), x, y, od, rsum);
+  }
+  block_reduce2<256>(delta, rsum, s_red);
+  if (threadIdx.x == 0) {
+    part_delta[blockIdx.x] = delta;
+    part_rsum[blockIdx.x] = rsum;
+  }
+}
+
+// ---- layout build -----------------------------------------------------------------------------
+// One thread per column; a hub column keeps its thread longer, once per build.
+__global__ void k_pb_keys(const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
+                          int sb, int db, int cbits, uint64_t* __restrict__ keys) {
+  const uint64_t smask = (1ull << sb) - 1ull, dmask = (1ull << db) - 1ull;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t hi_v = ((v >> db) << cbits), lo_v = (v & dmask) << sb;
+    const uint64_t e1 = col_ptr[v + 1];
+    for (uint64_t e = col_ptr[v]; e < e1; ++e) {
+      const uint64_t u = row_idx[e];
+      keys[e] = ((hi_v | (u >> sb)) << (db + sb)) | lo_v | (u & smask);
+    }
+  }
+}
+
+__global__ void k_pb_split(const uint64_t* __restrict__ keys, uint64_t nnz, int sb, int db,
+                           uint16_t* __restrict__ src_local, uint16_t* __restrict__ dst_local) {
+  const uint64_t smask = (1ull << sb) - 1ull, dmask = (1ull << db) - 1ull;
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nnz; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[e];
+    src_local[e] = (uint16_t)(k & smask);
+    dst_local[e] = (uint16_t)((k >> sb) & dmask);
+  }
+}
+
+// Run heads (first entry of each (bin, chunk) pair), counted per segment of PB_SEG entries.
+__device__ __forceinline__ bool pb_head(const uint64_t* keys, uint64_t e, int shift) {
+  return e == 0 || (keys[e] >> shift) != (keys[e - 1] >> shift);
+}
+
+__global__ __launch_bounds__(1024) void k_pb_head_count(const uint64_t* __restrict__ keys, uint64_t nnz, int shift,
+                                                        uint32_t* __restrict__ seg_count) {
+  __shared__ uint32_t s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const uint64_t e0 = (uint64_t)blockIdx.x * PB_SEG, e1 = std::min<uint64_t>(nnz, e0 + PB_SEG);
+  uint32_t c = 0;
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) c += pb_head(keys, e, shift) ? 1u : 0u;
+  atomicAdd(&s_cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) seg_count[blockIdx.x] = s_cnt;
+}
+
+// Writes (position, run id) of every head in order; seg_off is the exclusive scan of seg_count.
+__global__ __launch_bounds__(1024) void k_pb_head_write(const uint64_t* __restrict__ keys, uint64_t nnz, int shift,
+                                                        const uint64_t* __restrict__ seg_off,
+                                                        uint64_t* __restrict__ run_pos, uint64_t* __restrict__ run_id) {
+  __shared__ uint32_t s_wave[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t e0 = (uint64_t)blockIdx.x * PB_SEG, e1 = std::min<uint64_t>(nnz, e0 + PB_SEG);
+  uint64_t out = seg_off[blockIdx.x];
+  for (uint64_t b = e0; b < e1; b += 1024) {
+    const uint64_t e = b + threadIdx.x;
+    const bool h = e < e1 && pb_head(keys, e, shift);
+    const uint64_t m = __ballot(h);
+    const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for (int i = 0; i < 16; ++i) {
+      if (i < wave) wbase += s_wave[i];
+      total += s_wave[i];
+    }
+    if (h) {
+      run_pos[out + wbase + before] = e;
+      run_id[out + wbase + before] = keys[e] >> shift;
+    }
+    out += total;
+    __syncthreads();
+  }
+}
+
+__global__ void k_pb_zero(unsigned long long* __restrict__ p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+static int pb_bits_for(uint64_t n) {
+  int b = 0;
+  while ((1ull << b) < n) ++b;
+  return b;
+}
+
+template <typename T>
+static void upload(pe_ctx_s* ctx, dbuf<T>& d, const std::vector<T>& h) {
+  d.alloc(ctx, std::max<size_t>(1, h.size()));
+  if (!h.empty()) PE_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+}
+
+void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
+  pe_ctx_s* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  pe_binned_opts_t o{15u, 13u, 256u, 20u};
+  if (opts) {
+    if (opts->src_bits) o.src_bits = opts->src_bits;
+    if (opts->dst_bits) o.dst_bits = opts->dst_bits;
+    if (opts->run_cap) o.run_cap = opts->run_cap;
+    if (opts->unit_log2) o.unit_log2 = opts->unit_log2;
+  }
+  PE_REQUIRE(o.src_bits >= 4 && o.src_bits <= (uint32_t)PB_MAX_SRC_BITS, PE_ERR_INVALID, "src_bits must be in [4, 15]");
+  PE_REQUIRE(o.dst_bits >= 4 && o.dst_bits <= (uint32_t)PB_MAX_DST_BITS, PE_ERR_INVALID, "dst_bits must be in [4, 13]");
+  PE_REQUIRE(o.run_cap >= 64, PE_ERR_INVALID, "run_cap must be at least 64");
+  PE_REQUIRE(o.unit_log2 >= 10 && o.unit_log2 <= 30, PE_ERR_INVALID, "unit_log2 must be in [10, 30]");
+  if (g->pb && g->pb->src_bits == o.src_bits && g->pb->dst_bits == o.dst_bits && g->pb->run_cap == o.run_cap &&
+      g->pb->unit_log2 == o.unit_log2)
+    return;
+  g->pb.reset();
+  auto L = std::make_unique<pb_layout>();
+  const int sb = (int)o.src_bits, db = (int)o.dst_bits;
+  L->src_bits = o.src_bits;
+  L->dst_bits = o.dst_bits;
+  L->run_cap = o.run_cap;
+  L->unit_log2 = o.unit_log2;
+  const uint32_t V = g->V;
+  const uint64_t nnz = g->nnz;
+  L->nchunks = (uint32_t)((V + (1ull << sb) - 1) >> sb);
+  L->nbins = (uint32_t)((V + (1ull << db) - 1) >> db);
+  const int cbits = pb_bits_for(L->nchunks), bbits = pb_bits_for(L->nbins);
+  const int shift = sb + db;  // run id = key >> shift = bin << cbits | chunk
+  const int end_bit = bbits + cbits + shift;
+  PE_REQUIRE(end_bit <= 64, PE_ERR_RANGE, "binned key does not fit 64 bits");
+  L->weighted = g->has_weights;
+
+  std::vector<uint64_t> h_pos, h_id;
+  {
+    dbuf<uint64_t> keys(ctx, std::max<uint64_t>(1, nnz));
+    if (nnz) {
+      k_pb_keys<<<grid_for(V, 256, 1u << 20), 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, sb, db, cbits, keys.p);
+      PE_HIP(hipGetLastError());
+    }
+    if (L->weighted) {
+      L->w.alloc(ctx, std::max<uint64_t>(1, nnz));
+      if (nnz) PE_HIP(hipMemcpyAsync(L->w.p, g->wn.p, nnz * sizeof(float), hipMemcpyDeviceToDevice, s));
+      sort_keys(ctx, keys, &L->w, nnz, end_bit);
+    } else {
+      sort_keys(ctx, keys, nullptr, nnz, end_bit);
+    }
+    L->src_local.alloc(ctx, std::max<uint64_t>(1, nnz));
+    L->dst_local.alloc(ctx, std::max<uint64_t>(1, nnz));
+    if (nnz) {
+      k_pb_split<<<grid_for(nnz, 256, 1u << 20), 256, 0, s>>>(keys.p, nnz, sb, db, L->src_local.p, L->dst_local.p);
+      PE_HIP(hipGetLastError());
+      const uint64_t nseg = (nnz + PB_SEG - 1) / PB_SEG;
+      PE_REQUIRE(nseg < 0x7FFFFFFFull, PE_ERR_RANGE, "too many entries for the run scan");
+      dbuf<uint32_t> seg_count(ctx, nseg);
+      k_pb_head_count<<<(unsigned)nseg, 1024, 0, s>>>(keys.p, nnz, shift, seg_count.p);
+      PE_HIP(hipGetLastError());
+      std::vector<uint32_t> h_cnt(nseg);
+      PE_HIP(hipMemcpyAsync(h_cnt.data(), seg_count.p, nseg * 4, hipMemcpyDeviceToHost, s));
+      PE_HIP(hipStreamSynchronize(s));
+      std::vector<uint64_t> h_off(nseg);
+      uint64_t nruns = 0;
+      for (uint64_t i = 0; i < nseg; ++i) {
+        h_off[i] = nruns;
+        nruns += h_cnt[i];
+      }
+      dbuf<uint64_t> seg_off(ctx, nseg), run_pos(ctx, nruns), run_id(ctx, nruns);
+      PE_HIP(hipMemcpyAsync(seg_off.p, h_off.data(), nseg * 8, hipMemcpyHostToDevice, s));
+      k_pb_head_write<<<(unsigned)nseg, 1024, 0, s>>>(keys.p, nnz, shift, seg_off.p, run_pos.p, run_id.p);
+      PE_HIP(hipGetLastError());
+      h_pos.resize(nruns);
+      h_id.resize(nruns);
+      PE_HIP(hipMemcpyAsync(h_pos.data(), run_pos.p, nruns * 8, hipMemcpyDeviceToHost, s));
+      PE_HIP(hipMemcpyAsync(h_id.data(), run_id.p, nruns * 8, hipMemcpyDeviceToHost, s));
+      PE_HIP(hipStreamSynchronize(s));
+    }
+  }
+
+  // host: chunk-major run records, scatter units, bin ranges, gather units
+  const uint64_t nruns = h_pos.size();
+  const uint64_t cmask = (1ull << cbits) - 1ull;
+  std::vector<uint64_t> rec_per_chunk(L->nchunks + 1, 0);
+  for (uint64_t r = 0; r < nruns; ++r) {
+    const uint64_t len = (r + 1 < nruns ? h_pos[r + 1] : nnz) - h_pos[r];
+    rec_per_chunk[(h_id[r] & cmask) + 1] += (len + o.run_cap - 1) / o.run_cap;
+  }
+  for (uint32_t c = 0; c < L->nchunks; ++c) rec_per_chunk[c + 1] += rec_per_chunk[c];
+  const uint64_t nrec = rec_per_chunk[L->nchunks];
+  L->nrecords = nrec;
+  std::vector<uint64_t> rs(nrec);
+  std::vector<uint32_t> rl(nrec);
+  {
+    std::vector<uint64_t> fill(rec_per_chunk.begin(), rec_per_chunk.end() - 1);
+    for (uint64_t r = 0; r < nruns; ++r) {  // bin-major, so records stay in bin order inside a chunk
+      const uint64_t end = r + 1 < nruns ? h_pos[r + 1] : nnz;
+      uint64_t& f = fill[h_id[r] & cmask];
+      for (uint64_t p = h_pos[r]; p < end; p += o.run_cap) {
+        rs[f] = p;
+        rl[f] = (uint32_t)std::min<uint64_t>(o.run_cap, end - p);
+        ++f;
+      }
+    }
+  }
+  std::vector<pb_unit1> u1;
+  std::vector<uint64_t> u1_size;
+  for (uint32_t c = 0; c < L->nchunks; ++c) {
+    uint64_t r = rec_per_chunk[c], acc = 0, r0 = r;
+    for (; r < rec_per_chunk[c + 1]; ++r) {
+      if (acc > 0 && acc + rl[r] > PB_UNIT1) {
+        u1.push_back(pb_unit1{r0, r, c, 0u});
+        u1_size.push_back(acc);
+        r0 = r;
+        acc = 0;
+      }
+      acc += rl[r];
+    }
+    if (acc > 0) {
+      u1.push_back(pb_unit1{r0, r, c, 0u});
+      u1_size.push_back(acc);
+    }
+  }
+  std::vector<uint64_t> bin_ptr(L->nbins + 1, nnz);
+  {
+    uint32_t b = 0;
+    for (uint64_t r = 0; r < nruns; ++r) {
+      const uint32_t rb = (uint32_t)(h_id[r] >> cbits);
+      while (b <= rb) bin_ptr[b++] = h_pos[r];
+    }
+    while (b <= L->nbins) bin_ptr[b++] = nnz;
+  }
+  const uint64_t unit2 = 1ull << o.unit_log2;
+  std::vector<pb_unit2> u2;
+  std::vector<uint32_t> split;
+  for (uint32_t b = 0; b < L->nbins; ++b) {
+    const uint64_t e0 = bin_ptr[b], e1 = bin_ptr[b + 1];
+    if (e1 - e0 <= unit2) {
+      u2.push_back(pb_unit2{e0, e1, b, 0xFFFFFFFFu});
+    } else {
+      const uint32_t slot = (uint32_t)split.size();
+      split.push_back(b);
+      for (uint64_t e = e0; e < e1; e += unit2) u2.push_back(pb_unit2{e, std::min(e1, e + unit2), b, slot});
+    }
+  }
+  // largest units first: persistent blocks take units round-robin
+  {
+    std::vector<uint32_t> idx(u1.size());
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return u1_size[a] > u1_size[b]; });
+    std::vector<pb_unit1> t(u1.size());
+    for (size_t i = 0; i < idx.size(); ++i) t[i] = u1[idx[i]];
+    u1.swap(t);
+    std::stable_sort(u2.begin(), u2.end(),
+                     [](const pb_unit2& a, const pb_unit2& b) { return (a.e1 - a.e0) > (b.e1 - b.e0); });
+  }
+  PE_REQUIRE(u1.size() < 0xFFFFFFFFull && u2.size() < 0xFFFFFFFFull, PE_ERR_RANGE, "too many binned units");
+  L->n1 = (uint32_t)u1.size();
+  L->n2 = (uint32_t)u2.size();
+  L->nsplit = (uint32_t)split.size();
+  upload(ctx, L->rec_start, rs);
+  upload(ctx, L->rec_len, rl);
+  upload(ctx, L->units1, u1);
+  upload(ctx, L->units2, u2);
+  upload(ctx, L->split_bin, split);
+  L->values.alloc(ctx, std::max<uint64_t>(1, nnz));
+  const uint64_t nacc = std::max<uint64_t>(1, (uint64_t)L->nsplit << (db + 1));
+  L->acc.alloc(ctx, nacc);
+  k_pb_zero<<<grid_for(nacc, 256, 65536), 256, 0, s>>>(L->acc.p, nacc);
+  PE_HIP(hipGetLastError());
+  PE_HIP(hipStreamSynchronize(s));
+  g->pb = std::move(L);
+}
+
+unsigned binned_grid_gather(pe_graph_s* g) {
+  return persistent_grid(g->ctx, std::max<uint32_t>(1, g->pb->n2), 1);
+}
+
+unsigned binned_grid_split(pe_graph_s* g) {
+  return grid_for((uint64_t)g->pb->nsplit << g->pb->dst_bits, 256, (unsigned)std::max(1, g->ctx->num_cus) * 4);
+}
+
+void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const rank_args& a, iter_ctl* ctl,
+                      double* part_delta, double* part_rsum) {
+  PE_REQUIRE(g->pb, PE_ERR_STATE, "binned layout not built");
+  const pb_layout& L = *g->pb;
+  PE_REQUIRE(!weighted || L.weighted, PE_ERR_STATE, "weighted iteration on a layout built without weights");
+  hipStream_t s = g->ctx->stream;
+  const unsigned g1 = persistent_grid(g->ctx, std::max<uint32_t>(1, L.n1), 1);
+  const unsigned g2 = binned_grid_gather(g), gs = binned_grid_split(g);
+  {
+    ktime_scope kt(g->ctx, PE_KT_PULL_BIN_SCATTER);
+    k_pb_scatter<PB_SCATTER_RECORDS><<<g1, PB_THREADS, 0, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_local.p, g->V,
+                                           (int)L.src_bits, b0, b1, ctl, L.values.p);
+    PE_HIP(hipGetLastError());
+  }
+  {
+    ktime_scope kt(g->ctx, PE_KT_PULL_BIN_GATHER);
+    if (weighted)
+      k_pb_gather<true, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, g->V,
+                                                  (int)L.dst_bits, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
+                                                  part_rsum);
+    else
+      k_pb_gather<false, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, g->V,
+                                                   (int)L.dst_bits, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
+                                                   part_rsum);
+    PE_HIP(hipGetLastError());
+    k_pb_apply_split<<<gs, 256, 0, s>>>(L.split_bin.p, L.nsplit, (int)L.dst_bits, g->V, L.acc.p, b0, b1, g->od.p,
+                                        a, ctl, part_delta + g2, part_rsum + g2);
+    PE_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace pe
+
+extern "C" int pe_graph_set_pull_layout(pe_graph_t* g, int layout, const pe_binned_opts_t* opts) {
+  if (!g) return PE_ERR_INVALID;
+  return pe::guarded(g->ctx, [&] {
+    if (layout == PE_PULL_MERGE_PATH) {
+      g->pull_layout = PE_PULL_MERGE_PATH;
+      g->pb.reset();
+      return;
+    }
+    PE_REQUIRE(layout == PE_PULL_BINNED, PE_ERR_INVALID, "layout must be PE_PULL_MERGE_PATH or PE_PULL_BINNED");
+    pe::binned_build(g, opts);
+    g->pull_layout = PE_PULL_BINNED;
+  });
+}

@@ -187,6 +187,15 @@ class Graph:
         check(self.lib.pe_graph_set_tile(self.h, items), self.ctx.h)
         check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
 
+    def set_pull_layout(self, layout, src_bits=0, dst_bits=0, run_cap=0, unit_log2=0):
+        """Pull layout of the FP32 rank iterations: "merge_path" (CSC gather) or
+        "binned" (partition-centric, exact sums; builds a bin-ordered copy of the
+        edges).  Zero options keep the library defaults."""
+        code = {"merge_path": L.PE_PULL_MERGE_PATH, "binned": L.PE_PULL_BINNED}[layout]
+        o = L.BinnedOpts(src_bits, dst_bits, run_cap, unit_log2)
+        check(self.lib.pe_graph_set_pull_layout(self.h, code, C.byref(o)), self.ctx.h)
+        check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
+
     def autotune(self, pull=True, batch=False, bfs=False):
         """Time the engine's tunables on this graph (pe_autotune); the pull tile
         choice is applied to the graph.  Returns the chosen values."""
