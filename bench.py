@@ -4,11 +4,13 @@
 A step is one PageRank pull iteration (gather over every in-edge + fused
 apply) over an RMAT graph resident in HBM: scale 27, edge factor 16,
 a/b/c/d = .57/.19/.19/.05, 2^27 ids, 2^31 directed edges, FP32 state with
-compensated sums.  `value` is GTEPS = edges / step time.
+exact (binned) or compensated (merge-path) sums.  Both pull layouts are timed;
+`value` is GTEPS = edges / step time of the faster one.
 
 Printed as one JSON line (rank 0), with
-  roofline     achieved HBM GB/s of the pull tile kernel (algorithmic bytes per
-               launch / average launch time, HIP events on the library stream)
+  roofline     achieved HBM GB/s of the pull kernels of the headline layout
+               (algorithmic bytes per iteration / average kernel time per
+               iteration, HIP events on the library stream)
   cpu_baseline the FP64 CPU oracle (OpenMP restatement of the reference's pull
                loop) timed on a bounded slice of the same graph on the host
   accuracy     L1 / L-infinity of a full FP32 run against the FP64 run
@@ -65,10 +67,10 @@ def cpu_baseline(g, seconds_target=10.0, threads=None):
                       f"{spent:.1f} s"}
 
 
-def profiled_traffic(scale):
-    """HBM bytes per launch of k_pull_tiles from the committed rocprofv3 summary of this workload."""
+def profiled_traffic(kernel, scale):
+    """HBM bytes per pull iteration from the committed rocprofv3 summary of this workload."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"k_pull_tiles_rmat{scale}.md")
+                        f"{kernel}_rmat{scale}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -77,11 +79,48 @@ def profiled_traffic(scale):
     return None
 
 
+def measure_pull(ctx, g, layout, steps, warmup):
+    """Time `steps` FP32 PageRank iterations on one pull layout: per-kernel HIP events on the
+    library stream (timed run), then the same iterations replayed from a captured graph."""
+    from plato_amd import _lib as L
+    from plato_amd.algo import pagerank, pagerank_opts_t
+    t0 = time.time()
+    g.set_pull_layout(layout)
+    layout_s = time.time() - t0
+    if warmup:
+        pagerank(g, pagerank_opts_t(warmup, 0.85, 0.0))
+    kts = ([L.PE_KT_PULL_TILE, L.PE_KT_PULL_FIXUP] if layout == "merge_path"
+           else [L.PE_KT_PULL_BIN_SCATTER, L.PE_KT_PULL_BIN_GATHER])
+    ctx.set_kernel_timing(True)
+    for k in kts:
+        ctx.kernel_time(k)
+    ctx.sync()
+    res = pagerank(g, pagerank_opts_t(steps, 0.85, 0.0))
+    ctx.sync()
+    times = [ctx.kernel_time(k) for k in kts]
+    ctx.set_kernel_timing(False)
+    avg = [ms / max(n, 1) for ms, n in times]
+    res_g = pagerank(g, pagerank_opts_t(steps, 0.85, 0.0))
+    out = {"ms_per_step": res.seconds * 1e3 / steps, "graph_ms": res_g.seconds * 1e3 / steps}
+    if layout == "merge_path":
+        out.update(kernel="k_pull_tiles", kernel_ms=avg[0],
+                   extra={"fixup_avg_ms": round(avg[1], 4)})
+    else:
+        info = g.info
+        out.update(kernel="k_pb_scatter+k_pb_gather", kernel_ms=avg[0] + avg[1],
+                   extra={"scatter_avg_ms": round(avg[0], 4), "gather_avg_ms": round(avg[1], 4),
+                          "layout_build_s": round(layout_s, 2), "run_records": int(info.binned_runs),
+                          "layout_bytes": int(info.binned_bytes)})
+    log(f"[bench] {layout}: {out['ms_per_step']:.3f} ms/iteration (timed), {out['graph_ms']:.3f} ms "
+        f"(graph replay), kernel {out['kernel_ms']:.3f} ms")
+    return out
+
+
 def run_bfs(ctx, scale, roots, seed=11):
     from plato_amd import Graph
     from plato_amd.algo import bfs_opts_t, breadth_first_search
     t0 = time.time()
-    g = Graph.rmat(ctx, scale, 16, seed=seed, directed=False)
+    g = Graph.rmat(ctx, scale, 16, seed=seed, directed=False, relabel=True)
     build_s = time.time() - t0
     od = g.out_degrees()
     rng = np.random.default_rng(seed)
@@ -96,7 +135,8 @@ def run_bfs(ctx, scale, roots, seed=11):
     teps = np.array(teps)
     out = {"scale": scale, "roots": roots, "harmonic_mean_gteps": float(len(teps) / np.sum(1 / teps)),
            "min_gteps": float(teps.min()), "max_gteps": float(teps.max()),
-           "mean_ms": float(np.mean(secs) * 1e3), "build_s": round(build_s, 2)}
+           "mean_ms": float(np.mean(secs) * 1e3), "build_s": round(build_s, 2),
+           "relabel_by_degree": True}
     g.close()
     return out
 
@@ -195,6 +235,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tile", type=int, default=0, help="merge-path items per tile (0 = default)")
     ap.add_argument("--no-relabel", action="store_true", help="keep input ids (no degree renumbering)")
+    ap.add_argument("--pull-layout", choices=["both", "merge_path", "binned"], default="both",
+                    help="pull layouts to time; the faster one is the headline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference dispatch for traffic counters)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -215,7 +257,6 @@ def main():
         sys.exit(2)
 
     from plato_amd import Context, Graph
-    from plato_amd._lib import PE_KT_PULL_TILE, PE_KT_PULL_FIXUP
     from plato_amd.algo import pagerank, pagerank_opts_t
 
     ctx = Context(0)
@@ -238,32 +279,29 @@ def main():
     log(f"[bench] V={V} E={E} built in {build_s:.1f}s tiles={g.info.num_tiles} "
         f"fixups={g.info.num_fixup_rows}")
 
-    opts_w = pagerank_opts_t(args.warmup, 0.85, 0.0)
-    if args.warmup:
-        pagerank(g, opts_w)
-    # timed region: K iterations, per-kernel events on the library stream
-    ctx.set_kernel_timing(True)
-    ctx.kernel_time(PE_KT_PULL_TILE)
-    ctx.kernel_time(PE_KT_PULL_FIXUP)
-    ctx.sync()
-    res = pagerank(g, pagerank_opts_t(args.steps, 0.85, 0.0))
-    ctx.sync()
-    tile_ms, tile_n = ctx.kernel_time(PE_KT_PULL_TILE)
-    fix_ms, fix_n = ctx.kernel_time(PE_KT_PULL_FIXUP)
-    ctx.set_kernel_timing(False)
-    ms_per_step = res.seconds * 1e3 / args.steps
+    layouts = ["merge_path", "binned"] if args.pull_layout == "both" else [args.pull_layout]
+    runs = {lay: measure_pull(ctx, g, lay, args.steps, args.warmup) for lay in layouts}
+    best = min(runs, key=lambda k: runs[k]["ms_per_step"])
+    head = runs[best]
+    g.set_pull_layout(best)
+    ms_per_step = head["ms_per_step"]
     gteps = E / (ms_per_step * 1e-3) / 1e9
-    # the same iterations replayed from a captured graph (no per-kernel events)
-    res_g = pagerank(g, pagerank_opts_t(args.steps, 0.85, 0.0))
-    ms_graph = res_g.seconds * 1e3 / args.steps
-    log(f"[bench] pull iteration {ms_per_step:.3f} ms ({gteps:.1f} GTEPS); graph replay "
-        f"{ms_graph:.3f} ms; tile kernel {tile_ms / max(tile_n, 1):.3f} ms x {tile_n}")
-
     # algorithmic bytes of one pull iteration: col_ptr 8(V+1), row_idx 4E,
     # x read 4V, out-degree 4V, new state 4V
     alg_bytes = 8 * (V + 1) + 4 * E + 12 * V
-    tile_avg_s = tile_ms / max(tile_n, 1) * 1e-3
-    achieved = alg_bytes / tile_avg_s / 1e9
+    kern_s = head["kernel_ms"] * 1e-3
+    achieved = alg_bytes / kern_s / 1e9
+    if best == "merge_path":
+        traffic = profiled_traffic("k_pull_tiles", args.scale) if (not args.tile and not args.no_relabel) else None
+        tsrc = f"profiles/k_pull_tiles_rmat{args.scale}.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+    else:
+        traffic = profiled_traffic("k_pb_scatter_gather", args.scale) if not args.no_relabel else None
+        tsrc = f"profiles/k_pb_scatter_gather_rmat{args.scale}.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
+            "kernel": head["kernel"], "alg_bytes_per_launch": alg_bytes,
+            "avg_launch_ms": round(head["kernel_ms"], 4)}
+    roof.update(head["extra"])
     out = {
         "metric": METRIC,
         "value": round(gteps, 2),
@@ -280,18 +318,13 @@ def main():
         "config": {"workload": f"pagerank_pull_iteration_rmat{args.scale}", "scale": args.scale,
                    "edge_factor": args.edge_factor, "V": V, "E": E, "seed": args.seed,
                    "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "damping": 0.85,
-                   "state": "fp32 rank/od, compensated sums, fp64 apply",
-                   "tile_items": g.info.tile_items, "relabel_by_degree": bool(g.info.relabeled),
-                   "graph_build_s": round(build_s, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (profiled_traffic(args.scale)
-                                 if (not args.tile and not args.no_relabel) else None),
-                     "traffic_source": f"profiles/k_pull_tiles_rmat{args.scale}.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "kernel": "k_pull_tiles", "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": round(tile_avg_s * 1e3, 4),
-                     "fixup_avg_ms": round(fix_ms / max(fix_n, 1), 4)},
-        "graph_replay_ms_per_step": round(ms_graph, 4),
+                   "state": "fp32 rank/od, fp64 apply",
+                   "pull_layout": best, "tile_items": g.info.tile_items,
+                   "relabel_by_degree": bool(g.info.relabeled), "graph_build_s": round(build_s, 2)},
+        "roofline": roof,
+        "graph_replay_ms_per_step": round(head["graph_ms"], 4),
+        "pull_layouts": {k: {kk: vv for kk, vv in r.items() if kk != "extra"} | r["extra"]
+                         for k, r in runs.items()},
     }
     if not args.no_accuracy:
         log("[bench] accuracy: FP32 vs FP64 runs to delta/sum < 1e-7 and 1e-12")

@@ -47,10 +47,10 @@ static constexpr int PB_FRAC = 96;          // fractional bits of the fixed-poin
 static constexpr uint64_t PB_UNIT1 = 1ull << 20;  // max entries per scatter unit
 static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the run scan
 #ifndef PB_SCATTER_RECORDS
-#define PB_SCATTER_RECORDS 4  // run records per wave at a time in the scatter
+#define PB_SCATTER_RECORDS 8  // run records per wave at a time in the scatter
 #endif
 #ifndef PB_GATHER_UNROLL
-#define PB_GATHER_UNROLL 2    // 8-entry groups per lane at a time in the gather
+#define PB_GATHER_UNROLL 4    // 8-entry groups per lane at a time in the gather
 #endif
 
 struct u128 {
