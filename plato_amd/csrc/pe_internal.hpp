@@ -133,23 +133,28 @@ struct pb_unit2 {  // gather work: entries [e0, e1) of one bin; slot ~0u = the w
   uint32_t bin, slot;
 };
 struct pb_layout {
-  uint32_t src_bits = 15, dst_bits = 13, run_cap = 2048, unit_log2 = 20;
+  uint32_t src_bits = 15, dst_bits = 13, run_cap = 256, unit_log2 = 20;
   uint32_t nchunks = 0, nbins = 0;
   uint64_t nrecords = 0;
+  uint64_t nupdates = 0;                // distinct (source, bin) pairs
   uint32_t n1 = 0, n2 = 0, nsplit = 0;
   bool weighted = false;
-  dbuf<uint16_t> src_local, dst_local;  // [nnz] bin order: (bin, chunk, dst, src)
+  dbuf<uint16_t> src_upd;               // [nupdates] source (in its chunk) of each update
+  dbuf<uint16_t> dst_local;             // [nnz] bin order: destination in its bin, bit 15 =
+                                        // first entry of a new update
+  dbuf<uint32_t> gbase;                 // [nnz/8 + 1] updates before each aligned group of 8
   dbuf<float> w;                        // [nnz] bin order, weighted graphs
-  dbuf<float> values;                   // [nnz] contributions of the current iteration
-  dbuf<uint64_t> rec_start;             // [nrecords] chunk-major run records
+  dbuf<float> values;                   // [nupdates] contributions of the current iteration
+  dbuf<uint64_t> rec_start;             // [nrecords] chunk-major run records (update space)
   dbuf<uint32_t> rec_len;
   dbuf<pb_unit1> units1;
   dbuf<pb_unit2> units2;
   dbuf<uint32_t> split_bin;             // [nsplit] bins summed by several units
   dbuf<unsigned long long> acc;         // [2 * nsplit << dst_bits] lo, hi words
   uint64_t bytes() const {
-    return src_local.bytes() + dst_local.bytes() + w.bytes() + values.bytes() + rec_start.bytes() +
-           rec_len.bytes() + units1.bytes() + units2.bytes() + split_bin.bytes() + acc.bytes();
+    return src_upd.bytes() + dst_local.bytes() + gbase.bytes() + w.bytes() + values.bytes() +
+           rec_start.bytes() + rec_len.bytes() + units1.bytes() + units2.bytes() + split_bin.bytes() +
+           acc.bytes();
   }
 };
 }  // namespace pe

@@ -86,6 +86,40 @@ __device__ __forceinline__ float batch_apply(const lane_cfg& L, double damping, 
 #define PE_BATCH_UNROLL 8
 #endif
 static constexpr int BU = PE_BATCH_UNROLL;  // row gathers in flight per wave
+// Cache hints: the new block Y and the edge list stream through once per pass, so
+// they are stored / loaded non-temporally and leave the caches to the hot rows of X
+// (with degree relabeling the first 2^20 rows cover ~88% of the gathers at scale 24).
+// Rows at or above PE_BATCH_COLD_ROW are loaded non-temporally too (0 = never).
+#ifndef PE_BATCH_NT_STREAM
+#define PE_BATCH_NT_STREAM 0
+#endif
+#ifndef PE_BATCH_COLD_ROW
+#define PE_BATCH_COLD_ROW 0
+#endif
+
+__device__ __forceinline__ uint32_t ld_edge(const uint32_t* p) {
+#if PE_BATCH_NT_STREAM
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
+__device__ __forceinline__ void st_row(float* p, float v) {
+#if PE_BATCH_NT_STREAM
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
+__device__ __forceinline__ float ld_row(const float* X, uint32_t u, int lane) {
+  const float* p = X + (uint64_t)u * BK + lane;
+#if PE_BATCH_COLD_ROW
+  if (u >= (uint32_t)PE_BATCH_COLD_ROW) return __builtin_nontemporal_load(p);
+#endif
+  return *p;
+}
 
 __global__ __launch_bounds__(256) void k_batch_tasks(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
@@ -116,15 +150,15 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
         for (; e + BU <= ee; e += BU) {
           float vals[BU];
 #pragma unroll
-          for (int i = 0; i < BU; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
+          for (int i = 0; i < BU; ++i) vals[i] = ld_row(X, ld_edge(row_idx + e + i), lane);
 #pragma unroll
           for (int i = 0; i < BU; ++i) kahan_add(s, c, vals[i]);
         }
-        for (; e < ee; ++e) kahan_add(s, c, X[(uint64_t)row_idx[e] * BK + lane]);
+        for (; e < ee; ++e) kahan_add(s, c, ld_row(X, ld_edge(row_idx + e), lane));
         const uint64_t sm = seedmask[v];
         const float xold = X[(uint64_t)v * BK + lane];
-        Y[(uint64_t)v * BK + lane] =
-            batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta);
+        st_row(Y + (uint64_t)v * BK + lane,
+               batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta));
       }
     } else {
       float s = 0.f, c = 0.f;
@@ -132,11 +166,11 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
       for (; e + BU <= t.e1; e += BU) {
         float vals[BU];
 #pragma unroll
-        for (int i = 0; i < BU; ++i) vals[i] = X[(uint64_t)row_idx[e + i] * BK + lane];
+        for (int i = 0; i < BU; ++i) vals[i] = ld_row(X, ld_edge(row_idx + e + i), lane);
 #pragma unroll
         for (int i = 0; i < BU; ++i) kahan_add(s, c, vals[i]);
       }
-      for (; e < t.e1; ++e) kahan_add(s, c, X[(uint64_t)row_idx[e] * BK + lane]);
+      for (; e < t.e1; ++e) kahan_add(s, c, ld_row(X, ld_edge(row_idx + e), lane));
       partial[(uint64_t)t.chunk * BK + lane] = make_float2(s, c);
     }
   }
@@ -199,14 +233,25 @@ __global__ __launch_bounds__(256) void k_batch_fixup(
 }
 
 // Per-column delta, state transitions, buffer flip.  One block of 64 threads.
-__global__ void k_batch_finish(const double* __restrict__ part_delta, uint32_t nparts,
+static constexpr int FIN_GROUPS = 16;
+
+__global__ __launch_bounds__(FIN_GROUPS * BK) void k_batch_finish(const double* __restrict__ part_delta, uint32_t nparts,
                                batch_ctl* __restrict__ ctl, uint32_t* __restrict__ status,
                                uint32_t* __restrict__ iters, double* __restrict__ last_delta,
                                double tol, uint32_t max_iters) {
-  const int j = threadIdx.x;
-  if (j < BK) {
+  // FIN_GROUPS groups of 64 lanes each sum every FIN_GROUPS-th block's partials
+  // (fixed order), then group sums are combined in group order: deterministic
+  __shared__ double s_sum[FIN_GROUPS][BK];
+  const int j = threadIdx.x & (BK - 1), grp = threadIdx.x / BK;
+  {
     double d = 0.0;
-    for (uint32_t b = 0; b < nparts; ++b) d += part_delta[(uint64_t)b * BK + j];
+    for (uint32_t b = grp; b < nparts; b += FIN_GROUPS) d += part_delta[(uint64_t)b * BK + j];
+    s_sum[grp][j] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x < BK) {
+    double d = 0.0;
+    for (int q = 0; q < FIN_GROUPS; ++q) d += s_sum[q][j];
     const uint32_t st = status[j];
     if (st == COL_LOADING) {
       status[j] = COL_ACTIVE;
@@ -224,7 +269,7 @@ __global__ void k_batch_finish(const double* __restrict__ part_delta, uint32_t n
     }
   }
   __syncthreads();
-  if (j == 0) {
+  if (threadIdx.x == 0) {
     ctl->cur ^= 1u;
     ctl->iteration += 1;
   }
@@ -394,7 +439,7 @@ struct pe_ppr_batch_s {
     k_batch_fixup<<<fix_grid, 256, 0, s>>>(g->od.p, seedmask.p, longs.p, nlong, X0.p, X1.p, ctl.p, status.p,
                                            init_dist.p, opts.damping, partial.p,
                                            part_delta.p + (uint64_t)task_grid * BK);
-    k_batch_finish<<<1, 64, 0, s>>>(part_delta.p, task_grid + fix_grid, ctl.p, status.p, iters.p,
+    k_batch_finish<<<1, FIN_GROUPS * BK, 0, s>>>(part_delta.p, task_grid + fix_grid, ctl.p, status.p, iters.p,
                                     last_delta.p, opts.tol, opts.max_iters);
     PE_HIP(hipGetLastError());
   }

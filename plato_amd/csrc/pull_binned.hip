@@ -26,9 +26,13 @@
 // sum into one vertex is at most the total rank mass, at most V < 2^32.
 //
 // Layout (bin order): every CSC entry (v <- u) gets the key
-//   bin = v >> dst_bits | chunk = u >> src_bits | v & dmask | u & smask
-// and the entries are radix-sorted by it.  A run is the entries of one (bin, chunk)
-// pair.  The scatter walks run records (runs cut to at most run_cap entries) grouped
+//   bin = v >> dst_bits | chunk = u >> src_bits | u & smask | v & dmask
+// and the entries are radix-sorted by it.  An update is one (bin, source) pair: the
+// scatter writes x[u] once per update, not once per edge (partition-centric, as in
+// Lakhotia et al.), and every entry carries its destination plus a flag bit marking
+// the first entry of a new update; a prefix count per aligned group of 8 entries lets
+// the gather find each entry's update.  A run is the updates of one (bin, chunk)
+// pair.  The scatter walks run records (runs cut to at most run_cap updates) grouped
 // by chunk; the gather walks bins, split into units of at most 2^unit_log2 entries
 // (units of a split bin meet in global 128-bit accumulators, applied by a last pass).
 #include <algorithm>
@@ -112,7 +116,7 @@ __device__ __forceinline__ void atomic_add128(unsigned long long* lo, unsigned l
 template <int RB>
 __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
     const pb_unit1* __restrict__ units, uint32_t nunits, const uint64_t* __restrict__ rec_start,
-    const uint32_t* __restrict__ rec_len, const uint16_t* __restrict__ src_local, uint32_t V, int src_bits,
+    const uint32_t* __restrict__ rec_len, const uint16_t* __restrict__ src_upd, uint32_t V, int src_bits,
     const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl, float* __restrict__ values) {
   __shared__ float xs[1 << PB_MAX_SRC_BITS];
   const float* x = buf0;
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const uint32_t k = k0 + j * 64 + lane;
-            sl[q][j] = k < len[q] ? (uint32_t)src_local[s[q] + k] : 0u;
+            sl[q][j] = k < len[q] ? (uint32_t)src_upd[s[q] + k] : 0u;
           }
 #pragma unroll
         for (int q = 0; q < RB; ++q)
@@ -167,15 +171,22 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
 }
 
 // ---- gather + apply -------------------------------------------------------------------------
+// Contribution of entry i (unaligned head or tail of a unit): its update is the last
+// one started at or before i, counted from the prefix of its aligned group.
 template <bool WEIGHTED>
-__device__ __forceinline__ u128 pb_term(const float* values, const float* w, uint64_t i) {
-  return WEIGHTED ? fixed_f64((double)w[i] * (double)values[i]) : fixed_f32(values[i]);
+__device__ __forceinline__ u128 pb_term(const uint16_t* dst_local, const uint32_t* gbase, const float* values,
+                                        const float* w, uint64_t i) {
+  uint32_t cnt = 0;
+  for (uint64_t j = i & ~7ull; j <= i; ++j) cnt += (uint32_t)(dst_local[j] >> 15);
+  const float x = values[gbase[i >> 3] + cnt - 1];
+  return WEIGHTED ? fixed_f64((double)w[i] * (double)x) : fixed_f32(x);
 }
 
 template <bool WEIGHTED, int GU>
 __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
     const pb_unit2* __restrict__ units, uint32_t nunits, const float* __restrict__ values,
-    const float* __restrict__ w, const uint16_t* __restrict__ dst_local, uint32_t V, int dst_bits,
+    const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ gbase,
+    uint32_t V, int dst_bits,
     float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
     unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum) {
   __shared__ unsigned long long slo[1 << PB_MAX_DST_BITS];
@@ -202,56 +213,74 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
       shi[i] = 0ull;
     }
     __syncthreads();
-    // Entries in aligned groups of 8 per lane (one 16-byte load of destinations, two of
-    // values); a lane adds runs of equal destinations in registers and issues one LDS
-    // atomic per destination change.  Unaligned head and tail entries go one by one.
+    // Entries in aligned groups of 8 per lane (one 16-byte load of destinations, the
+    // group's update count, then the 8 values by update index: entries of one update
+    // read the same address); a lane adds runs of equal destinations in registers and
+    // issues one LDS atomic per destination change.  Unaligned head and tail entries go
+    // one by one.
     const uint64_t a0 = (u.e0 + 7) & ~7ull, a1 = u.e1 & ~7ull;
     const bool grouped = a0 < a1;
     {
       const uint64_t h1 = grouped ? a0 : u.e1;
-      for (uint64_t i = u.e0 + tid; i < h1; i += PB_THREADS)
-        atomic_add128(&slo[dst_local[i]], &shi[dst_local[i]], pb_term<WEIGHTED>(values, w, i));
+      for (uint64_t i = u.e0 + tid; i < h1; i += PB_THREADS) {
+        const uint32_t d = dst_local[i] & 0x7FFFu;
+        atomic_add128(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
+      }
       if (grouped)
-        for (uint64_t i = a1 + tid; i < u.e1; i += PB_THREADS)
-          atomic_add128(&slo[dst_local[i]], &shi[dst_local[i]], pb_term<WEIGHTED>(values, w, i));
+        for (uint64_t i = a1 + tid; i < u.e1; i += PB_THREADS) {
+          const uint32_t d = dst_local[i] & 0x7FFFu;
+          atomic_add128(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
+        }
     }
     if (grouped) {
-      const uint64_t ng = (a1 - a0) >> 3;
+      const uint64_t ng = (a1 - a0) >> 3, gfirst = a0 >> 3;
       const uint4* dq = reinterpret_cast<const uint4*>(dst_local + a0);
-      const float4* vq = reinterpret_cast<const float4*>(values + a0);
       const float4* wq = WEIGHTED ? reinterpret_cast<const float4*>(w + a0) : nullptr;
       for (uint64_t g0 = tid; g0 < ng; g0 += (uint64_t)PB_THREADS * GU) {
         uint4 dd[GU];
-        float4 va[GU], vb[GU], wa[GU], wb[GU];
+        uint32_t gb[GU];
+        float4 wa[GU], wb[GU];
 #pragma unroll
         for (int q = 0; q < GU; ++q) {
           const uint64_t g = g0 + (uint64_t)q * PB_THREADS;
           if (g < ng) {
             dd[q] = dq[g];
-            va[q] = vq[2 * g];
-            vb[q] = vq[2 * g + 1];
+            gb[q] = gbase[gfirst + g];
             if (WEIGHTED) {
               wa[q] = wq[2 * g];
               wb[q] = wq[2 * g + 1];
             }
           }
         }
+        float xv[GU][8];
 #pragma unroll
         for (int q = 0; q < GU; ++q) {
           if (g0 + (uint64_t)q * PB_THREADS >= ng) break;
-          const uint32_t d[8] = {dd[q].x & 0xFFFFu, dd[q].x >> 16, dd[q].y & 0xFFFFu, dd[q].y >> 16,
-                                 dd[q].z & 0xFFFFu, dd[q].z >> 16, dd[q].w & 0xFFFFu, dd[q].w >> 16};
-          const float xv[8] = {va[q].x, va[q].y, va[q].z, va[q].w, vb[q].x, vb[q].y, vb[q].z, vb[q].w};
+          const uint32_t raw[8] = {dd[q].x & 0xFFFFu, dd[q].x >> 16, dd[q].y & 0xFFFFu, dd[q].y >> 16,
+                                   dd[q].z & 0xFFFFu, dd[q].z >> 16, dd[q].w & 0xFFFFu, dd[q].w >> 16};
+          uint32_t idx = gb[q] - 1;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            idx += raw[k] >> 15;
+            xv[q][k] = values[idx];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < GU; ++q) {
+          if (g0 + (uint64_t)q * PB_THREADS >= ng) break;
+          const uint32_t d[8] = {dd[q].x & 0x7FFFu, (dd[q].x >> 16) & 0x7FFFu, dd[q].y & 0x7FFFu,
+                                 (dd[q].y >> 16) & 0x7FFFu, dd[q].z & 0x7FFFu, (dd[q].z >> 16) & 0x7FFFu,
+                                 dd[q].w & 0x7FFFu, (dd[q].w >> 16) & 0x7FFFu};
           float wv[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
           if (WEIGHTED) {
             wv[0] = wa[q].x; wv[1] = wa[q].y; wv[2] = wa[q].z; wv[3] = wa[q].w;
             wv[4] = wb[q].x; wv[5] = wb[q].y; wv[6] = wb[q].z; wv[7] = wb[q].w;
           }
           uint32_t cd = d[0];
-          u128 acc = WEIGHTED ? fixed_f64((double)wv[0] * (double)xv[0]) : fixed_f32(xv[0]);
+          u128 acc = WEIGHTED ? fixed_f64((double)wv[0] * (double)xv[q][0]) : fixed_f32(xv[q][0]);
 #pragma unroll
           for (int k = 1; k < 8; ++k) {
-            const u128 f = WEIGHTED ? fixed_f64((double)wv[k] * (double)xv[k]) : fixed_f32(xv[k]);
+            const u128 f = WEIGHTED ? fixed_f64((double)wv[k] * (double)xv[q][k]) : fixed_f32(xv[q][k]);
             if (d[k] == cd) {
               acc = add128(acc, f);
             } else {
@@ -332,68 +361,92 @@ __global__ void k_pb_keys(const uint64_t* __restrict__ col_ptr, const uint32_t* 
                           int sb, int db, int cbits, uint64_t* __restrict__ keys) {
   const uint64_t smask = (1ull << sb) - 1ull, dmask = (1ull << db) - 1ull;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t hi_v = ((v >> db) << cbits), lo_v = (v & dmask) << sb;
+    const uint64_t hi_v = ((v >> db) << cbits), lo_v = v & dmask;
     const uint64_t e1 = col_ptr[v + 1];
     for (uint64_t e = col_ptr[v]; e < e1; ++e) {
       const uint64_t u = row_idx[e];
-      keys[e] = ((hi_v | (u >> sb)) << (db + sb)) | lo_v | (u & smask);
+      keys[e] = ((hi_v | (u >> sb)) << (db + sb)) | ((u & smask) << db) | lo_v;
     }
   }
 }
 
-__global__ void k_pb_split(const uint64_t* __restrict__ keys, uint64_t nnz, int sb, int db,
-                           uint16_t* __restrict__ src_local, uint16_t* __restrict__ dst_local) {
-  const uint64_t smask = (1ull << sb) - 1ull, dmask = (1ull << db) - 1ull;
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nnz; e += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = keys[e];
-    src_local[e] = (uint16_t)(k & smask);
-    dst_local[e] = (uint16_t)((k >> sb) & dmask);
-  }
-}
-
-// Run heads (first entry of each (bin, chunk) pair), counted per segment of PB_SEG entries.
-__device__ __forceinline__ bool pb_head(const uint64_t* keys, uint64_t e, int shift) {
+// First entry of a new update (bin, chunk, source) and of a new run (bin, chunk).
+__device__ __forceinline__ bool pb_new(const uint64_t* keys, uint64_t e, int shift) {
   return e == 0 || (keys[e] >> shift) != (keys[e - 1] >> shift);
 }
 
-__global__ __launch_bounds__(1024) void k_pb_head_count(const uint64_t* __restrict__ keys, uint64_t nnz, int shift,
-                                                        uint32_t* __restrict__ seg_count) {
-  __shared__ uint32_t s_cnt;
-  if (threadIdx.x == 0) s_cnt = 0;
+// Updates and run heads counted per segment of PB_SEG entries.
+__global__ __launch_bounds__(1024) void k_pb_count(const uint64_t* __restrict__ keys, uint64_t nnz, int db, int shift,
+                                                   uint32_t* __restrict__ seg_upd, uint32_t* __restrict__ seg_run) {
+  __shared__ uint32_t s_cnt[2];
+  if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const uint64_t e0 = (uint64_t)blockIdx.x * PB_SEG, e1 = std::min<uint64_t>(nnz, e0 + PB_SEG);
-  uint32_t c = 0;
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) c += pb_head(keys, e, shift) ? 1u : 0u;
-  atomicAdd(&s_cnt, c);
+  uint32_t cu = 0, cr = 0;
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    cu += pb_new(keys, e, db) ? 1u : 0u;
+    cr += pb_new(keys, e, shift) ? 1u : 0u;
+  }
+  atomicAdd(&s_cnt[0], cu);
+  atomicAdd(&s_cnt[1], cr);
   __syncthreads();
-  if (threadIdx.x == 0) seg_count[blockIdx.x] = s_cnt;
+  if (threadIdx.x == 0) {
+    seg_upd[blockIdx.x] = s_cnt[0];
+    seg_run[blockIdx.x] = s_cnt[1];
+  }
 }
 
-// Writes (position, run id) of every head in order; seg_off is the exclusive scan of seg_count.
-__global__ __launch_bounds__(1024) void k_pb_head_write(const uint64_t* __restrict__ keys, uint64_t nnz, int shift,
-                                                        const uint64_t* __restrict__ seg_off,
-                                                        uint64_t* __restrict__ run_pos, uint64_t* __restrict__ run_id) {
-  __shared__ uint32_t s_wave[16];
+// In entry order: destination words with the update flag, update counts of aligned
+// groups, the source of every update, and (update, entry, run id) of every run head.
+// upd_off / run_off are the exclusive scans of seg_upd / seg_run.
+__global__ __launch_bounds__(1024) void k_pb_write(const uint64_t* __restrict__ keys, uint64_t nnz, int sb, int db,
+                                                   const uint64_t* __restrict__ upd_off,
+                                                   const uint64_t* __restrict__ run_off,
+                                                   uint16_t* __restrict__ dst_local, uint32_t* __restrict__ gbase,
+                                                   uint16_t* __restrict__ src_upd, uint64_t* __restrict__ run_upd,
+                                                   uint64_t* __restrict__ run_pos, uint64_t* __restrict__ run_id) {
+  __shared__ uint32_t s_u[16], s_r[16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int shift = sb + db;
+  const uint64_t smask = (1ull << sb) - 1ull, dmask = (1ull << db) - 1ull;
   const uint64_t e0 = (uint64_t)blockIdx.x * PB_SEG, e1 = std::min<uint64_t>(nnz, e0 + PB_SEG);
-  uint64_t out = seg_off[blockIdx.x];
+  uint64_t ou = upd_off[blockIdx.x], orr = run_off[blockIdx.x];
   for (uint64_t b = e0; b < e1; b += 1024) {
     const uint64_t e = b + threadIdx.x;
-    const bool h = e < e1 && pb_head(keys, e, shift);
-    const uint64_t m = __ballot(h);
-    const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    const bool in = e < e1;
+    const bool fu = in && pb_new(keys, e, db);
+    const bool fr = in && pb_new(keys, e, shift);
+    const uint64_t mu = __ballot(fu), mr = __ballot(fr);
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (lane == 0) {
+      s_u[wave] = (uint32_t)__popcll(mu);
+      s_r[wave] = (uint32_t)__popcll(mr);
+    }
     __syncthreads();
-    uint32_t wbase = 0, total = 0;
+    uint32_t bu = 0, br = 0, tu = 0, tr = 0;
     for (int i = 0; i < 16; ++i) {
-      if (i < wave) wbase += s_wave[i];
-      total += s_wave[i];
+      if (i < wave) {
+        bu += s_u[i];
+        br += s_r[i];
+      }
+      tu += s_u[i];
+      tr += s_r[i];
     }
-    if (h) {
-      run_pos[out + wbase + before] = e;
-      run_id[out + wbase + before] = keys[e] >> shift;
+    if (in) {
+      const uint64_t k = keys[e];
+      const uint64_t u_ex = ou + bu + (uint64_t)__popcll(mu & below);  // updates before e
+      dst_local[e] = (uint16_t)((k & dmask) | (fu ? 0x8000ull : 0ull));
+      if ((e & 7ull) == 0) gbase[e >> 3] = (uint32_t)u_ex;
+      if (fu) src_upd[u_ex] = (uint16_t)((k >> db) & smask);
+      if (fr) {
+        const uint64_t r = orr + br + (uint64_t)__popcll(mr & below);
+        run_upd[r] = u_ex;
+        run_pos[r] = e;
+        run_id[r] = k >> shift;
+      }
     }
-    out += total;
+    ou += tu;
+    orr += tr;
     __syncthreads();
   }
 }
@@ -449,7 +502,7 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   PE_REQUIRE(end_bit <= 64, PE_ERR_RANGE, "binned key does not fit 64 bits");
   L->weighted = g->has_weights;
 
-  std::vector<uint64_t> h_pos, h_id;
+  std::vector<uint64_t> h_upd, h_pos, h_id;
   {
     dbuf<uint64_t> keys(ctx, std::max<uint64_t>(1, nnz));
     if (nnz) {
@@ -463,43 +516,53 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
     } else {
       sort_keys(ctx, keys, nullptr, nnz, end_bit);
     }
-    L->src_local.alloc(ctx, std::max<uint64_t>(1, nnz));
     L->dst_local.alloc(ctx, std::max<uint64_t>(1, nnz));
+    L->gbase.alloc(ctx, nnz / 8 + 1);
     if (nnz) {
-      k_pb_split<<<grid_for(nnz, 256, 1u << 20), 256, 0, s>>>(keys.p, nnz, sb, db, L->src_local.p, L->dst_local.p);
-      PE_HIP(hipGetLastError());
       const uint64_t nseg = (nnz + PB_SEG - 1) / PB_SEG;
-      PE_REQUIRE(nseg < 0x7FFFFFFFull, PE_ERR_RANGE, "too many entries for the run scan");
-      dbuf<uint32_t> seg_count(ctx, nseg);
-      k_pb_head_count<<<(unsigned)nseg, 1024, 0, s>>>(keys.p, nnz, shift, seg_count.p);
+      PE_REQUIRE(nseg < 0x7FFFFFFFull, PE_ERR_RANGE, "too many entries for the update scan");
+      dbuf<uint32_t> seg_upd(ctx, nseg), seg_run(ctx, nseg);
+      k_pb_count<<<(unsigned)nseg, 1024, 0, s>>>(keys.p, nnz, db, shift, seg_upd.p, seg_run.p);
       PE_HIP(hipGetLastError());
-      std::vector<uint32_t> h_cnt(nseg);
-      PE_HIP(hipMemcpyAsync(h_cnt.data(), seg_count.p, nseg * 4, hipMemcpyDeviceToHost, s));
+      std::vector<uint32_t> hu(nseg), hr(nseg);
+      PE_HIP(hipMemcpyAsync(hu.data(), seg_upd.p, nseg * 4, hipMemcpyDeviceToHost, s));
+      PE_HIP(hipMemcpyAsync(hr.data(), seg_run.p, nseg * 4, hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
-      std::vector<uint64_t> h_off(nseg);
-      uint64_t nruns = 0;
+      std::vector<uint64_t> ou(nseg), orr(nseg);
+      uint64_t nupd = 0, nruns = 0;
       for (uint64_t i = 0; i < nseg; ++i) {
-        h_off[i] = nruns;
-        nruns += h_cnt[i];
+        ou[i] = nupd;
+        orr[i] = nruns;
+        nupd += hu[i];
+        nruns += hr[i];
       }
-      dbuf<uint64_t> seg_off(ctx, nseg), run_pos(ctx, nruns), run_id(ctx, nruns);
-      PE_HIP(hipMemcpyAsync(seg_off.p, h_off.data(), nseg * 8, hipMemcpyHostToDevice, s));
-      k_pb_head_write<<<(unsigned)nseg, 1024, 0, s>>>(keys.p, nnz, shift, seg_off.p, run_pos.p, run_id.p);
+      PE_REQUIRE(nupd < 0xFFFFFFFFull, PE_ERR_RANGE,
+                 "binned layout: more than 2^32 (source, bin) updates; use the merge-path layout");
+      L->nupdates = nupd;
+      L->src_upd.alloc(ctx, nupd);
+      dbuf<uint64_t> upd_off(ctx, nseg), run_off(ctx, nseg), run_upd(ctx, nruns), run_pos(ctx, nruns),
+          run_id(ctx, nruns);
+      PE_HIP(hipMemcpyAsync(upd_off.p, ou.data(), nseg * 8, hipMemcpyHostToDevice, s));
+      PE_HIP(hipMemcpyAsync(run_off.p, orr.data(), nseg * 8, hipMemcpyHostToDevice, s));
+      k_pb_write<<<(unsigned)nseg, 1024, 0, s>>>(keys.p, nnz, sb, db, upd_off.p, run_off.p, L->dst_local.p,
+                                                 L->gbase.p, L->src_upd.p, run_upd.p, run_pos.p, run_id.p);
       PE_HIP(hipGetLastError());
+      h_upd.resize(nruns);
       h_pos.resize(nruns);
       h_id.resize(nruns);
+      PE_HIP(hipMemcpyAsync(h_upd.data(), run_upd.p, nruns * 8, hipMemcpyDeviceToHost, s));
       PE_HIP(hipMemcpyAsync(h_pos.data(), run_pos.p, nruns * 8, hipMemcpyDeviceToHost, s));
       PE_HIP(hipMemcpyAsync(h_id.data(), run_id.p, nruns * 8, hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
     }
   }
 
-  // host: chunk-major run records, scatter units, bin ranges, gather units
-  const uint64_t nruns = h_pos.size();
+  // host: chunk-major run records over updates, scatter units, bin ranges, gather units
+  const uint64_t nruns = h_pos.size(), nupd = L->nupdates;
   const uint64_t cmask = (1ull << cbits) - 1ull;
   std::vector<uint64_t> rec_per_chunk(L->nchunks + 1, 0);
   for (uint64_t r = 0; r < nruns; ++r) {
-    const uint64_t len = (r + 1 < nruns ? h_pos[r + 1] : nnz) - h_pos[r];
+    const uint64_t len = (r + 1 < nruns ? h_upd[r + 1] : nupd) - h_upd[r];
     rec_per_chunk[(h_id[r] & cmask) + 1] += (len + o.run_cap - 1) / o.run_cap;
   }
   for (uint32_t c = 0; c < L->nchunks; ++c) rec_per_chunk[c + 1] += rec_per_chunk[c];
@@ -510,9 +573,9 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   {
     std::vector<uint64_t> fill(rec_per_chunk.begin(), rec_per_chunk.end() - 1);
     for (uint64_t r = 0; r < nruns; ++r) {  // bin-major, so records stay in bin order inside a chunk
-      const uint64_t end = r + 1 < nruns ? h_pos[r + 1] : nnz;
+      const uint64_t end = r + 1 < nruns ? h_upd[r + 1] : nupd;
       uint64_t& f = fill[h_id[r] & cmask];
-      for (uint64_t p = h_pos[r]; p < end; p += o.run_cap) {
+      for (uint64_t p = h_upd[r]; p < end; p += o.run_cap) {
         rs[f] = p;
         rl[f] = (uint32_t)std::min<uint64_t>(o.run_cap, end - p);
         ++f;
@@ -579,7 +642,7 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   upload(ctx, L->units1, u1);
   upload(ctx, L->units2, u2);
   upload(ctx, L->split_bin, split);
-  L->values.alloc(ctx, std::max<uint64_t>(1, nnz));
+  L->values.alloc(ctx, std::max<uint64_t>(1, nupd));
   const uint64_t nacc = std::max<uint64_t>(1, (uint64_t)L->nsplit << (db + 1));
   L->acc.alloc(ctx, nacc);
   k_pb_zero<<<grid_for(nacc, 256, 65536), 256, 0, s>>>(L->acc.p, nacc);
@@ -606,18 +669,18 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
   const unsigned g2 = binned_grid_gather(g), gs = binned_grid_split(g);
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_SCATTER);
-    k_pb_scatter<PB_SCATTER_RECORDS><<<g1, PB_THREADS, 0, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_local.p, g->V,
+    k_pb_scatter<PB_SCATTER_RECORDS><<<g1, PB_THREADS, 0, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V,
                                            (int)L.src_bits, b0, b1, ctl, L.values.p);
     PE_HIP(hipGetLastError());
   }
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_GATHER);
     if (weighted)
-      k_pb_gather<true, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, g->V,
+      k_pb_gather<true, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
                                                   (int)L.dst_bits, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                   part_rsum);
     else
-      k_pb_gather<false, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, g->V,
+      k_pb_gather<false, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
                                                    (int)L.dst_bits, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                    part_rsum);
     PE_HIP(hipGetLastError());
