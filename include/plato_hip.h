@@ -142,12 +142,12 @@ int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items);
  * weighted or not) -- the same aggregate_message pass (message_passing.hpp:88-144)
  * under two decompositions:
  *   PE_PULL_MERGE_PATH  CSC gather of x[u] per edge, compensated FP32 sums (default);
- *   PE_PULL_BINNED      partition-centric: contributions are streamed into
- *                       destination bins, then each bin is summed in LDS as exact
- *                       128-bit fixed-point integers (order-independent, bit-
- *                       reproducible).  Builds a second, bin-ordered copy of the
- *                       edges (4 B per edge, +4 B with weights) and 4 B per edge of
- *                       scratch on first use.
+ *   PE_PULL_BINNED      partition-centric: one contribution per (source, bin) pair
+ *                       is streamed into destination bins, then each bin is summed
+ *                       in LDS as fixed-point integers (terms truncated at 2^-64,
+ *                       order-independent, bit-reproducible).  Builds a second,
+ *                       bin-ordered copy of the edges (2.5 B per edge, +4 B with
+ *                       weights, + 6 B per (source, bin) pair) on first use.
  * FP64 iterations and raw sums (pe_pull_sum) always use the merge-path gather.
  * opts may be NULL (defaults); its fields tune the binned layout. */
 #define PE_PULL_MERGE_PATH 0

@@ -15,15 +15,15 @@
 // blocking", IPDPS 2017; Lakhotia, Kannan, Prasanna, "Accelerating PageRank using
 // partition-centric processing", USENIX ATC 2018.)
 //
-// Sums are exact.  Each contribution (non-negative, finite, below 2^32) becomes a
-// 128-bit fixed-point integer with 96 fractional bits -- exact for FP32 values down
-// to 2^-73 and for the FP64 products w * x of weighted graphs down to 2^-43 relative
-// to their leading bit; lower bits are truncated, deterministically.  Lanes of a wave
-// pre-add runs of equal destinations, then 64-bit LDS atomics (with the carry taken
-// into the high word) accumulate.  Integer addition is associative, so every sum is
-// independent of order, launch shape and layout parameters: results are bit-
-// reproducible.  The bound on the values holds for every rank iteration here: the
-// sum into one vertex is at most the total rank mass, at most V < 2^32.
+// Sums are exact and order-independent.  Each contribution (non-negative, finite,
+// below 2^32) becomes a fixed-point integer with 96 fractional bits, truncated to a
+// resolution of 2^-64 (the FP64 products w * x of weighted graphs are formed exactly
+// first); these integers are added with 64-bit LDS atomics into two words per
+// destination that never carry into each other.  Integer addition is associative, so
+// every sum is independent of order, launch shape and layout parameters: results are
+// bit-reproducible.  The truncation moves a sum by less than 2^-64 per term (under
+// 2^-33 over 2^31 edges).  The bound on the values holds for every rank iteration
+// here: the sum into one vertex is at most the total rank mass, at most V < 2^32.
 //
 // Layout (bin order): every CSC entry (v <- u) gets the key
 //   bin = v >> dst_bits | chunk = u >> src_bits | u & smask | v & dmask
@@ -39,6 +39,7 @@
 #include <numeric>
 #include <vector>
 
+#include "pb_fixed.hpp"
 #include "pe_internal.hpp"
 #include "pull.hpp"
 
@@ -47,7 +48,6 @@ namespace pe {
 static constexpr int PB_THREADS = 1024;
 static constexpr int PB_MAX_SRC_BITS = 15;  // 128 KiB of FP32 sources in LDS
 static constexpr int PB_MAX_DST_BITS = 13;  // 128 KiB of 128-bit accumulators in LDS
-static constexpr int PB_FRAC = 96;          // fractional bits of the fixed-point sums
 static constexpr uint64_t PB_UNIT1 = 1ull << 20;  // max entries per scatter unit
 static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the run scan
 #ifndef PB_SCATTER_RECORDS
@@ -57,56 +57,10 @@ static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the 
 #define PB_GATHER_UNROLL 4    // 8-entry groups per lane at a time in the gather
 #endif
 
-struct u128 {
-  uint64_t lo, hi;
-};
-
-__device__ __forceinline__ u128 add128(u128 a, u128 b) {
-  u128 r;
-  r.lo = a.lo + b.lo;
-  r.hi = a.hi + b.hi + (r.lo < a.lo ? 1ull : 0ull);
-  return r;
-}
-
-// m * 2^k as a 128-bit integer, bits below 2^0 dropped (branch-free); exact for
-// k + bits(m) <= 128, which values below 2^32 satisfy
-__device__ __forceinline__ u128 shl128(uint64_t m, int k) {
-  const uint64_t l_left = m << (k & 63), l_right = m >> ((-k) & 63);
-  const uint64_t h_left = m << ((k - 64) & 63), h_right = m >> ((64 - k) & 63);
-  u128 r;
-  r.lo = (k >= 0) ? ((k < 64) ? l_left : 0ull) : ((k > -64) ? l_right : 0ull);
-  r.hi = (k >= 64) ? ((k < 128) ? h_left : 0ull) : ((k > 0) ? h_right : 0ull);
-  return r;
-}
-
-// FP32 value -> fixed point (zero, subnormals and negative values give 0)
-__device__ __forceinline__ u128 fixed_f32(float v) {
-  const uint32_t b = __float_as_uint(v);
-  const int e = (int)((b >> 23) & 0xFFu);
-  if (e == 0 || (b >> 31)) return u128{0ull, 0ull};
-  const uint64_t m = (uint64_t)((b & 0x7FFFFFu) | 0x800000u);
-  return shl128(m, e - 150 + PB_FRAC);
-}
-
-// FP64 value -> fixed point
-__device__ __forceinline__ u128 fixed_f64(double v) {
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const int e = (int)((b >> 52) & 0x7FFull);
-  if (e == 0 || (b >> 63)) return u128{0ull, 0ull};
-  const uint64_t m = (b & ((1ull << 52) - 1ull)) | (1ull << 52);
-  return shl128(m, e - 1075 + PB_FRAC);
-}
-
-__device__ __forceinline__ double from_fixed(uint64_t lo, uint64_t hi) {
-  return ldexp((double)hi, 64 - PB_FRAC) + ldexp((double)lo, -PB_FRAC);
-}
-
-// 128-bit add into two 64-bit words (LDS or global): the low word's carry goes high
-__device__ __forceinline__ void atomic_add128(unsigned long long* lo, unsigned long long* hi, u128 v) {
-  if (v.lo) {
-    const unsigned long long old = atomicAdd(lo, (unsigned long long)v.lo);
-    if (old + v.lo < old) v.hi += 1ull;
-  }
+// Adds one converted term into a destination's two accumulator words.
+__device__ __forceinline__ void acc_add(unsigned long long* lo, unsigned long long* hi, u128 v) {
+  const uint64_t d1 = v.lo >> 32;
+  if (d1) atomicAdd(lo, (unsigned long long)d1);
   if (v.hi) atomicAdd(hi, (unsigned long long)v.hi);
 }
 
@@ -179,7 +133,7 @@ __device__ __forceinline__ u128 pb_term(const uint16_t* dst_local, const uint32_
   uint32_t cnt = 0;
   for (uint64_t j = i & ~7ull; j <= i; ++j) cnt += (uint32_t)(dst_local[j] >> 15);
   const float x = values[gbase[i >> 3] + cnt - 1];
-  return WEIGHTED ? fixed_f64((double)w[i] * (double)x) : fixed_f32(x);
+  return WEIGHTED ? fixed_f64((double)w[i] * (double)x) : fixed_f32_acc(x);
 }
 
 template <bool WEIGHTED, int GU>
@@ -215,21 +169,20 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
     __syncthreads();
     // Entries in aligned groups of 8 per lane (one 16-byte load of destinations, the
     // group's update count, then the 8 values by update index: entries of one update
-    // read the same address); a lane adds runs of equal destinations in registers and
-    // issues one LDS atomic per destination change.  Unaligned head and tail entries go
-    // one by one.
+    // read the same address), one LDS atomic per entry.  Unaligned head and tail
+    // entries go one by one.
     const uint64_t a0 = (u.e0 + 7) & ~7ull, a1 = u.e1 & ~7ull;
     const bool grouped = a0 < a1;
     {
       const uint64_t h1 = grouped ? a0 : u.e1;
       for (uint64_t i = u.e0 + tid; i < h1; i += PB_THREADS) {
         const uint32_t d = dst_local[i] & 0x7FFFu;
-        atomic_add128(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
+        acc_add(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
       }
       if (grouped)
         for (uint64_t i = a1 + tid; i < u.e1; i += PB_THREADS) {
           const uint32_t d = dst_local[i] & 0x7FFFu;
-          atomic_add128(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
+          acc_add(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
         }
     }
     if (grouped) {
@@ -276,20 +229,11 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
             wv[0] = wa[q].x; wv[1] = wa[q].y; wv[2] = wa[q].z; wv[3] = wa[q].w;
             wv[4] = wb[q].x; wv[5] = wb[q].y; wv[6] = wb[q].z; wv[7] = wb[q].w;
           }
-          uint32_t cd = d[0];
-          u128 acc = WEIGHTED ? fixed_f64((double)wv[0] * (double)xv[q][0]) : fixed_f32(xv[q][0]);
 #pragma unroll
-          for (int k = 1; k < 8; ++k) {
-            const u128 f = WEIGHTED ? fixed_f64((double)wv[k] * (double)xv[q][k]) : fixed_f32(xv[q][k]);
-            if (d[k] == cd) {
-              acc = add128(acc, f);
-            } else {
-              atomic_add128(&slo[cd], &shi[cd], acc);
-              cd = d[k];
-              acc = f;
-            }
+          for (int k = 0; k < 8; ++k) {
+            const u128 f = WEIGHTED ? fixed_f64((double)wv[k] * (double)xv[q][k]) : fixed_f32_acc(xv[q][k]);
+            acc_add(&slo[d[k]], &shi[d[k]], f);
           }
-          atomic_add128(&slo[cd], &shi[cd], acc);
         }
       }
     }
@@ -305,8 +249,8 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
       unsigned long long* alo = acc + ((uint64_t)u.slot << (dst_bits + 1));
       unsigned long long* ahi = alo + pd;
       for (uint32_t i = tid; i < pd; i += PB_THREADS) {
-        const u128 s{slo[i], shi[i]};
-        if (s.lo | s.hi) atomic_add128(&alo[i], &ahi[i], s);
+        if (slo[i]) atomicAdd(&alo[i], slo[i]);
+        if (shi[i]) atomicAdd(&ahi[i], shi[i]);
       }
     }
     __syncthreads();

@@ -1,6 +1,7 @@
 """GPU: the binned (partition-centric) pull layout of the FP32 rank iterations.
 
-The binned gather sums every column exactly (128-bit fixed point), so
+The binned gather sums every column exactly in fixed point (each term truncated at
+2^-64, then integer sums, independent of order), so
   * against the FP64 oracle it meets the FP32 tolerances of the merge-path
     gather: normalised L1 <= 2e-7, relative L-infinity <= 2e-6 after 20 PageRank
     iterations (weighted: L1 <= 5e-7; PPR: absolute L1 <= 1e-6);

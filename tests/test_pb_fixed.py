@@ -1,0 +1,18 @@
+"""CPU: the binned pull's fixed-point terms (plato_amd/csrc/pb_fixed.hpp, the same
+code the kernels compile) against a 128-bit integer reference -- exhaustive over every
+non-negative FP32 value below 2^32, FP64 weighted products, and two-word sums."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_fixed_point_terms_exhaustive(tmp_path):
+    exe = tmp_path / "test_pb_fixed"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "plato_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "cpp", "test_pb_fixed.cc"), "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "fp32 exhaustive: 0 mismatches" in r.stdout
