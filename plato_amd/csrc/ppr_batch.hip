@@ -303,17 +303,29 @@ __global__ void k_batch_extract(const float* X0, const float* X1, const batch_ct
 
 // Top-k support: per-slot histogram of the top 11 bits of the (non-negative)
 // float pattern, then candidates at or above the threshold bin.
-__global__ void k_batch_hist(const float* X0, const float* X1, const batch_ctl* __restrict__ ctl, uint32_t V,
-                             const uint32_t* __restrict__ slots, uint32_t n, unsigned int* __restrict__ hist) {
+// Histogram of the 2048 high-bit buckets of the positive scores of each slot: every
+// block counts its vertices in LDS, then adds its non-zero buckets to the global one.
+__global__ __launch_bounds__(256) void k_batch_hist(const float* X0, const float* X1,
+                                                    const batch_ctl* __restrict__ ctl, uint32_t V,
+                                                    const uint32_t* __restrict__ slots, uint32_t n,
+                                                    unsigned int* __restrict__ hist) {
+  __shared__ unsigned int sh[2048];
   const float* X = ctl->cur ? X1 : X0;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x)
-    for (uint32_t i = 0; i < n; ++i) {
-      const float f = X[v * BK + slots[i]];
-      if (f > 0.f) atomicAdd(&hist[i * 2048u + (__float_as_uint(f) >> 20)], 1u);
+  for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t j = threadIdx.x; j < 2048; j += blockDim.x) sh[j] = 0u;
+    __syncthreads();
+    const uint32_t slot = slots[i];
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+      const float f = X[v * BK + slot];
+      if (f > 0.f) atomicAdd(&sh[__float_as_uint(f) >> 20], 1u);
     }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < 2048; j += blockDim.x)
+      if (sh[j]) atomicAdd(&hist[i * 2048u + j], sh[j]);
+    __syncthreads();
+  }
 }
-
 __global__ void k_batch_candidates(const float* X0, const float* X1, const batch_ctl* __restrict__ ctl,
                                    uint32_t V, const uint32_t* __restrict__ slots, uint32_t n,
                                    const uint32_t* __restrict__ thr_bin, uint32_t cap,
@@ -647,7 +659,7 @@ int pe_ppr_batch_topk(pe_ppr_batch_t* b, const uint32_t* slots, uint32_t n, uint
     auto ds = device_slots(b, slots, n);
     dbuf<unsigned int> hist(ctx, (uint64_t)n * 2048);
     PE_HIP(hipMemsetAsync(hist.p, 0, (uint64_t)n * 2048 * 4, s));
-    k_batch_hist<<<grid_for(V, 256, 8192), 256, 0, s>>>(b->X0.p, b->X1.p, b->ctl.p, V, ds.p, n, hist.p);
+    k_batch_hist<<<persistent_grid(ctx, grid_for(V, 256), 4), 256, 0, s>>>(b->X0.p, b->X1.p, b->ctl.p, V, ds.p, n, hist.p);
     std::vector<unsigned int> h((uint64_t)n * 2048);
     PE_HIP(hipMemcpyAsync(h.data(), hist.p, h.size() * 4, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
