@@ -1,6 +1,7 @@
 // PageRank driver with the flags of example/pagerank_simple.cc:36-51
 // (--input --output --is_directed --iterations --damping --eps), running on the
 // device engine.  Output: `v,rank` lines in <output>/0000_0.csv when --output is given.
+// --pull_layout binned selects the partition-centric pull (FP32 runs).
 #include <chrono>
 #include <cstdio>
 #include <string>
@@ -23,6 +24,7 @@ int main(int argc, char** argv) {
   opts.damping_ = std::atof(flag(argc, argv, "damping", "0.85"));
   opts.eps_ = std::atof(flag(argc, argv, "eps", "0.001"));
   opts.precision_ = std::atoi(flag(argc, argv, "precision", "32"));
+  if (std::string(flag(argc, argv, "pull_layout", "merge_path")) == "binned") pdcsc->set_pull_layout(PE_PULL_BINNED);
   auto ranks = plato::algo::pagerank(*pdcsc, graph_info, opts);
   auto t2 = std::chrono::steady_clock::now();
   std::printf("vertices %u edges %llu max_v_i %u load %.3fs pagerank %.3fs\n", graph_info.vertices_,

@@ -62,6 +62,12 @@ class graph_facade {
   }
   std::shared_ptr<partition_t> partitioner() const { return part_; }
   pe_graph_t* handle() const { return g_->h; }
+  // Pull layout of the FP32 rank iterations on this graph (PE_PULL_MERGE_PATH or
+  // PE_PULL_BINNED; include/plato_hip.h).  The binned layout is built on the first call.
+  void set_pull_layout(int layout, const pe_binned_opts_t* opts = nullptr) {
+    check_status(pe_graph_set_pull_layout(g_->h, layout, opts), "pe_graph_set_pull_layout");
+    check_status(pe_graph_get_info(g_->h, &g_->info), "pe_graph_get_info");
+  }
   const pe_graph_info_t& info() const { return g_->info; }
   std::shared_ptr<device_graph> shared() const { return g_; }
 

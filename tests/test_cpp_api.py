@@ -56,6 +56,21 @@ def test_pagerank_driver_output_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_pagerank_driver_binned_layout(tmp_path):
+    """--pull_layout binned (FP32): normalised L1 <= 2e-7 against the FP64 oracle."""
+    s, d = oracle.read_edge_csv(os.path.join(GOLDEN, "v100_e4300_da_c3.csv"))
+    r = subprocess.run([os.path.join(BIN, "pagerank_simple"), "--input",
+                        os.path.join(GOLDEN, "v100_e4300_da_c3.csv"), "--is_directed=true",
+                        "--iterations", "20", "--eps", "0", "--precision", "32",
+                        "--pull_layout", "binned", "--output", str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(tmp_path / "0000_0.csv", delimiter=",")
+    ref, _ = oracle.pagerank(oracle.CSC(100, s, d), 20, 0.85, 0.0)
+    assert np.abs(got[:, 1] - ref).sum() / ref.sum() <= 2e-7
+
+
+@pytest.mark.gpu
 def test_bfs_driver_levels(tmp_path):
     path = os.path.join(GOLDEN, "v100_e2150_ua_c3.csv")
     s, d = oracle.read_edge_csv(path)
