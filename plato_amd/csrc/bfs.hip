@@ -22,6 +22,8 @@
 
 #include "pe_internal.hpp"
 
+#include <rocprim/device/device_scan.hpp>
+
 namespace pe {
 
 static constexpr uint32_t UNVIS = 0xFFFFFFFFu;
@@ -75,6 +77,75 @@ __device__ __forceinline__ unsigned long long wave_append(uint64_t mask,
   if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
   base = __shfl(base, leader, 64);
   return base + wave_prefix(mask);
+}
+
+// Claims the unvisited target of out-edge i at level L and appends it to the next
+// queue (wave-aggregated; the lanes of the wave call it together).
+__device__ __forceinline__ void push_edge(uint64_t i, bool live, const uint32_t* __restrict__ idx,
+                                          uint32_t* __restrict__ level, uint32_t L,
+                                          uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
+                                          bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od,
+                                          unsigned long long& active) {
+  bool claimed = false;
+  uint32_t w = 0;
+  if (live) {
+    w = idx[i];
+    if (level[w] == UNVIS) claimed = (atomicMin(&level[w], L) == UNVIS);
+  }
+  const uint64_t m = __ballot(claimed);
+  if (m) {
+    const unsigned long long pos = wave_append(m, &cnt->next_count);
+    if (claimed) {
+      next_queue[pos] = w;
+      atomicOr((unsigned long long*)&next_bits[w >> 6], 1ull << (w & 63));
+      active += od[w];
+    }
+  }
+}
+
+// Out-degree of every frontier vertex (deg[n] = 0), scanned into edge offsets.
+__global__ void k_frontier_degrees(const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
+                                   uint64_t* __restrict__ deg) {
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q <= n; q += (uint64_t)gridDim.x * blockDim.x)
+    deg[q] = q < n ? ptr[queue[q] + 1] - ptr[queue[q]] : 0ull;
+}
+
+// Load-balanced top-down step for frontiers of high mean degree: the frontier's edges
+// (offsets pref[0..n], pref[n] = total) are cut into chunks of 256 edges, one chunk
+// per wave at a time, so a hub's edges spread over many waves instead of one.
+__global__ __launch_bounds__(256) void k_bfs_push_lb(
+    const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ pref,
+    const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
+    uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits, bfs_counters* __restrict__ cnt,
+    const uint32_t* __restrict__ od) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t T = pref[n];
+  unsigned long long active = 0;
+  for (uint64_t c = wave_id; c * 256 < T; c += nwaves) {
+    const uint64_t e0 = c * 256;
+    uint64_t lo = 0, hi = n - 1;  // last q with pref[q] <= e0
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) >> 1;
+      if (pref[mid] <= e0) lo = mid;
+      else hi = mid - 1;
+    }
+    uint64_t q = lo;
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t j = e0 + (uint64_t)k * 64 + lane;
+      const bool live = j < T;
+      uint64_t i = 0;
+      if (live) {
+        while (pref[q + 1] <= j) ++q;
+        i = ptr[queue[q]] + (j - pref[q]);
+      }
+      push_edge(i, live, idx, level, L, next_queue, next_bits, cnt, od, active);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
+  if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
 }
 
 __global__ __launch_bounds__(256) void k_bfs_push(
@@ -241,6 +312,9 @@ __global__ void k_reached_edges(const uint32_t* __restrict__ level, const uint32
 }
 
 // Device state of one BFS engine, reused across roots.
+// frontiers whose active edges exceed this many per vertex use the load-balanced push
+static constexpr uint64_t LB_MEAN_DEGREE = 32;
+
 struct bfs_state {
   pe_graph_s* g;
   uint64_t nwords;
@@ -248,6 +322,9 @@ struct bfs_state {
   dbuf<uint64_t> b0, b1;
   dbuf<bfs_counters> cnt;
   dbuf<unsigned long long> red;
+  dbuf<uint64_t> deg, pref;  // frontier degrees and their scan (load-balanced push)
+  dbuf<uint8_t> scan_tmp;
+  size_t scan_bytes = 0;
   bfs_counters* hcnt;  // pinned host mirror
 
   explicit bfs_state(pe_graph_s* graph) : g(graph) {
@@ -262,6 +339,11 @@ struct bfs_state {
     b1.alloc(ctx, nwords);
     cnt.alloc(ctx, 1);
     red.alloc(ctx, 2);
+    deg.alloc(ctx, (uint64_t)V + 1);
+    pref.alloc(ctx, (uint64_t)V + 1);
+    PE_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, deg.p, pref.p, (uint64_t)0, (size_t)V + 1,
+                                   rocprim::plus<uint64_t>(), ctx->stream));
+    scan_tmp.alloc(ctx, std::max<size_t>(1, scan_bytes));
     hcnt = static_cast<bfs_counters*>(ctx->host_pinned);
   }
 
@@ -299,9 +381,20 @@ struct bfs_state {
         PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
         k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(bn, nwords);
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
-        const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
-        k_bfs_push<<<grid, 256, 0, s>>>(qc, n, nullptr, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
-                                        g->od.p);
+        if (A > LB_MEAN_DEGREE * n) {
+          // high mean degree (hub frontiers): spread the edges evenly over the waves
+          k_frontier_degrees<<<grid_for(n + 1, 256, 65536), 256, 0, s>>>(qc, n, g->csr_ptr(), deg.p);
+          size_t tb = scan_bytes;
+          PE_HIP(rocprim::exclusive_scan(scan_tmp.p, tb, deg.p, pref.p, (uint64_t)0, (size_t)n + 1,
+                                         rocprim::plus<uint64_t>(), s));
+          const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, std::max<uint64_t>(1, (A + 1023) / 1024));
+          k_bfs_push_lb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
+                                             g->od.p);
+        } else {
+          const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
+          k_bfs_push<<<grid, 256, 0, s>>>(qc, n, nullptr, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
+                                          g->od.p);
+        }
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
         k_bfs_pull<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,

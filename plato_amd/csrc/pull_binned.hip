@@ -46,7 +46,9 @@
 namespace pe {
 
 static constexpr int PB_THREADS = 1024;
-static constexpr int PB_MAX_SRC_BITS = 15;  // 128 KiB of FP32 sources in LDS
+#ifndef PB_MAX_SRC_BITS
+#define PB_MAX_SRC_BITS 15  // 128 KiB of FP32 sources in LDS
+#endif
 static constexpr int PB_MAX_DST_BITS = 13;  // 128 KiB of 128-bit accumulators in LDS
 static constexpr uint64_t PB_UNIT1 = 1ull << 20;  // max entries per scatter unit
 static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the run scan
@@ -415,7 +417,7 @@ static void upload(pe_ctx_s* ctx, dbuf<T>& d, const std::vector<T>& h) {
 void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   pe_ctx_s* ctx = g->ctx;
   hipStream_t s = ctx->stream;
-  pe_binned_opts_t o{15u, 13u, 256u, 20u};
+  pe_binned_opts_t o{(uint32_t)PB_MAX_SRC_BITS, 13u, 256u, 20u};
   if (opts) {
     if (opts->src_bits) o.src_bits = opts->src_bits;
     if (opts->dst_bits) o.dst_bits = opts->dst_bits;
