@@ -185,11 +185,21 @@ __global__ __launch_bounds__(256) void k_bfs_push(
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
 }
 
-__global__ __launch_bounds__(256) void k_bfs_pull(
+// Bottom-up step.  The first PULL_HOT_WORDS words of the frontier bitmap (ids below
+// 64 * PULL_HOT_WORDS) are staged in LDS once per block: with degree relabeling those
+// are the hubs, which most in-edge lists start with, so most membership tests hit LDS.
+static constexpr uint32_t PULL_THREADS = 1024;
+static constexpr uint64_t PULL_HOT_WORDS = 8192;  // 64 KiB: ids below 2^19
+
+__global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
     uint64_t nwords, uint32_t* __restrict__ level, const uint64_t* __restrict__ cur_bits, uint32_t L,
     uint64_t* __restrict__ next_bits, uint32_t* __restrict__ next_queue,
     uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+  __shared__ uint64_t hot[PULL_HOT_WORDS];
+  const uint64_t nh = nwords < PULL_HOT_WORDS ? nwords : PULL_HOT_WORDS;
+  for (uint64_t i = threadIdx.x; i < nh; i += blockDim.x) hot[i] = cur_bits[i];
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   unsigned long long active = 0;
   for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
@@ -201,7 +211,8 @@ __global__ __launch_bounds__(256) void k_bfs_pull(
       const uint64_t lim = (e - b > PULL_CAP) ? b + PULL_CAP : e;
       for (uint64_t i = b; i < lim; ++i) {
         const uint32_t u = row_idx[i];
-        if ((cur_bits[u >> 6] >> (u & 63)) & 1ull) { claimed = true; break; }
+        const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
+        if ((wd >> (u & 63)) & 1ull) { claimed = true; break; }
       }
       is_heavy = !claimed && e > lim;
     }
@@ -353,7 +364,7 @@ struct bfs_state {
     hipStream_t s = ctx->stream;
     const uint64_t Ep = g->directed ? g->edges : g->edges * 2;  // dualmode.hpp:261-262
     const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
-    const unsigned pull_grid = grid_for(nwords * 64, 256, 1u << 20);
+    const unsigned pull_grid = persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), 2);
     const unsigned heavy_grid = persistent_grid(ctx, 1u << 20, 4);
     PE_HIP(hipEventRecord(ctx->t0, s));
     k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, g->V, root);
@@ -397,7 +408,7 @@ struct bfs_state {
         }
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
-        k_bfs_pull<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,
+        k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,
                                              qn, heavy.p, cnt.p, g->od.p);
         k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, qn,
                                                     heavy.p, cnt.p, g->od.p);
@@ -498,7 +509,7 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
           st.cnt.p, g->od.p);
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
-      k_bfs_pull<<<grid_for(nwords * 64, 256, 1u << 20), 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords,
+      k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), 2), PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords,
                                                                        st.level.p, current->p, L, next->p, st.q1.p,
                                                                        st.heavy.p, st.cnt.p, g->od.p);
       k_bfs_pull_heavy<<<persistent_grid(ctx, 1u << 20, 4), 256, 0, s>>>(
