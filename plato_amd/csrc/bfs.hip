@@ -188,8 +188,13 @@ __global__ __launch_bounds__(256) void k_bfs_push(
 // Bottom-up step.  The first PULL_HOT_WORDS words of the frontier bitmap (ids below
 // 64 * PULL_HOT_WORDS) are staged in LDS once per block: with degree relabeling those
 // are the hubs, which most in-edge lists start with, so most membership tests hit LDS.
+#ifndef PULL_BLOCKS_PER_CU
+#define PULL_BLOCKS_PER_CU 1
+#endif
 static constexpr uint32_t PULL_THREADS = 1024;
-static constexpr uint64_t PULL_HOT_WORDS = 8192;  // 64 KiB: ids below 2^19
+#ifndef PULL_HOT_WORDS
+#define PULL_HOT_WORDS 16384  // 128 KiB: ids below 2^20
+#endif
 
 __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     uint64_t* __restrict__ next_bits, uint32_t* __restrict__ next_queue,
     uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   __shared__ uint64_t hot[PULL_HOT_WORDS];
-  const uint64_t nh = nwords < PULL_HOT_WORDS ? nwords : PULL_HOT_WORDS;
+  const uint64_t nh = nwords < (uint64_t)PULL_HOT_WORDS ? nwords : (uint64_t)PULL_HOT_WORDS;
   for (uint64_t i = threadIdx.x; i < nh; i += blockDim.x) hot[i] = cur_bits[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -364,7 +369,7 @@ struct bfs_state {
     hipStream_t s = ctx->stream;
     const uint64_t Ep = g->directed ? g->edges : g->edges * 2;  // dualmode.hpp:261-262
     const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
-    const unsigned pull_grid = persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), 2);
+    const unsigned pull_grid = persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU);
     const unsigned heavy_grid = persistent_grid(ctx, 1u << 20, 4);
     PE_HIP(hipEventRecord(ctx->t0, s));
     k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, g->V, root);
@@ -509,7 +514,7 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
           st.cnt.p, g->od.p);
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
-      k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), 2), PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords,
+      k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords,
                                                                        st.level.p, current->p, L, next->p, st.q1.p,
                                                                        st.heavy.p, st.cnt.p, g->od.p);
       k_bfs_pull_heavy<<<persistent_grid(ctx, 1u << 20, 4), 256, 0, s>>>(
