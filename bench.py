@@ -141,6 +141,49 @@ def run_bfs(ctx, scale, roots, seed=11):
     return out
 
 
+def csv_text(src, dst):
+    """`src,dst\n` lines with 10-digit zero-padded ids (22 bytes per edge), built with numpy."""
+    E = src.size
+    out = np.empty((E, 22), np.uint8)
+    for col, arr in ((0, src), (11, dst)):
+        v = arr.astype(np.uint64)
+        for k in range(9, -1, -1):
+            out[:, col + k] = (v % 10).astype(np.uint8) + 48
+            v //= 10
+    out[:, 10] = ord(",")
+    out[:, 21] = ord("\n")
+    return out.tobytes()
+
+
+def run_ingest(ctx, scale, seed=21):
+    """Edge-list text -> device graph (pe_graph_create_csv: H2D copy, parse, CSC/CSR build)
+    against the reference parser's rules restated on the CPU (oracle, one thread, sample)."""
+    from plato_amd import Graph
+    src, dst = ctx.rmat_edges(scale, seed, 0, 16 << scale)
+    text = csv_text(src, dst)
+    del src, dst
+    Graph.from_csv_bytes(ctx, text[: 22 * 4096], build_csr=True).close()  # warm-up
+    best = 1e30
+    for _ in range(2):
+        ctx.sync()
+        t0 = time.perf_counter()
+        gg = Graph.from_csv_bytes(ctx, text, build_csr=True)
+        ctx.sync()
+        best = min(best, time.perf_counter() - t0)
+        E = gg.info.edges
+        gg.close()
+    import oracle  # CPU baseline: the reference parser's rules on a sample of the same text
+    sample = text[: 22 * (1 << 21)]
+    t0 = time.perf_counter()
+    ns = oracle.parse_csv(sample)[0].size
+    t_cpu = time.perf_counter() - t0
+    return {"scale": scale, "edges": int(E), "text_bytes": len(text), "seconds": round(best, 4),
+            "text_gb_per_s": round(len(text) / best / 1e9, 2), "medges_per_s": round(E / best / 1e6, 1),
+            "includes": "host-to-device copy of the text, parse, CSC + CSR build",
+            "cpu_reference_parser": {"medges_per_s": round(ns / t_cpu / 1e6, 2), "cores": 1,
+                                     "sample": f"first {ns} lines of the same text"}}
+
+
 def ppr_seeds(od, rng, n=64):
     """Half from the top 1% by out-degree, half with 1 <= od <= median (uneven convergence)."""
     cand = np.flatnonzero(od > 0)
@@ -268,6 +311,8 @@ def main():
     ap.add_argument("--serve-qps", default="4,16,64")
     ap.add_argument("--serve-queries", type=int, default=128)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--ingest-scale", type=int, default=22)
+    ap.add_argument("--no-ingest", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported",
@@ -366,6 +411,9 @@ def main():
     if not args.no_ppr:
         log(f"[bench] batched PPR scale {args.ppr_scale}, k = 64")
         out["ppr_batched"] = run_batched_ppr(ctx, args.ppr_scale, single_runs=not args.ppr_block_only)
+    if not args.no_ingest:
+        log(f"[bench] CSV ingestion scale {args.ingest_scale}")
+        out["ingest"] = run_ingest(ctx, args.ingest_scale)
     if not args.no_serve:
         log(f"[bench] serving scale {args.serve_scale}")
         out["serving"] = run_serving(ctx, args.serve_scale,

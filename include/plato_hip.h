@@ -130,6 +130,26 @@ int pe_rmat_edges(pe_ctx_t* ctx, uint32_t scale, uint64_t seed, double a, double
                   uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst, float* w);
 int pe_graph_destroy(pe_graph_t* g);
 int pe_graph_get_info(const pe_graph_t* g, pe_graph_info_t* info);
+/* Build from CSV text in host memory, parsed on the device with the reference's
+ * csv_parser rules (parsers.hpp:225-293): one `src,dst[,w]` edge per '\n'-separated
+ * line, fields split on runs of ',', ids read as strtoul(, 10) reads them (leading
+ * whitespace, an optional sign, digits up to the first other character), the weight
+ * as float_decoder's strtof (parsers.hpp:72-76); empty lines and lines missing a field
+ * are skipped.  flags as pe_graph_create, plus PE_GRAPH_ENCODE_IDS: compact the ids
+ * that occur (src and dst, 64-bit) to 0..n-1 in ascending order, replacing
+ * vid_encoder_t::encode (vid_encoder.hpp:41-102, which numbers them in hash-table
+ * order); pe_graph_vertex_ids returns the original id of each.  Without it, V = the
+ * largest id + 1 and an id above 2^32 - 1 fails with PE_ERR_RANGE (the reference's
+ * CHECK), naming the line.  Weight literals are decimal with at most 15 significant
+ * digits and a normal float value; they round exactly as strtof does (literals
+ * further than 10^22 from the integers are rejected when their value lies within a
+ * few double ulps of a float rounding midpoint).  Other literals (inf, nan, hex,
+ * subnormal or overflowing values) fail with PE_ERR_RANGE, naming the line. */
+#define PE_GRAPH_ENCODE_IDS 16u
+int pe_graph_create_csv(pe_ctx_t* ctx, const char* text, uint64_t nbytes, uint32_t flags, pe_graph_t** out);
+/* original_ids[V]: the id each dense input id stands for (identity unless encoded). */
+int pe_graph_vertex_ids(pe_graph_t* g, uint64_t* original_ids);
+
 /* Copy the CSC (col_ptr[V+1], row_idx[nnz], optional normalised weights[nnz]). */
 int pe_graph_download_csc(pe_graph_t* g, uint64_t* col_ptr, uint32_t* row_idx, float* wn);
 int pe_graph_download_csr(pe_graph_t* g, uint64_t* row_ptr, uint32_t* col_idx);

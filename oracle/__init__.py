@@ -67,6 +67,10 @@ def lib():
     L.orc_bfs.argtypes = [C.c_uint32, _u64p, _u32p, _u64p, _u32p, _u32p, C.c_uint64,
                           C.c_double, C.c_uint32, _i32p, _u8p, _u64p, _u64p, C.c_uint32,
                           C.POINTER(C.c_uint32)]
+    L.orc_parse_csv.argtypes = [C.c_char_p, C.c_uint64, C.c_int, C.c_uint64, _u32p, _u32p, C.c_void_p]
+    L.orc_parse_csv.restype = C.c_int64
+    L.orc_format_csv.argtypes = [C.c_uint64, _u32p, _u32p, C.c_void_p, C.c_uint64]
+    L.orc_format_csv.restype = C.c_int64
     L.orc_topk.argtypes = [C.c_uint32, _f64p, C.c_uint32, _u32p, _f64p]
     _lib = L
     return L
@@ -226,6 +230,31 @@ def topk(r, k):
     lib().orc_topk(r.size, r, k, ids, vals)
     kk = min(k, r.size)
     return ids[:kk], vals[:kk]
+
+
+def parse_csv(data: bytes, weighted=False):
+    """Plato's csv_parser semantics over a byte buffer (orc_parse_csv).  Returns
+    (src, dst[, w]) in file order; raises ValueError(line) where the reference CHECK-fails."""
+    n = len(data)
+    cap = data.count(b"\n") + 1
+    s = np.zeros(cap, np.uint32)
+    d = np.zeros(cap, np.uint32)
+    w = np.zeros(cap, np.float32)
+    r = lib().orc_parse_csv(data, n, int(weighted), cap, s, d, w.ctypes.data if weighted else None)
+    if r < 0:
+        raise ValueError(f"id above 2^32 - 1 on line {-r - 1}")
+    return (s[:r], d[:r], w[:r]) if weighted else (s[:r], d[:r])
+
+
+def format_csv(src, dst) -> bytes:
+    """`src,dst` lines for the given edges (test data for the parser)."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    cap = 22 * src.size + 1
+    buf = np.empty(cap, np.uint8)
+    n = lib().orc_format_csv(src.size, src, dst, buf.ctypes.data, cap)
+    assert n >= 0
+    return buf[:n].tobytes()
 
 
 def read_edge_csv(path, weighted=False):

@@ -11,6 +11,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdio.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -440,4 +441,82 @@ void orc_topk(uint32_t V, const double* r, uint32_t k, uint32_t* ids, double* va
   if (k > V) k = V;
   for (uint32_t i = 0; i < k; ++i) { ids[i] = p[i]; vals[i] = r[p[i]]; }
   free(p);
+}
+
+/* ---- edge-list ingestion (parsers.hpp:225-293) ------------------------------ */
+int64_t orc_parse_csv(const char* bytes, uint64_t n, int weighted, uint64_t cap, uint32_t* src,
+                      uint32_t* dst, float* w) {
+  uint64_t pos = 0, count = 0, line = 0;
+  char* buf = (char*)malloc(n + 1);
+  while (pos < n) {
+    uint64_t e = pos;
+    while (e < n && bytes[e] != '\n') ++e;
+    const uint64_t len = e - pos;
+    memcpy(buf, bytes + pos, len);
+    buf[len] = '\0';
+    pos = e + 1;  /* past the '\n' (or past the end) */
+    const uint64_t this_line = line++;
+    if (buf[0] == '\0') continue;
+    char* save = NULL;
+    char* tok = strtok_r(buf, ",", &save);
+    if (!tok) continue;
+    const unsigned long s = strtoul(tok, NULL, 10);
+    if (s > 0xFFFFFFFFul) { free(buf); return -(int64_t)this_line - 1; }
+    tok = strtok_r(NULL, ",", &save);
+    if (!tok) continue;
+    const unsigned long d = strtoul(tok, NULL, 10);
+    if (d > 0xFFFFFFFFul) { free(buf); return -(int64_t)this_line - 1; }
+    float wt = 0.0f;
+    if (weighted) {
+      tok = strtok_r(NULL, ",", &save);
+      if (!tok) continue;
+      wt = strtof(tok, NULL);
+    }
+    if (count < cap) {
+      src[count] = (uint32_t)s;
+      dst[count] = (uint32_t)d;
+      if (weighted && w) w[count] = wt;
+    }
+    ++count;
+  }
+  free(buf);
+  return (int64_t)count;
+}
+
+static int orc_u32_digits(uint32_t x, char* tmp) {
+  int n = 0;
+  do { tmp[n++] = (char)('0' + x % 10u); x /= 10u; } while (x);
+  return n;
+}
+
+int64_t orc_format_csv(uint64_t E, const uint32_t* src, const uint32_t* dst, char* out, uint64_t cap) {
+  const uint64_t B = 1u << 16;
+  const uint64_t nb = (E + B - 1) / B;
+  uint64_t* len = (uint64_t*)calloc(nb + 1, sizeof(uint64_t));
+  #pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < (int64_t)nb; ++b) {
+    char tmp[12];
+    uint64_t L = 0;
+    for (uint64_t i = (uint64_t)b * B; i < E && i < ((uint64_t)b + 1) * B; ++i)
+      L += (uint64_t)orc_u32_digits(src[i], tmp) + (uint64_t)orc_u32_digits(dst[i], tmp) + 2;
+    len[b + 1] = L;
+  }
+  for (uint64_t b = 0; b < nb; ++b) len[b + 1] += len[b];
+  if (len[nb] > cap) { free(len); return -1; }
+  #pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < (int64_t)nb; ++b) {
+    char tmp[12];
+    char* o = out + len[b];
+    for (uint64_t i = (uint64_t)b * B; i < E && i < ((uint64_t)b + 1) * B; ++i) {
+      int n = orc_u32_digits(src[i], tmp);
+      while (n) *o++ = tmp[--n];
+      *o++ = ',';
+      n = orc_u32_digits(dst[i], tmp);
+      while (n) *o++ = tmp[--n];
+      *o++ = '\n';
+    }
+  }
+  const int64_t total = (int64_t)len[nb];
+  free(len);
+  return total;
 }

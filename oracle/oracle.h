@@ -117,6 +117,26 @@ int orc_bfs(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
             int32_t* levels, uint8_t* modes, uint64_t* frontier, uint64_t* active_edges,
             uint32_t max_epochs, uint32_t* visited);
 
+/* ---- edge-list ingestion ---------------------------------------------------- */
+
+/* Plato's csv_parser (parsers.hpp:225-293) over a byte buffer, with the same C
+ * library calls: lines as std::istream::getline splits them on '\n' (an empty line
+ * is skipped, the text after the last '\n' is a line), fields by strtok_r on ",",
+ * src and dst by strtoul(, 10), the optional weight by float_decoder's strtof
+ * (parsers.hpp:72-76).  A line without a destination (or, when weighted, without a
+ * weight) is skipped.  Writes up to cap edges in file order and returns their number;
+ * returns -(line + 1) for the first line whose src or dst exceeds 2^32 - 1 (where the
+ * reference's CHECK aborts; src is checked before dst is looked for).
+ * Pinned: the reference's data files give their known edges; hand-written lines with
+ * whitespace, CRLF, repeated commas, signs, missing fields and trailing text. */
+int64_t orc_parse_csv(const char* bytes, uint64_t n, int weighted, uint64_t cap, uint32_t* src,
+                      uint32_t* dst, float* w);
+
+/* Writes `src,dst\n` lines (decimal) into out (capacity cap bytes); returns the
+ * number of bytes, or -1 if cap is too small.  Test data generator for the parser
+ * (OpenMP over line blocks). */
+int64_t orc_format_csv(uint64_t E, const uint32_t* src, const uint32_t* dst, char* out, uint64_t cap);
+
 /* ---- top-k ----------------------------------------------------------------- */
 
 /* Top k of r by (rank desc, id asc) (trustrank.hpp:109-112). */

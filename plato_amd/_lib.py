@@ -19,6 +19,7 @@ PE_KT_PULL_TILE, PE_KT_PULL_FIXUP, PE_KT_BFS_PUSH, PE_KT_BFS_PULL, PE_KT_PPR_BLO
 PE_KT_PULL_BIN_SCATTER, PE_KT_PULL_BIN_GATHER = 5, 6
 PE_PULL_MERGE_PATH, PE_PULL_BINNED = 0, 1
 PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED, PE_GRAPH_RELABEL_DEGREE = 1, 2, 4, 8
+PE_GRAPH_ENCODE_IDS = 16
 
 
 class PlatoHipError(RuntimeError):
@@ -103,6 +104,8 @@ SIGNATURES = {
     "pe_ctx_kernel_time": [_vp, _i32, _P(_dbl), _P(_u64)],
     "pe_ctx_num_cus": [_vp, _P(_i32)],
     "pe_graph_create": [_vp, _vp, _vp, _vp, _u64, _u32, _u32, _P(_vp)],
+    "pe_graph_create_csv": [_vp, C.c_char_p, _u64, _u32, _P(_vp)],
+    "pe_graph_vertex_ids": [_vp, _vp],
     "pe_graph_create_rmat": [_vp, _u32, _u32, _u64, _dbl, _dbl, _dbl, _u32, _P(_vp)],
     "pe_rmat_edges": [_vp, _u32, _u64, _dbl, _dbl, _dbl, _u64, _u64, _vp, _vp, _vp],
     "pe_graph_destroy": [_vp],

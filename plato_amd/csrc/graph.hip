@@ -475,8 +475,8 @@ static void build_relabel(pe_graph_s* g, dbuf<uint32_t>& ds, dbuf<uint32_t>& dd,
 }
 
 // Common builder over a device edge list.
-static void build_from_edges(pe_graph_s* g, dbuf<uint32_t>& ds, dbuf<uint32_t>& dd, dbuf<float>& dw,
-                             uint32_t flags) {
+void build_from_edges(pe_graph_s* g, dbuf<uint32_t>& ds, dbuf<uint32_t>& dd, dbuf<float>& dw,
+                      uint32_t flags) {
   pe_ctx_s* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const bool sym = flags & PE_GRAPH_SYMMETRIZE;

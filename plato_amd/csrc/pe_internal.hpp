@@ -197,6 +197,11 @@ struct pe_graph_s {
   pe::dbuf<uint32_t> perm;       // [V]
   pe::dbuf<uint32_t> rank_of;    // [V]
 
+  // ids compacted at ingestion (pe_graph_create_csv with PE_GRAPH_ENCODE_IDS):
+  // original id of every dense input id
+  bool encoded = false;
+  pe::dbuf<uint64_t> vid_decode;  // [V]
+
   // pull layout of the FP32 rank iterations
   int pull_layout = PE_PULL_MERGE_PATH;
   std::unique_ptr<pe::pb_layout> pb;
@@ -206,7 +211,7 @@ struct pe_graph_s {
   uint64_t device_bytes() const {
     return col_ptr.bytes() + row_idx.bytes() + wn.bytes() + od.bytes() + row_ptr.bytes() +
            col_idx.bytes() + perm.bytes() + rank_of.bytes() + tile_row.bytes() + tile_edge.bytes() + fix_row.bytes() +
-           fix_t0.bytes() + fix_t1.bytes() + (pb ? pb->bytes() : 0);
+           fix_t0.bytes() + fix_t1.bytes() + vid_decode.bytes() + (pb ? pb->bytes() : 0);
   }
 };
 
@@ -215,6 +220,9 @@ namespace pe {
 void graph_build_tiles(pe_graph_s* g, uint32_t tile_items);
 // stable radix sort of packed keys over bits [0, end_bit), optional float payload
 void sort_keys(pe_ctx_s* ctx, dbuf<uint64_t>& keys, dbuf<float>* vals, uint64_t n, int end_bit);
+// builds the CSC (and CSR, weights, relabeling per flags) from device edge arrays
+// (g->V and g->edges set by the caller)
+void build_from_edges(pe_graph_s* g, dbuf<uint32_t>& ds, dbuf<uint32_t>& dd, dbuf<float>& dw, uint32_t flags);
 // pull_binned.hip: builds g->pb (no-op if built with the same options)
 void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts);
 // relabeled graphs: input ids -> internal ids (host arrays), and V-long device

@@ -134,6 +134,36 @@ class Graph:
         return cls(ctx, h)
 
     @classmethod
+    def from_csv_bytes(cls, ctx, data: bytes, weighted=False, directed=True, build_csr=False,
+                       relabel=False, encode_ids=False):
+        """Edges parsed on the device from CSV text (`src,dst[,w]` lines, the reference's
+        csv_parser rules); encode_ids compacts the ids that occur to 0..n-1
+        (vertex_ids() maps them back)."""
+        flags = 0 if directed else L.PE_GRAPH_SYMMETRIZE
+        if build_csr:
+            flags |= L.PE_GRAPH_BUILD_CSR
+        if relabel:
+            flags |= L.PE_GRAPH_RELABEL_DEGREE
+        if weighted:
+            flags |= L.PE_GRAPH_WEIGHTED
+        if encode_ids:
+            flags |= L.PE_GRAPH_ENCODE_IDS
+        h = C.c_void_p()
+        check(ctx.lib.pe_graph_create_csv(ctx.h, data, len(data), flags, C.byref(h)), ctx.h)
+        return cls(ctx, h)
+
+    @classmethod
+    def from_csv(cls, ctx, path, **kw):
+        with open(path, "rb") as f:
+            return cls.from_csv_bytes(ctx, f.read(), **kw)
+
+    def vertex_ids(self):
+        """Original id of every (dense) vertex id: identity unless the ids were encoded."""
+        out = np.zeros(self.V, np.uint64)
+        check(self.lib.pe_graph_vertex_ids(self.h, out.ctypes.data), self.ctx.h)
+        return out
+
+    @classmethod
     def rmat(cls, ctx, scale, edge_factor=16, seed=1, a=0.57, b=0.19, c=0.19, directed=True,
              build_csr=False, weighted=False, relabel=False):
         flags = 0 if directed else L.PE_GRAPH_SYMMETRIZE

@@ -263,3 +263,39 @@ def test_topk_tie_break():
     ids, vals = oracle.topk(r, 4)
     order = np.lexsort((np.arange(r.size), -r))[:4]   # rank desc, id asc
     assert np.array_equal(ids, order) and np.array_equal(vals, r[order])
+
+
+# ---- edge-list ingestion (parsers.hpp:225-293) ---------------------------------------------------
+def test_parse_csv_reference_files():
+    """The reference's data files parse to their known contents (same edges as the
+    line-by-line reader the other pins use; counts from dcsc.cc / structure.cc)."""
+    for name, n in (("graph_10_9.csv", 9), ("v100_e2150_ua_c3.csv", 2150), ("v100_e4300_da_c3.csv", 4300),
+                    ("complete_binary_tree_10_9.csv", 9)):
+        s, d = oracle.parse_csv(open(golden(name), "rb").read())
+        s2, d2 = oracle.read_edge_csv(golden(name))
+        assert s.size == n and np.array_equal(s, s2) and np.array_equal(d, d2)
+    s, d, w = oracle.parse_csv(open(golden("graph_10_9.csv"), "rb").read(), weighted=True)
+    assert np.array_equal(s, np.arange(9)) and np.array_equal(d, np.arange(1, 10))
+    assert w[0] == np.float32(0.123) and w[1] == np.float32(1.779)
+
+
+def test_parse_csv_strtok_strtoul_rules():
+    """Hand-worked lines: strtok_r on ',' (repeated and leading commas collapse), strtoul
+    (leading blanks, sign, digits up to the first other byte), empty and short lines skipped,
+    CRLF, last line without a newline."""
+    s, d = oracle.parse_csv(b" 1, 2\n\n,,3,,4,\r\n5\n+6,-0\n7,8xyz\n9,10")
+    assert s.tolist() == [1, 3, 6, 7, 9] and d.tolist() == [2, 4, 0, 8, 10]
+    s, d, w = oracle.parse_csv(b"1,2,0.5\n3,4\n5,6,abc\n7,8, 1e-3\n", weighted=True)
+    assert s.tolist() == [1, 5, 7] and w.tolist() == [0.5, 0.0, np.float32(1e-3)]
+    assert oracle.parse_csv(b"")[0].size == 0
+    for bad, line in ((b"1,2\n4294967296,3\n", 1), (b"4294967296\n", 0), (b"1,-1\n", 0)):
+        with pytest.raises(ValueError, match=f"line {line}"):
+            oracle.parse_csv(bad)
+
+
+def test_format_csv_round_trip():
+    s, d = oracle.rmat_edges(12, 8, 2)
+    s2, d2 = oracle.parse_csv(oracle.format_csv(s, d))
+    assert np.array_equal(s, s2) and np.array_equal(d, d2)
+    assert oracle.format_csv(np.array([0, 4294967295], np.uint32), np.array([10, 7], np.uint32)) == \
+        b"0,10\n4294967295,7\n"
