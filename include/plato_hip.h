@@ -288,6 +288,37 @@ int pe_ppr(pe_graph_t* g, const uint32_t* seeds, uint32_t nseeds, const pe_rank_
 int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, double* out,
                     double* deltas, uint8_t* modes, pe_rank_stats_t* stats);
 
+/* TrustRank (plato/algo/trustrank/trustrank.hpp:23-259): good seeds chosen among the
+ * good vertices (select_good_seeds, :80-145), then the seed-set restart iteration of
+ * pe_ppr (:163-259).  select_method: 0 inverse PageRank (PageRank of the reversed
+ * graph: pass g_rev from pe_graph_create_reverse, or NULL for a symmetrized g),
+ * 1 PageRank, 2 random.  The ranking PageRank uses the reference's defaults (100
+ * iterations, damping 0.85, eps 0.001, FP64); the top seed_num vertices by (rank desc,
+ * id asc) are kept if good.  Random selection draws with std::minstd_rand0 seeded by
+ * random_seed (the reference seeds it from std::random_device, so its draws are not
+ * reproducible).  If V <= seed_num every good vertex is a seed. */
+typedef struct {
+  uint32_t iterations;     /* trustrank_opts_t::iteration_ (100) */
+  double damping;          /* damping_ (0.85) */
+  uint32_t seed_num;       /* seed_num_ (100) */
+  uint32_t select_method;  /* select_method_ (0) */
+  double eps;              /* eps_ (0.001) */
+  uint64_t random_seed;    /* method 2 */
+  int32_t precision;       /* of the TrustRank iteration: 32 or 64 (0 = 64, like the reference) */
+} pe_trustrank_opts_t;
+/* good_words: host bitmap_t words ((V >> 6) + 1) of the good vertices, or NULL (all).
+ * seeds_words: host words receiving the chosen seeds; nseeds: their count. */
+int pe_trustrank_select(pe_graph_t* g, pe_graph_t* g_rev, const uint64_t* good_words,
+                        const pe_trustrank_opts_t* opts, uint64_t* seeds_words, uint32_t* nseeds);
+/* The whole algorithm; out: host FP64 [V]; seeds_words optional. */
+int pe_trustrank(pe_graph_t* g, pe_graph_t* g_rev, const uint64_t* good_words, const pe_trustrank_opts_t* opts,
+                 double* out, uint64_t* seeds_words, pe_rank_stats_t* stats);
+/* A new graph with every edge of the directed graph g reversed (same input ids), for
+ * pulls over out-edges such as inverse PageRank (the reference runs pagerank over its
+ * bcsr_t out-edge structure, trustrank.hpp:101-103).  flags: PE_GRAPH_BUILD_CSR,
+ * PE_GRAPH_RELABEL_DEGREE. */
+int pe_graph_create_reverse(pe_graph_t* g, uint32_t flags, pe_graph_t** out);
+
 typedef struct {
   uint32_t epochs;
   uint32_t visited;       /* |visited| (the reference's return value) */

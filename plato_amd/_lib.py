@@ -37,6 +37,12 @@ class GraphInfo(C.Structure):
                 ("pull_layout", C.c_int32), ("binned_runs", C.c_uint32), ("binned_bytes", C.c_uint64)]
 
 
+class TrustRankOpts(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("damping", C.c_double), ("seed_num", C.c_uint32),
+                ("select_method", C.c_uint32), ("eps", C.c_double), ("random_seed", C.c_uint64),
+                ("precision", C.c_int32)]
+
+
 class BinnedOpts(C.Structure):
     _fields_ = [("src_bits", C.c_uint32), ("dst_bits", C.c_uint32), ("run_cap", C.c_uint32),
                 ("unit_log2", C.c_uint32)]
@@ -106,6 +112,9 @@ SIGNATURES = {
     "pe_graph_create": [_vp, _vp, _vp, _vp, _u64, _u32, _u32, _P(_vp)],
     "pe_graph_create_csv": [_vp, C.c_char_p, _u64, _u32, _P(_vp)],
     "pe_graph_vertex_ids": [_vp, _vp],
+    "pe_graph_create_reverse": [_vp, _u32, _P(_vp)],
+    "pe_trustrank_select": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "pe_trustrank": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pe_graph_create_rmat": [_vp, _u32, _u32, _u64, _dbl, _dbl, _dbl, _u32, _P(_vp)],
     "pe_rmat_edges": [_vp, _u32, _u64, _dbl, _dbl, _dbl, _u64, _u64, _vp, _vp, _vp],
     "pe_graph_destroy": [_vp],

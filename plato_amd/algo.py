@@ -33,6 +33,17 @@ class personalized_pagerank_opts_t:
 
 
 @dataclass
+class trustrank_opts_t:
+    """plato/algo/trustrank/trustrank.hpp:23-33 (random_seed_ replaces std::random_device)"""
+    iteration_: int = 100
+    damping_: float = 0.85
+    seed_num_: int = 100
+    select_method_: int = 0   # 0 inverse PageRank, 1 PageRank, 2 random
+    eps_: float = 0.001
+    random_seed_: int = 0
+
+
+@dataclass
 class bfs_opts_t:
     """plato/algo/bfs/bfs.hpp:37-39"""
     root_: int = 0
@@ -135,3 +146,57 @@ def breadth_first_search(graph: Graph, opts: bfs_opts_t = bfs_opts_t(),
     k = min(st.epochs, max_epochs)
     return BfsResult(levels, modes[:k], frontier[:k], active[:k], st.epochs, st.visited,
                      st.reached_edges, st.seconds)
+
+
+def _tr_opts(opts, precision):
+    o = L.TrustRankOpts()
+    o.iterations, o.damping, o.seed_num = opts.iteration_, opts.damping_, opts.seed_num_
+    o.select_method, o.eps, o.random_seed, o.precision = (opts.select_method_, opts.eps_,
+                                                          opts.random_seed_, precision)
+    return o
+
+
+def _good_words(graph, good):
+    if good is None:
+        return None
+    g = np.zeros(graph.V, bool)
+    g[np.asarray(good, np.int64)] = True
+    words = np.zeros((graph.V >> 6) + 1, np.uint64)
+    idx = np.flatnonzero(g)
+    np.bitwise_or.at(words, idx >> 6, np.left_shift(np.uint64(1), (idx & 63).astype(np.uint64)))
+    return words
+
+
+def _words_to_ids(words, V):
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:V]
+    return np.flatnonzero(bits).astype(np.uint32)
+
+
+def trustrank_seeds(graph: Graph, opts: trustrank_opts_t = trustrank_opts_t(), good=None,
+                    reverse: Graph = None) -> np.ndarray:
+    """select_good_seeds (trustrank.hpp:80-145): the chosen seed ids, ascending."""
+    o = _tr_opts(opts, 64)
+    gw = _good_words(graph, good)
+    words = np.zeros((graph.V >> 6) + 1, np.uint64)
+    n = C.c_uint32()
+    check(graph.lib.pe_trustrank_select(graph.h, reverse.h if reverse else None,
+                                        gw.ctypes.data if gw is not None else None, C.byref(o),
+                                        words.ctypes.data, C.byref(n)), graph.ctx.h)
+    return _words_to_ids(words, graph.V)
+
+
+def trustrank(graph: Graph, opts: trustrank_opts_t = trustrank_opts_t(), good=None,
+              reverse: Graph = None, precision: int = 64):
+    """plato::algo::trustrank (trustrank.hpp:163-259).  Returns (RankResult, seed ids).
+    good: ids of the good vertices (None: all); reverse: the reversed graph, needed by
+    inverse PageRank (select_method_ 0) on a directed graph."""
+    o = _tr_opts(opts, precision)
+    gw = _good_words(graph, good)
+    words = np.zeros((graph.V >> 6) + 1, np.uint64)
+    out = np.zeros(graph.V, np.float64)
+    st = L.RankStats()
+    check(graph.lib.pe_trustrank(graph.h, reverse.h if reverse else None,
+                                 gw.ctypes.data if gw is not None else None, C.byref(o),
+                                 out.ctypes.data, words.ctypes.data, C.byref(st)), graph.ctx.h)
+    res = RankResult(out, np.zeros(0), st.iterations_run, st.seconds, st.gteps)
+    return res, _words_to_ids(words, graph.V)

@@ -157,6 +157,13 @@ class Graph:
         with open(path, "rb") as f:
             return cls.from_csv_bytes(ctx, f.read(), **kw)
 
+    def reverse(self, build_csr=False, relabel=False):
+        """A new graph with every edge reversed (pe_graph_create_reverse), same input ids."""
+        flags = (L.PE_GRAPH_BUILD_CSR if build_csr else 0) | (L.PE_GRAPH_RELABEL_DEGREE if relabel else 0)
+        h = C.c_void_p()
+        check(self.lib.pe_graph_create_reverse(self.h, flags, C.byref(h)), self.ctx.h)
+        return Graph(self.ctx, h)
+
     def vertex_ids(self):
         """Original id of every (dense) vertex id: identity unless the ids were encoded."""
         out = np.zeros(self.V, np.uint64)

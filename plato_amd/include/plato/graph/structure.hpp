@@ -6,11 +6,12 @@
 // one-partition identity.  The factories keep the names and arguments of
 // create_dcsc_seqs_from_path / create_bcsr_seqd_from_path /
 // create_dualmode_seq_from_path (plato/graph/structure.hpp:473-555, 987-1086)
-// and read Plato's CSV edge format `src,dst[,w]` (parsers.hpp:225-293).
+// over Plato's CSV edge format `src,dst[,w]` (parsers.hpp:225-293), parsed on the device.
 #pragma once
 
 #include <algorithm>
 #include <fstream>
+#include <iterator>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -93,46 +94,24 @@ template <typename T>
 inline T dummy_decoder(const std::string&) { return T(); }
 
 namespace detail {
-struct edge_list {
-  std::vector<uint32_t> src, dst;
-  std::vector<float> w;
-};
-
-// `src,dst[,w]` per line, whitespace around fields tolerated (parsers.hpp:225-293)
-inline edge_list read_csv(const std::string& path, bool weighted) {
-  edge_list el;
-  std::ifstream in(path);
+inline std::string read_file(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
   if (!in) fatal("open", -1, path.c_str());
-  std::string line;
-  while (std::getline(in, line)) {
-    std::stringstream ss(line);
-    std::string a, b, c;
-    if (!std::getline(ss, a, ',') || !std::getline(ss, b, ',')) continue;
-    try {
-      const unsigned long s = std::stoul(a), d = std::stoul(b);
-      el.src.push_back((uint32_t)s);
-      el.dst.push_back((uint32_t)d);
-      if (weighted) el.w.push_back(std::getline(ss, c, ',') ? std::stof(c) : 1.0f);
-    } catch (...) {
-      continue;  // malformed line: skipped, as the reference logs and skips
-    }
-  }
-  return el;
+  return std::string((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
 }
 
-inline std::shared_ptr<device_graph> build(graph_info_t* info, const edge_list& el, bool weighted,
+// The file's text goes to the device, which parses it with csv_parser's rules
+// (parsers.hpp:225-293) and builds the CSC/CSR (pe_graph_create_csv).
+inline std::shared_ptr<device_graph> build(graph_info_t* info, const std::string& path, bool weighted,
                                            bool build_csr, bool relabel) {
-  uint32_t V = 0;
-  for (size_t i = 0; i < el.src.size(); ++i) V = std::max(V, std::max(el.src[i], el.dst[i]) + 1);
-  if (V == 0) V = 1;
+  const std::string text = read_file(path);
   uint32_t flags = info->is_directed_ ? 0u : PE_GRAPH_SYMMETRIZE;
   if (build_csr) flags |= PE_GRAPH_BUILD_CSR;
   if (weighted) flags |= PE_GRAPH_WEIGHTED;
   if (relabel) flags |= PE_GRAPH_RELABEL_DEGREE;
   auto g = std::make_shared<device_graph>();
-  check_status(pe_graph_create(cluster_info_t::get_instance().ctx(), el.src.data(), el.dst.data(),
-                               weighted ? el.w.data() : nullptr, el.src.size(), V, flags, &g->h),
-               "pe_graph_create");
+  check_status(pe_graph_create_csv(cluster_info_t::get_instance().ctx(), text.data(), text.size(), flags, &g->h),
+               "pe_graph_create_csv");
   check_status(pe_graph_get_info(g->h, &g->info), "pe_graph_get_info");
   info->vertices_ = g->info.vertices;
   info->edges_ = g->info.edges;
@@ -146,7 +125,7 @@ std::shared_ptr<dcsc_t<EDATA>> create_dcsc_seqs_from_path(graph_info_t* info, co
                                                           edge_format_t = edge_format_t::CSV,
                                                           DECODER = nullptr, bool relabel = false) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
-  auto g = detail::build(info, detail::read_csv(path, weighted), weighted, false, relabel);
+  auto g = detail::build(info, path, weighted, false, relabel);
   return std::make_shared<dcsc_t<EDATA>>(g);
 }
 
@@ -155,7 +134,7 @@ std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqd_from_path(graph_info_t* info, co
                                                           edge_format_t = edge_format_t::CSV,
                                                           DECODER = nullptr) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
-  auto g = detail::build(info, detail::read_csv(path, weighted), weighted, true, false);
+  auto g = detail::build(info, path, weighted, true, false);
   return std::make_shared<bcsr_t<EDATA>>(g);
 }
 
@@ -164,7 +143,7 @@ std::pair<std::shared_ptr<bcsr_t<EDATA>>, std::shared_ptr<dcsc_t<EDATA>>> create
     graph_info_t* info, const std::string& path, edge_format_t = edge_format_t::CSV, DECODER = nullptr,
     bool relabel = false) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
-  auto g = detail::build(info, detail::read_csv(path, weighted), weighted, true, relabel);
+  auto g = detail::build(info, path, weighted, true, relabel);
   return {std::make_shared<bcsr_t<EDATA>>(g), std::make_shared<dcsc_t<EDATA>>(g)};
 }
 
