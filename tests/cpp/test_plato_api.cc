@@ -9,6 +9,7 @@
 #include "plato/algo/bfs/bfs.hpp"
 #include "plato/algo/pagerank/pagerank.hpp"
 #include "plato/algo/ppr/personalized_pagerank.hpp"
+#include "plato/algo/trustrank/trustrank.hpp"
 
 static int failures = 0;
 #define EXPECT(cond)                                                   \
@@ -129,6 +130,40 @@ static void BFS_BinaryTree() {
   EXPECT(plato::algo::breadth_first_search(*u.second, *u.first, undirected, opts) == 20);
 }
 
+// TrustRank on the chain 0->1->...->9 (trustrank.hpp:80-259).  PageRank grows along
+// the chain, so its single top seed is 9; inverse PageRank ranks the reversed chain,
+// whose top is 0.  With good = {0} and seed_num >= 10 every good vertex is a seed and
+// TrustRank is the PPR from 0: r(v) = 0.15 * 0.85^v.
+static void TrustRank_Chain() {
+  plato::graph_info_t graph_info(true);
+  auto g = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/graph_10_9.csv");
+  plato::bitmap_t<> all(graph_info.max_v_i_ + 1);
+  all.fill();
+  plato::algo::trustrank_opts_t opts;
+  opts.seed_num_ = 1;
+  opts.select_method_ = 1;
+  auto s1 = plato::algo::select_good_seeds(*g.second, *g.first, graph_info, all, opts);
+  auto v1 = s1.to_vector();
+  EXPECT(v1.size() == 1 && v1[0] == 9);
+  opts.select_method_ = 0;
+  auto s0 = plato::algo::select_good_seeds(*g.second, *g.first, graph_info, all, opts);
+  auto v0 = s0.to_vector();
+  EXPECT(v0.size() == 1 && v0[0] == 0);
+  plato::bitmap_t<> good(graph_info.max_v_i_ + 1);
+  good.set_bit(0);
+  opts.seed_num_ = 100;
+  opts.iteration_ = 20;
+  opts.eps_ = 0.0;
+  auto r = plato::algo::trustrank(*g.second, *g.first, graph_info, good, opts);
+  double expect = 0.15;
+  for (plato::vid_t v = 0; v < 10; ++v) {
+    EXPECT(std::fabs(r[v] - expect) <= 1e-14 * expect);
+    expect *= 0.85;
+  }
+  auto pairs = plato::algo::generate_rank_pairs(*g.second, graph_info);
+  EXPECT(pairs.size() == 10 && pairs[9].v_i == 9 && pairs[9].rank > pairs[0].rank);
+}
+
 int main(int argc, char** argv) {
   dir = argc > 1 ? argv[1] : "tests/golden";
   struct { const char* name; void (*fn)(); } cases[] = {
@@ -138,6 +173,7 @@ int main(int argc, char** argv) {
       {"PageRank.ChainClosedForm", PageRank_ChainClosedForm},
       {"PPR.ChainClosedForm", PPR_ChainClosedForm},
       {"BFS.BinaryTree", BFS_BinaryTree},
+      {"TrustRank.Chain", TrustRank_Chain},
   };
   for (auto& c : cases) {
     const int before = failures;

@@ -16,7 +16,7 @@ def test_host_programs_build():
     r = subprocess.run(["make", "-s", "-f", "plato_amd/host/Makefile", f"ROOT={ROOT}"], cwd=ROOT,
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "test_plato_api"):
+    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "test_plato_api"):
         assert os.access(os.path.join(BIN, p), os.X_OK)
 
 
@@ -38,7 +38,7 @@ def test_cpp_api_known_answers():
                        text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("  OK  ") == 6
+    assert r.stdout.count("  OK  ") == 7
 
 
 @pytest.mark.gpu
@@ -81,3 +81,33 @@ def test_bfs_driver_levels(tmp_path):
     ref = oracle.bfs(oracle.CSC(100, s, d, symmetrize=True), 7)
     assert np.array_equal(got[:, 1], ref["levels"])
     assert f"visited: {ref['visited']}" in r.stdout
+
+
+@pytest.mark.gpu
+def test_trustrank_driver_matches_oracle(tmp_path):
+    """select_method 1 (PageRank seeds) on a directed RMAT scale-10 graph written as CSV,
+    good = even ids: ranks equal the oracle's seed-set PPR over the oracle's top-30 good
+    seeds (FP64, rtol 1e-12).  The test requires a gap of more than 1e-12 relative
+    between consecutive top ranks, so summation order cannot reorder the seeds."""
+    s, d = oracle.rmat_edges(10, 16, 4)
+    path = tmp_path / "edges.csv"
+    path.write_text("".join(f"{a},{b}\n" for a, b in zip(s.tolist(), d.tolist())))
+    V = int(max(s.max(), d.max())) + 1
+    g = oracle.CSC(V, s, d)
+    pr, _ = oracle.pagerank(g, 100, 0.85, 0.001)
+    ids, vals = oracle.topk(pr, 31)
+    assert np.all(np.abs(np.diff(vals)) > 1e-12 * vals[0])
+    seeds = [int(v) for v in ids[:30] if v % 2 == 0]
+    goods = tmp_path / "goods.csv"
+    goods.write_text("".join(f"{v}\n" for v in range(0, V, 2)))
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([os.path.join(BIN, "trustrank"), "--input_edges", str(path), "--input_goods", str(goods),
+                        "--is_directed=true", "--seed_num", "30", "--select_method", "1",
+                        "--iterations", "30", "--eps", "0", "--output", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(out / "0000_0.csv", delimiter=",")
+    ref, _ = oracle.ppr(g, seeds, 30, 0.85, 0.0)
+    assert np.array_equal(got[:, 0], np.arange(V))
+    np.testing.assert_allclose(got[:, 1], ref, rtol=1e-12, atol=1e-300)
