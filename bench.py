@@ -79,11 +79,12 @@ def profiled_traffic(kernel, scale):
     return None
 
 
-def measure_pull(ctx, g, layout, steps, warmup):
+def measure_pull(ctx, g, layout, steps, warmup, weighted=False):
     """Time `steps` FP32 PageRank iterations on one pull layout: per-kernel HIP events on the
     library stream (timed run), then the same iterations replayed from a captured graph."""
     from plato_amd import _lib as L
-    from plato_amd.algo import pagerank, pagerank_opts_t
+    from plato_amd.algo import pagerank as plain, pagerank_opts_t, weighted_pagerank
+    pagerank = weighted_pagerank if weighted else plain
     t0 = time.time()
     g.set_pull_layout(layout)
     layout_s = time.time() - t0
@@ -111,8 +112,44 @@ def measure_pull(ctx, g, layout, steps, warmup):
                    extra={"scatter_avg_ms": round(avg[0], 4), "gather_avg_ms": round(avg[1], 4),
                           "layout_build_s": round(layout_s, 2), "run_records": int(info.binned_runs),
                           "layout_bytes": int(info.binned_bytes)})
-    log(f"[bench] {layout}: {out['ms_per_step']:.3f} ms/iteration (timed), {out['graph_ms']:.3f} ms "
+    log(f"[bench] {'weighted ' if weighted else ''}{layout}: {out['ms_per_step']:.3f} ms/iteration (timed), {out['graph_ms']:.3f} ms "
         f"(graph replay), kernel {out['kernel_ms']:.3f} ms")
+    return out
+
+
+def run_weighted(ctx, args, layouts):
+    """weighted_pagerank on the same RMAT graph with FP32 edge weights uniform in (0, 1],
+    normalised per source: both pull layouts, algorithmic bytes 8(V+1) + 8E + 12V."""
+    from plato_amd import Graph
+    from plato_amd.algo import pagerank_opts_t, weighted_pagerank
+    log(f"[bench] weighted RMAT scale {args.scale}")
+    t0 = time.time()
+    g = Graph.rmat(ctx, args.scale, args.edge_factor, seed=args.seed, weighted=True,
+                   relabel=not args.no_relabel)
+    build_s = time.time() - t0
+    V, E = g.V, g.nnz
+    runs = {lay: measure_pull(ctx, g, lay, args.steps, args.warmup, weighted=True) for lay in layouts}
+    best = min(runs, key=lambda k: runs[k]["ms_per_step"])
+    head = runs[best]
+    alg_bytes = 8 * (V + 1) + 8 * E + 12 * V
+    achieved = alg_bytes / (head["kernel_ms"] * 1e-3) / 1e9
+    out = {"gteps": round(E / (head["ms_per_step"] * 1e-3) / 1e9, 2),
+           "ms_per_step": round(head["ms_per_step"], 4), "pull_layout": best, "graph_build_s": round(build_s, 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": head["kernel"],
+                        "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(head["kernel_ms"], 4)},
+           "pull_layouts": {k: {kk: vv for kk, vv in r.items() if kk != "extra"} | r["extra"]
+                            for k, r in runs.items()}}
+    if not args.no_accuracy:
+        g.set_pull_layout(best)
+        r32 = weighted_pagerank(g, pagerank_opts_t(1000, 0.85, 0.0), precision=32, l1_norm_tol=1e-7)
+        r64 = weighted_pagerank(g, pagerank_opts_t(1000, 0.85, 0.0), precision=64, l1_norm_tol=1e-12)
+        diff = np.abs(r32.rank - r64.rank)
+        out["accuracy"] = {"fp32_iterations": r32.iterations, "fp64_iterations": r64.iterations,
+                           "l1_normalised": float(diff.sum() / np.abs(r64.rank).sum()),
+                           "linf_rel": float(np.max(diff / np.maximum(np.abs(r64.rank), 1e-300)))}
+        del r32, r64, diff
+    g.close()
     return out
 
 
@@ -300,6 +337,7 @@ def main():
                     help="also run a 1 GiB device copy (reference dispatch for traffic counters)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-weighted", action="store_true", help="skip the weighted PageRank leg")
     ap.add_argument("--bfs-scale", type=int, default=24)
     ap.add_argument("--bfs-roots", type=int, default=64)
     ap.add_argument("--no-bfs", action="store_true")
@@ -405,6 +443,8 @@ def main():
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g)
     g.close()
+    if not args.no_weighted:
+        out["weighted"] = run_weighted(ctx, args, layouts)
     if not args.no_bfs:
         log(f"[bench] bfs scale {args.bfs_scale}, {args.bfs_roots} roots")
         out["bfs"] = run_bfs(ctx, args.bfs_scale, args.bfs_roots)

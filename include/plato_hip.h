@@ -350,7 +350,7 @@ typedef struct {
   double damping;       /* 1 - alpha */
   double tol;           /* per-column delta threshold (the reference's eps_) */
   uint32_t max_iters;   /* iteration cap per column (the reference's iteration_) */
-  uint32_t task_edges;  /* edges per wave task (0: 256); longer columns are split */
+  uint32_t task_edges;  /* edges per wave task (0: 4096); longer columns are split */
 } pe_ppr_batch_opts_t;
 
 int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_batch_t** out);

@@ -2,7 +2,7 @@
 // fastest.  Tunables and the knob each one sets:
 //   pull tile size      merge-path items per tile of the pull gather (1024/2048/4096);
 //                       applied to the graph
-//   batch task size     edges per wave task of the 64-column PPR block; longer
+//   batch task size     edges per wave task of the 64-column PPR block (256 to 16384); longer
 //                       columns are split (the block's degree threshold)
 //   BFS switch ratio    push iff active edges / E' < ratio (the reference's rule,
 //                       0.05 by default; levels do not depend on it)
@@ -79,7 +79,7 @@ int pe_autotune(pe_graph_t* g, uint32_t what, pe_tune_result_t* out) {
   return pe::guarded(g->ctx, [&] {
     pe_tune_result_t r{};
     r.tile_items = g->tile_items;
-    r.batch_task_edges = 256;
+    r.batch_task_edges = 4096;
     r.bfs_push_threshold = 0.05;
     if (what & PE_TUNE_PULL) {
       const uint32_t cand[3] = {1024, 2048, 4096};
@@ -96,7 +96,7 @@ int pe_autotune(pe_graph_t* g, uint32_t what, pe_tune_result_t* out) {
       r.pull_ms = best;
     }
     if (what & PE_TUNE_BATCH) {
-      const uint32_t cand[4] = {128, 256, 512, 1024};
+      const uint32_t cand[4] = {256, 1024, 4096, 16384};
       double best = 1e300;
       for (uint32_t c : cand) {
         const double ms = time_batch(g, c, 2);

@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void k_bfs_push_lb(
     uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits, bfs_counters* __restrict__ cnt,
     const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   const uint64_t T = pref[n];
   unsigned long long active = 0;
   for (uint64_t c = wave_id; c * 256 < T; c += nwaves) {
@@ -155,8 +155,8 @@ __global__ __launch_bounds__(256) void k_bfs_push(
     uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
     bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   const uint64_t n = n_dev ? (uint64_t)*n_dev : n_host;
   unsigned long long active = 0;
   for (uint64_t q = wave_id; q < n; q += nwaves) {
@@ -249,8 +249,8 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
     const uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt,
     const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   const unsigned long long n = cnt->heavy_count;
   unsigned long long active = 0;
   for (uint64_t h = wave_id; h < n; h += nwaves) {

@@ -120,6 +120,20 @@ inline unsigned persistent_grid(pe_ctx_s* ctx, uint64_t tiles, unsigned blocks_p
   return cap ? static_cast<unsigned>(cap) : 1u;
 }
 
+// A wave-uniform value marked as such: addresses derived from it are computed in
+// scalar registers and their loads go through the scalar path (one load per wave).
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Index of the calling wave in the grid, and the number of waves (both wave-uniform).
+__device__ __forceinline__ uint64_t grid_wave_id() {
+  return wave_uniform((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6);
+}
+__device__ __forceinline__ uint64_t grid_waves() { return ((uint64_t)gridDim.x * blockDim.x) >> 6; }
+
 }  // namespace pe
 
 // ---- binned pull layout (pull_binned.hip) -------------------------------------------

@@ -83,7 +83,7 @@ __device__ __forceinline__ float batch_apply(const lane_cfg& L, double damping, 
 }
 
 #ifndef PE_BATCH_UNROLL
-#define PE_BATCH_UNROLL 8
+#define PE_BATCH_UNROLL 4
 #endif
 static constexpr int BU = PE_BATCH_UNROLL;  // row gathers in flight per wave
 // Cache hints: the new block Y and the edge list stream through once per pass, so
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
   L.init_dist = init_dist[lane];
   L.restart = (1.0 - damping) * L.init_dist;
   double delta = 0.0;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   for (uint64_t ti = wave_id; ti < ntasks; ti += nwaves) {
     const bt_task t = tasks[ti];
     if (t.chunk == 0xFFFFFFFFu) {
@@ -200,8 +200,8 @@ __global__ __launch_bounds__(256) void k_batch_fixup(
   L.init_dist = init_dist[lane];
   L.restart = (1.0 - damping) * L.init_dist;
   double delta = 0.0;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   for (uint64_t li = wave_id; li < nlong; li += nwaves) {
     const bt_long lc = longs[li];
     float s = 0.f, c = 0.f;
@@ -514,7 +514,7 @@ int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_b
     auto b = std::make_unique<pe_ppr_batch_s>();
     b->g = g;
     b->opts = *opts;
-    if (!b->opts.task_edges) b->opts.task_edges = 256;
+    if (!b->opts.task_edges) b->opts.task_edges = 4096;
     const uint64_t cells = (uint64_t)std::max<uint32_t>(1, g->V) * BK;
     b->X0.alloc(ctx, cells);
     b->X1.alloc(ctx, cells);

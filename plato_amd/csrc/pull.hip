@@ -346,8 +346,8 @@ __global__ __launch_bounds__(256) void k_pull_fixup_long(
     final_iter = a.final_iter;
   }
   const int lane = threadIdx.x & 63;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   double delta = 0.0, rsum = 0.0;
   for (uint64_t i = wave_id; i < n; i += nwaves) {
     const uint32_t t0 = fix_t0[i], t1 = fix_t1[i];

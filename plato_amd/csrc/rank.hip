@@ -64,8 +64,8 @@ __global__ __launch_bounds__(256) void k_ppr_push(const uint64_t* __restrict__ a
                                                   const uint32_t* __restrict__ idx, const T* __restrict__ x,
                                                   T* __restrict__ y, uint64_t* __restrict__ newly) {
   const int lane = threadIdx.x & 63;
-  const uint64_t wave_id = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave_id = grid_wave_id();
+  const uint64_t nwaves = grid_waves();
   for (uint64_t wi = wave_id; wi < nwords; wi += nwaves) {
     uint64_t w = active[wi];
     while (w) {

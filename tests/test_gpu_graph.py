@@ -184,7 +184,7 @@ def test_autotune_choices_are_valid_and_keep_results(ctx, tmp_path):
     before = pagerank(g, pagerank_opts_t(10, 0.85, 0.0), precision=64).rank
     r = g.autotune(pull=True, batch=True, bfs=True)
     assert r["tile_items"] in (1024, 2048, 4096) and g.info.tile_items == r["tile_items"]
-    assert r["batch_task_edges"] in (128, 256, 512, 1024)
+    assert r["batch_task_edges"] in (256, 1024, 4096, 16384)
     assert r["bfs_push_threshold"] in (0.01, 0.02, 0.05, 0.1, 0.2)
     assert r["pull_ms"] > 0 and r["batch_ms"] > 0 and r["bfs_ms"] > 0
     after = pagerank(g, pagerank_opts_t(10, 0.85, 0.0), precision=64).rank
