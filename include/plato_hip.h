@@ -174,7 +174,7 @@ int pe_graph_set_tile(pe_graph_t* g, uint32_t tile_items);
 #define PE_PULL_BINNED 1
 typedef struct {
   uint32_t src_bits;    /* log2 sources per chunk staged in LDS (<= 15; default 15) */
-  uint32_t dst_bits;    /* log2 destinations per bin summed in LDS (<= 13; default 13) */
+  uint32_t dst_bits;    /* log2 destinations per bin summed in LDS (<= 14; default 14) */
   uint32_t run_cap;     /* max entries per scatter run record (default 256) */
   uint32_t unit_log2;   /* log2 max entries per gather unit; bigger bins are split (default 20) */
 } pe_binned_opts_t;

@@ -1,5 +1,15 @@
 // Fixed-point terms of the binned pull (pull_binned.hip), host and device.
 //
+// The kernels use the single-word form (fixed64_*): a term x (non-negative, finite)
+// is the integer floor(x * 2^F) in one 64-bit word, with F fixed for a whole
+// iteration by pb_frac_bits(M), M a bound on the total of all terms (the rank mass
+// that flows along the edges): every per-destination sum of terms then stays below
+// 2^61, so 64-bit integer sums cannot overflow and are exact and order-independent.
+// Each term is truncated at 2^-F (at most one unit of 2^-F per term).
+//
+// The two-word form below keeps 96 fractional bits whatever the magnitude; it is
+// kept as the reference the single-word form is checked against on the host.
+//
 // A term x (non-negative, finite, below 2^32) is the integer R = floor(x * 2^PB_FRAC)
 // with PB_FRAC = 96, kept as two accumulator parts: `hi` = bits 64..127 of R and
 // `d1` = bits 32..63 of R (carried in u128::lo as d1 << 32); bits below 2^32 of R
@@ -80,6 +90,41 @@ PB_HD u128 fixed_f64(double v) {
   const uint64_t m = (b & ((1ull << 52) - 1ull)) | (1ull << 52);
   return shl128(m, e - 1075 + PB_FRAC);
 }
+
+// floor(m * 2^k) for k in (-64, 64) (0 for k <= -64), branch-free; the caller keeps
+// the result below 2^64.
+PB_HD uint64_t pb_shl64(uint64_t m, int k) {
+  const uint64_t l = m << (k & 63), r = m >> ((-k) & 63);
+  return k >= 0 ? (k < 64 ? l : 0ull) : (k > -64 ? r : 0ull);
+}
+
+// Fractional bits F for terms whose total is at most `mass`: mass * 2^F <= 2^61.
+PB_HD int pb_frac_bits(double mass) {
+  if (!(mass >= 0x1p-60)) mass = 0x1p-60;
+  int e = 0;
+  (void)frexp(mass, &e);  // mass < 2^e
+  return 61 - e;
+}
+
+// FP32 term -> floor(x * 2^F); zero and subnormals of FP32 give their exact floor too.
+PB_HD uint64_t fixed64_f32(float v, int F) {
+  const uint32_t b = pb_bits32(v);
+  const uint32_t e = (b >> 23) & 0xFFu;
+  const uint32_t m = e ? ((b & 0x7FFFFFu) | 0x800000u) : (b & 0x7FFFFFu);
+  return pb_shl64(m, (e ? (int)e : 1) - 150 + F);
+}
+
+// FP64 term (weighted graphs: the exact product w * x) -> floor(x * 2^F).
+PB_HD uint64_t fixed64_f64(double v, int F) {
+  const uint64_t b = pb_bits64(v);
+  if (b >> 63) return 0ull;
+  const int e = (int)((b >> 52) & 0x7FFull);
+  const uint64_t m = e ? ((b & ((1ull << 52) - 1ull)) | (1ull << 52)) : (b & ((1ull << 52) - 1ull));
+  return pb_shl64(m, (e ? e : 1) - 1075 + F);
+}
+
+// Value of a single-word accumulator.
+PB_HD double from_fixed64(uint64_t acc, int F) { return ldexp((double)acc, -F); }
 
 // Value of a destination's accumulator words (lo = sum of d1 parts, hi = sum of hi parts).
 PB_HD double from_fixed(uint64_t lo, uint64_t hi) {

@@ -147,7 +147,7 @@ struct pb_unit2 {  // gather work: entries [e0, e1) of one bin; slot ~0u = the w
   uint32_t bin, slot;
 };
 struct pb_layout {
-  uint32_t src_bits = 15, dst_bits = 13, run_cap = 256, unit_log2 = 20;
+  uint32_t src_bits = 15, dst_bits = 14, run_cap = 256, unit_log2 = 20;
   uint32_t nchunks = 0, nbins = 0;
   uint64_t nrecords = 0;
   uint64_t nupdates = 0;                // distinct (source, bin) pairs
@@ -164,7 +164,7 @@ struct pb_layout {
   dbuf<pb_unit1> units1;
   dbuf<pb_unit2> units2;
   dbuf<uint32_t> split_bin;             // [nsplit] bins summed by several units
-  dbuf<unsigned long long> acc;         // [2 * nsplit << dst_bits] lo, hi words
+  dbuf<unsigned long long> acc;         // [nsplit << dst_bits] fixed-point sums of split bins
   uint64_t bytes() const {
     return src_upd.bytes() + dst_local.bytes() + gbase.bytes() + w.bytes() + values.bytes() +
            rec_start.bytes() + rec_len.bytes() + units1.bytes() + units2.bytes() + split_bin.bytes() +

@@ -514,7 +514,7 @@ void rank_work::init(pe_graph_s* g, int precision) {
 void rank_iteration(pe_graph_s* g, int precision, void* b0, void* b1, bool weighted,
                     const rank_args& a, iter_ctl* ctl, rank_work& wk, double* deltas,
                     double* rsums) {
-  if (precision != 64 && a.mode == MODE_APPLY && g->pull_layout == PE_PULL_BINNED && g->pb) {
+  if (precision != 64 && a.mode == MODE_APPLY && a.mass > 0.0 && g->pull_layout == PE_PULL_BINNED && g->pb) {
     PE_REQUIRE(wk.pb_parts > 0, PE_ERR_STATE, "pull layout changed after the iteration was set up");
     binned_iteration(g, (float*)b0, (float*)b1, weighted, a, ctl, wk.part_delta.p, wk.part_rsum.p);
     k_iter_finish<<<1, 256, 0, g->ctx->stream>>>(wk.part_delta.p, wk.part_rsum.p, wk.pb_parts, ctl, deltas,

@@ -16,6 +16,9 @@ struct rank_args {
   int32_t divide_by_od;   // store r/od (PageRank, PPR) or r (weighted PageRank)
   int32_t final_iter;     // used when no controller is attached
   int32_t mode;           // MODE_APPLY or MODE_RAW_SUM (gather only)
+  double mass;            // bound on the total of all contributions sum_v s(v) (the rank
+                          // mass that flows along edges); sets the binned gather's
+                          // fixed-point scale
 };
 
 // Device-resident loop controller: lets iterations be enqueued (or replayed
@@ -115,6 +118,10 @@ unsigned binned_grid_gather(pe_graph_s* g);
 unsigned binned_grid_split(pe_graph_s* g);
 void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const rank_args& a, iter_ctl* ctl,
                       double* part_delta, double* part_rsum);
+// Bound on the total contribution of state x (rank_args::mass) for a binned iteration;
+// 0 when x has a negative or non-finite source value (such states take the merge-path
+// gather).  Synchronizes the stream.
+double binned_mass_bound(pe_graph_s* g, const float* x, bool weighted);
 void rank_apply_only(pe_graph_s* g, int precision, const void* x, void* y, const rank_args& a,
                      rank_work& wk);
 

@@ -15,16 +15,18 @@
 // blocking", IPDPS 2017; Lakhotia, Kannan, Prasanna, "Accelerating PageRank using
 // partition-centric processing", USENIX ATC 2018.)
 //
-// Sums are exact and order-independent.  Each contribution (non-negative, finite,
-// below 2^32) becomes a fixed-point integer with 96 fractional bits, truncated to a
-// resolution of 2^-64 (the FP64 products w * x of weighted graphs are formed exactly
-// first); these integers are added with 64-bit LDS atomics into two words per
-// destination that never carry into each other.  Integer addition is associative, so
-// every sum is independent of order, launch shape and layout parameters: results are
-// bit-reproducible.  The truncation moves a sum by less than 2^-64 per term (under
-// 2^-33 over 2^31 edges).  The bound on the values holds for every rank iteration
-// here: the sum into one vertex is at most the total rank mass, at most V < 2^32.
-//
+// Sums are exact and order-independent.  Each contribution (non-negative, finite)
+// becomes the integer floor(x * 2^F) (the FP64 products w * x of weighted graphs are
+// formed exactly first), and these integers are added with 64-bit LDS atomics into one
+// word per destination.  F is fixed per iteration from a bound M on the total of all
+// contributions (pb_frac_bits: M * 2^F <= 2^61): for the rank iterations M is the
+// rank mass, at most V for PageRank (ranks start at 1 and the restart adds
+// (1 - d) per vertex while the damped flow keeps at most d of the mass) and at most 1
+// for PPR, so no sum can reach 2^64.  Integer addition is associative: every sum is
+// independent of order, launch shape and layout parameters, and results are
+// bit-reproducible.  The truncation moves each term by less than 2^-F (2^-33 for
+// PageRank on 2^27 vertices, 2^-60 for PPR).
+
 // Layout (bin order): every CSC entry (v <- u) gets the key
 //   bin = v >> dst_bits | chunk = u >> src_bits | u & smask | v & dmask
 // and the entries are radix-sorted by it.  An update is one (bin, source) pair: the
@@ -49,7 +51,10 @@ static constexpr int PB_THREADS = 1024;
 #ifndef PB_MAX_SRC_BITS
 #define PB_MAX_SRC_BITS 15  // 128 KiB of FP32 sources in LDS
 #endif
-static constexpr int PB_MAX_DST_BITS = 13;  // 128 KiB of 128-bit accumulators in LDS
+static constexpr int PB_MAX_DST_BITS = 14;  // 128 KiB of 64-bit accumulators in LDS
+#ifndef PB_DST_BITS
+#define PB_DST_BITS 14  // destinations per bin (default)
+#endif
 static constexpr uint64_t PB_UNIT1 = 1ull << 20;  // max entries per scatter unit
 static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the run scan
 #ifndef PB_SCATTER_RECORDS
@@ -59,11 +64,9 @@ static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the 
 #define PB_GATHER_UNROLL 4    // 8-entry groups per lane at a time in the gather
 #endif
 
-// Adds one converted term into a destination's two accumulator words.
-__device__ __forceinline__ void acc_add(unsigned long long* lo, unsigned long long* hi, u128 v) {
-  const uint64_t d1 = v.lo >> 32;
-  if (d1) atomicAdd(lo, (unsigned long long)d1);
-  if (v.hi) atomicAdd(hi, (unsigned long long)v.hi);
+// Adds one converted term into a destination's accumulator word.
+__device__ __forceinline__ void acc_add(unsigned long long* acc, uint64_t v) {
+  if (v) atomicAdd(acc, (unsigned long long)v);
 }
 
 // ---- scatter ----------------------------------------------------------------------------
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
     if (ctl->done) return;
     if (ctl->cur) x = buf1;
   }
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int WAVES = PB_THREADS / 64;
   for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const pb_unit1 u = units[ui];
@@ -130,23 +133,22 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
 // Contribution of entry i (unaligned head or tail of a unit): its update is the last
 // one started at or before i, counted from the prefix of its aligned group.
 template <bool WEIGHTED>
-__device__ __forceinline__ u128 pb_term(const uint16_t* dst_local, const uint32_t* gbase, const float* values,
-                                        const float* w, uint64_t i) {
+__device__ __forceinline__ uint64_t pb_term(const uint16_t* dst_local, const uint32_t* gbase, const float* values,
+                                            const float* w, uint64_t i, int frac) {
   uint32_t cnt = 0;
   for (uint64_t j = i & ~7ull; j <= i; ++j) cnt += (uint32_t)(dst_local[j] >> 15);
   const float x = values[gbase[i >> 3] + cnt - 1];
-  return WEIGHTED ? fixed_f64((double)w[i] * (double)x) : fixed_f32_acc(x);
+  return WEIGHTED ? fixed64_f64((double)w[i] * (double)x, frac) : fixed64_f32(x, frac);
 }
 
 template <bool WEIGHTED, int GU>
 __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
     const pb_unit2* __restrict__ units, uint32_t nunits, const float* __restrict__ values,
     const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ gbase,
-    uint32_t V, int dst_bits,
+    uint32_t V, int dst_bits, int frac,
     float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
     unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum) {
-  __shared__ unsigned long long slo[1 << PB_MAX_DST_BITS];
-  __shared__ unsigned long long shi[1 << PB_MAX_DST_BITS];
+  __shared__ unsigned long long sacc[1 << PB_MAX_DST_BITS];
   __shared__ double s_red[2 * PB_THREADS / 64];
   bool final_iter = a.final_iter;
   const float* x = buf0;
@@ -164,10 +166,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
   double delta = 0.0, rsum = 0.0;
   for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const pb_unit2 u = units[ui];
-    for (uint32_t i = tid; i < pd; i += PB_THREADS) {
-      slo[i] = 0ull;
-      shi[i] = 0ull;
-    }
+    for (uint32_t i = tid; i < pd; i += PB_THREADS) sacc[i] = 0ull;
     __syncthreads();
     // Entries in aligned groups of 8 per lane (one 16-byte load of destinations, the
     // group's update count, then the 8 values by update index: entries of one update
@@ -179,12 +178,12 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
       const uint64_t h1 = grouped ? a0 : u.e1;
       for (uint64_t i = u.e0 + tid; i < h1; i += PB_THREADS) {
         const uint32_t d = dst_local[i] & 0x7FFFu;
-        acc_add(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
+        acc_add(&sacc[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i, frac));
       }
       if (grouped)
         for (uint64_t i = a1 + tid; i < u.e1; i += PB_THREADS) {
           const uint32_t d = dst_local[i] & 0x7FFFu;
-          acc_add(&slo[d], &shi[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i));
+          acc_add(&sacc[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i, frac));
         }
     }
     if (grouped) {
@@ -233,8 +232,8 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
           }
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
-            const u128 f = WEIGHTED ? fixed_f64((double)wv[k] * (double)xv[q][k]) : fixed_f32_acc(xv[q][k]);
-            acc_add(&slo[d[k]], &shi[d[k]], f);
+            acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)xv[q][k], frac)
+                                          : fixed64_f32(xv[q][k], frac));
           }
         }
       }
@@ -245,15 +244,12 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
       for (uint32_t i = tid; i < pd; i += PB_THREADS) {
         const uint64_t v = v0 + i;
         if (v >= V) break;
-        delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed(slo[i], shi[i]), x, y, od, rsum);
+        delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed64(sacc[i], frac), x, y, od, rsum);
       }
     } else {
-      unsigned long long* alo = acc + ((uint64_t)u.slot << (dst_bits + 1));
-      unsigned long long* ahi = alo + pd;
-      for (uint32_t i = tid; i < pd; i += PB_THREADS) {
-        if (slo[i]) atomicAdd(&alo[i], slo[i]);
-        if (shi[i]) atomicAdd(&ahi[i], shi[i]);
-      }
+      unsigned long long* ga = acc + ((uint64_t)u.slot << dst_bits);
+      for (uint32_t i = tid; i < pd; i += PB_THREADS)
+        if (sacc[i]) atomicAdd(&ga[i], sacc[i]);
     }
     __syncthreads();
   }
@@ -266,7 +262,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
 
 // Bins summed by several units: apply from the global accumulators, then clear them.
 __global__ __launch_bounds__(256) void k_pb_apply_split(
-    const uint32_t* __restrict__ split_bin, uint32_t nsplit, int dst_bits, uint32_t V,
+    const uint32_t* __restrict__ split_bin, uint32_t nsplit, int dst_bits, int frac, uint32_t V,
     unsigned long long* __restrict__ acc, float* buf0, float* buf1, const uint32_t* __restrict__ od,
     rank_args a, const iter_ctl* __restrict__ ctl, double* __restrict__ part_delta,
     double* __restrict__ part_rsum) {
@@ -287,18 +283,54 @@ __global__ __launch_bounds__(256) void k_pb_apply_split(
   double delta = 0.0, rsum = 0.0;
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t slot = j >> dst_bits, i = j & (pd - 1);
-    unsigned long long* alo = acc + (slot << (dst_bits + 1));
-    const uint64_t lo = alo[i], hi = alo[pd + i];
-    alo[i] = 0ull;
-    alo[pd + i] = 0ull;
+    unsigned long long* ga = acc + (slot << dst_bits);
+    const uint64_t sum = ga[i];
+    ga[i] = 0ull;
     const uint64_t v = ((uint64_t)split_bin[slot] << dst_bits) + i;
-    if (v < V) delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed(lo, hi), x, y, od, rsum);
+    if (v < V) delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed64(sum, frac), x, y, od, rsum);
   }
   block_reduce2<256>(delta, rsum, s_red);
   if (threadIdx.x == 0) {
     part_delta[blockIdx.x] = delta;
     part_rsum[blockIdx.x] = rsum;
   }
+}
+
+// Total contribution of a state x along the edges: sum over sources of od(u) * x(u)
+// (weighted graphs: x(u), their weights sum to 1 per source).  A block's partial is
+// -1 when some source value is negative or not finite.
+__global__ __launch_bounds__(256) void k_pb_mass(const float* __restrict__ x, const uint32_t* __restrict__ od,
+                                                 uint32_t V, int weighted, double* __restrict__ part) {
+  __shared__ double s_red[8];
+  double acc = 0.0, bad = 0.0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = od[v];
+    if (!d) continue;
+    const float xv = x[v];
+    if (!(xv >= 0.0f) || isinf(xv)) bad = 1.0;
+    else acc += weighted ? (double)xv : (double)d * (double)xv;
+  }
+  block_reduce2<256>(acc, bad, s_red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bad > 0.0 ? -1.0 : acc;
+}
+
+double binned_mass_bound(pe_graph_s* g, const float* x, bool weighted) {
+  pe_ctx_s* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const unsigned G = grid_for(g->V, 256, 1024);
+  dbuf<double> part(ctx, G);
+  k_pb_mass<<<G, 256, 0, s>>>(x, g->od.p, g->V, weighted ? 1 : 0, part.p);
+  PE_HIP(hipGetLastError());
+  std::vector<double> h(G);
+  PE_HIP(hipMemcpyAsync(h.data(), part.p, G * sizeof(double), hipMemcpyDeviceToHost, s));
+  PE_HIP(hipStreamSynchronize(s));
+  double total = 0.0;
+  for (double p : h) {
+    if (p < 0.0) return 0.0;
+    total += p;
+  }
+  // FP32 weights are rounded per edge (their sum per source exceeds 1 by < 2^-23)
+  return std::max(total * (1.0 + 0x1p-20), 0x1p-60);
 }
 
 // ---- layout build -----------------------------------------------------------------------------
@@ -417,7 +449,7 @@ static void upload(pe_ctx_s* ctx, dbuf<T>& d, const std::vector<T>& h) {
 void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   pe_ctx_s* ctx = g->ctx;
   hipStream_t s = ctx->stream;
-  pe_binned_opts_t o{(uint32_t)PB_MAX_SRC_BITS, 13u, 256u, 20u};
+  pe_binned_opts_t o{(uint32_t)PB_MAX_SRC_BITS, (uint32_t)PB_DST_BITS, 256u, 20u};
   if (opts) {
     if (opts->src_bits) o.src_bits = opts->src_bits;
     if (opts->dst_bits) o.dst_bits = opts->dst_bits;
@@ -425,7 +457,7 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
     if (opts->unit_log2) o.unit_log2 = opts->unit_log2;
   }
   PE_REQUIRE(o.src_bits >= 4 && o.src_bits <= (uint32_t)PB_MAX_SRC_BITS, PE_ERR_INVALID, "src_bits must be in [4, 15]");
-  PE_REQUIRE(o.dst_bits >= 4 && o.dst_bits <= (uint32_t)PB_MAX_DST_BITS, PE_ERR_INVALID, "dst_bits must be in [4, 13]");
+  PE_REQUIRE(o.dst_bits >= 4 && o.dst_bits <= (uint32_t)PB_MAX_DST_BITS, PE_ERR_INVALID, "dst_bits must be in [4, 14]");
   PE_REQUIRE(o.run_cap >= 64, PE_ERR_INVALID, "run_cap must be at least 64");
   PE_REQUIRE(o.unit_log2 >= 10 && o.unit_log2 <= 30, PE_ERR_INVALID, "unit_log2 must be in [10, 30]");
   if (g->pb && g->pb->src_bits == o.src_bits && g->pb->dst_bits == o.dst_bits && g->pb->run_cap == o.run_cap &&
@@ -589,7 +621,7 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   upload(ctx, L->units2, u2);
   upload(ctx, L->split_bin, split);
   L->values.alloc(ctx, std::max<uint64_t>(1, nupd));
-  const uint64_t nacc = std::max<uint64_t>(1, (uint64_t)L->nsplit << (db + 1));
+  const uint64_t nacc = std::max<uint64_t>(1, (uint64_t)L->nsplit << db);
   L->acc.alloc(ctx, nacc);
   k_pb_zero<<<grid_for(nacc, 256, 65536), 256, 0, s>>>(L->acc.p, nacc);
   PE_HIP(hipGetLastError());
@@ -610,6 +642,8 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
   PE_REQUIRE(g->pb, PE_ERR_STATE, "binned layout not built");
   const pb_layout& L = *g->pb;
   PE_REQUIRE(!weighted || L.weighted, PE_ERR_STATE, "weighted iteration on a layout built without weights");
+  PE_REQUIRE(a.mass > 0.0, PE_ERR_INTERNAL, "binned iteration without a bound on the contributions");
+  const int frac = pb_frac_bits(a.mass);
   hipStream_t s = g->ctx->stream;
   const unsigned g1 = persistent_grid(g->ctx, std::max<uint32_t>(1, L.n1), 1);
   const unsigned g2 = binned_grid_gather(g), gs = binned_grid_split(g);
@@ -623,14 +657,14 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_GATHER);
     if (weighted)
       k_pb_gather<true, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
-                                                  (int)L.dst_bits, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
+                                                  (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                   part_rsum);
     else
       k_pb_gather<false, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
-                                                   (int)L.dst_bits, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
+                                                   (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                    part_rsum);
     PE_HIP(hipGetLastError());
-    k_pb_apply_split<<<gs, 256, 0, s>>>(L.split_bin.p, L.nsplit, (int)L.dst_bits, g->V, L.acc.p, b0, b1, g->od.p,
+    k_pb_apply_split<<<gs, 256, 0, s>>>(L.split_bin.p, L.nsplit, (int)L.dst_bits, frac, g->V, L.acc.p, b0, b1, g->od.p,
                                         a, ctl, part_delta + g2, part_rsum + g2);
     PE_HIP(hipGetLastError());
   }

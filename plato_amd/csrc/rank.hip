@@ -336,8 +336,10 @@ int pe_rank_iterate(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted, const
     PE_REQUIRE(!weighted || g->has_weights, PE_ERR_STATE, "graph has no weights");
     rank_work wk;
     wk.init(g, vec_prec(x));
-    const rank_args a = make_args(p->damping, p->base, p->seed_mass, p->seeds ? p->seeds->p : nullptr,
-                                  p->divide_by_od, p);
+    rank_args a = make_args(p->damping, p->base, p->seed_mass, p->seeds ? p->seeds->p : nullptr,
+                            p->divide_by_od, p);
+    if (vec_prec(x) == 32 && g->pull_layout == PE_PULL_BINNED && g->pb)
+      a.mass = binned_mass_bound(g, (const float*)x->p, weighted != 0);
     rank_iteration(g, vec_prec(x), x->p, y->p, weighted != 0, a, nullptr, wk, wk.scalars.p,
                    wk.scalars.p + 1);
     double h = 0.0;
@@ -356,6 +358,7 @@ int pe_pagerank(pe_graph_t* g, const pe_rank_opts_t* opts, double* out, double* 
     // PageRank: x = 1/od (pagerank.hpp:90-98); weighted: x = 1 (weighted_pagerank.hpp:104-109)
     run.init_state(nullptr, 1.0, w ? 0 : 1);
     run.args = make_args(opts->damping, 1.0 - opts->damping, 0.0, nullptr, w ? 0 : 1, nullptr);
+    run.args.mass = (double)g->V;  // ranks start at 1; the total stays at most V
     run.run(opts->iterations, opts->eps, opts->l1_norm_tol);
     run.download(out, deltas);
     run.fill_stats(stats);
@@ -376,6 +379,7 @@ int pe_ppr(pe_graph_t* g, const uint32_t* seeds, uint32_t nseeds, const pe_rank_
     rank_run run(g, opts->precision ? opts->precision : 32, false);
     run.init_state(ss.words.p, init_dist, 1);
     run.args = make_args(opts->damping, 0.0, (1.0 - opts->damping) * init_dist, ss.words.p, 1, nullptr);
+    run.args.mass = 1.0;  // the restart mass is 1 in total, and so is the starting state
     run.run(opts->iterations, opts->eps, opts->l1_norm_tol);
     run.download(out, deltas);
     run.fill_stats(stats);
@@ -411,6 +415,7 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
       k_init_rank<float><<<grid_for(V, 256, 8192), 256, 0, s>>>(V, g->od.p, ss.words.p, 1.0, 1, (float*)b0.p);
     PE_HIP(hipGetLastError());
     rank_args a = make_args(opts->damping, 0.0, 1.0 - opts->damping, ss.words.p, 1, nullptr);
+    a.mass = 1.0;
     const uint64_t Ep = g->directed ? g->edges : g->edges * 2;
     bool do_pull = false;
     uint32_t executed = 0;

@@ -4,12 +4,18 @@
 //      fixed_f32_acc == bits 32..127 of floor(x * 2^96);
 //   2. 2^22 FP64 products of two random FP32 values (weighted terms): fixed_f64 ==
 //      floor(x * 2^96);
-//   3. sums: 2^20 random terms accumulated as the kernels do (two words, any order)
-//      equal the __int128 sum of the truncated terms, and from_fixed of the words is
-//      within one rounding of that sum.
+//   3. sums: 2^20 random terms accumulated in the two-word form (any order) equal the
+//      __int128 sum of the truncated terms, and from_fixed of the words is within one
+//      rounding of that sum;
+//   4. the single-word form the kernels use: fixed64_f32 == floor(x * 2^F) for every
+//      FP32 value x with x * 2^F < 2^64, at F = 33 (PageRank on 2^27 vertices) and
+//      F = 60 (PPR); fixed64_f64 on 2^22 FP64 products at both;
+//   5. pb_frac_bits(M) puts M * 2^F in [2^60, 2^61), and 2^20 terms whose values add
+//      up to M sum in one 64-bit word exactly as in __int128, below 2^61.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <vector>
 
 #include "pb_fixed.hpp"
 
@@ -76,5 +82,66 @@ int main() {
     if (std::fabs((long double)v - exact) > exact * 0x1p-51L) ++badsum;
   }
   std::printf("sums: %ld mismatches\n", badsum);
-  return (bad || bad64 || badsum) ? 1 : 0;
+  // 4. single-word terms
+  long bad1 = 0;
+  for (int F : {33, 60}) {
+    for (uint32_t e = 0; (int)e - 127 + 1 + F <= 64 && e < 255; ++e) {
+      for (uint32_t f = 0; f < (1u << 23); ++f) {
+        const uint32_t bits = (e << 23) | f;
+        float x;
+        __builtin_memcpy(&x, &bits, 4);
+        const uint64_t m = e ? (uint64_t)(f | (1u << 23)) : (uint64_t)f;
+        const u128i R = ref_fixed(m, (e ? (int)e : 1) - 150 + F);
+        if (pe::fixed64_f32(x, F) != (uint64_t)R || (R >> 64) != 0) {
+          if (bad1 < 5) std::printf("fixed64 mismatch F=%d bits=%08x\n", F, bits);
+          ++bad1;
+        }
+      }
+    }
+    for (int i = 0; i < (1 << 22); ++i) {
+      const float w = U(rng), xv = std::ldexp(U(rng), ex(rng) - (F > 40 ? 22 : 0));
+      const double p = (double)w * (double)xv;
+      uint64_t b;
+      __builtin_memcpy(&b, &p, 8);
+      const int e = (int)((b >> 52) & 0x7FF);
+      const uint64_t m = e ? ((b & ((1ull << 52) - 1)) | (1ull << 52)) : (b & ((1ull << 52) - 1));
+      const u128i R = ref_fixed(m, (e ? e : 1) - 1075 + F);
+      if (pe::fixed64_f64(p, F) != (uint64_t)R) ++bad1;
+    }
+  }
+  std::printf("single-word terms: %ld mismatches\n", bad1);
+  // 5. scale choice and single-word sums
+  long badf = 0;
+  for (int i = -60; i <= 40; ++i) {
+    for (double frac : {1.0, 1.37, 1.999}) {
+      const double M = std::ldexp(frac, i);
+      const int F = pe::pb_frac_bits(M);
+      const long double s = std::ldexp((long double)M, F);
+      if (!(s >= 0x1p60L && s < 0x1p61L)) ++badf;
+    }
+  }
+  for (double M : {1.0, 134217728.0}) {
+    const int F = pe::pb_frac_bits(M);
+    const int n = 1 << 20;
+    uint64_t acc = 0;
+    u128i S = 0;
+    double total = 0.0;
+    std::vector<float> xs(n);
+    for (int i = 0; i < n; ++i) {
+      xs[i] = (float)(U(rng) * 2.0 * M / n);
+      total += xs[i];
+    }
+    const float scale = (float)(M / total);  // values add up to about M
+    for (int i = 0; i < n; ++i) {
+      const float xv = xs[i] * scale;
+      const uint64_t t = pe::fixed64_f32(xv, F);
+      acc += t;
+      S += ref_fixed(xv == 0.0f ? 0 : ((pe::pb_bits32(xv) & 0x7FFFFFu) | 0x800000u),
+                     (int)((pe::pb_bits32(xv) >> 23) & 0xFF) - 150 + F);
+    }
+    if ((u128i)acc != S || S >= ((u128i)1 << 61)) ++badf;
+    if (std::fabs(pe::from_fixed64(acc, F) - (double)M) > 1e-6 * M) ++badf;
+  }
+  std::printf("scale and single-word sums: %ld mismatches\n", badf);
+  return (bad || bad64 || badsum || bad1 || badf) ? 1 : 0;
 }

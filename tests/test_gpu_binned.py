@@ -1,7 +1,7 @@
 """GPU: the binned (partition-centric) pull layout of the FP32 rank iterations.
 
 The binned gather sums every column exactly in fixed point (each term truncated at
-2^-64, then integer sums, independent of order), so
+2^-F with F set by the rank mass, then 64-bit integer sums, independent of order), so
   * against the FP64 oracle it meets the FP32 tolerances of the merge-path
     gather: normalised L1 <= 2e-7, relative L-infinity <= 2e-6 after 20 PageRank
     iterations (weighted: L1 <= 5e-7; PPR: absolute L1 <= 1e-6);
@@ -79,7 +79,7 @@ def test_binned_layout_info_and_release(s14):
 def test_binned_rejects_bad_options(s14):
     from plato_amd._lib import PlatoHipError
     g, _ = s14
-    for bad in (dict(src_bits=16), dict(dst_bits=14), dict(src_bits=3), dict(run_cap=32),
+    for bad in (dict(src_bits=16), dict(dst_bits=15), dict(src_bits=3), dict(run_cap=32),
                 dict(unit_log2=31)):
         with pytest.raises(PlatoHipError):
             g.set_pull_layout("binned", **bad)
