@@ -27,7 +27,9 @@
 namespace pe {
 
 static constexpr uint32_t UNVIS = 0xFFFFFFFFu;
-static constexpr int PULL_CAP = 32;
+#ifndef PULL_CAP
+#define PULL_CAP 32  // in-edges a pull lane scans before its vertex goes to the wave-per-vertex pass
+#endif
 
 struct bfs_counters {
   unsigned long long next_count;
