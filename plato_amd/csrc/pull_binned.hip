@@ -36,7 +36,7 @@
 // the gather find each entry's update.  A run is the updates of one (bin, chunk)
 // pair.  The scatter walks run records (runs cut to at most run_cap updates) grouped
 // by chunk; the gather walks bins, split into units of at most 2^unit_log2 entries
-// (units of a split bin meet in global 128-bit accumulators, applied by a last pass).
+// (units of a split bin meet in global 64-bit accumulators, applied by a last pass).
 #include <algorithm>
 #include <numeric>
 #include <vector>
