@@ -178,6 +178,34 @@ def run_bfs(ctx, scale, roots, seed=11):
     return out
 
 
+def run_kcore(ctx, scale, cpu_scale=20, seed=11):
+    """k-core (kcore_algo_t::compute_shell_index) on the symmetrized RMAT graph of the BFS
+    leg; the oracle's sequential restatement is timed on a smaller graph of the same
+    family beside the device run on that graph."""
+    import oracle  # CPU baseline (and the check of the device result on its graph)
+    from plato_amd import Graph
+    from plato_amd.algo import kcore
+    g = Graph.rmat(ctx, scale, 16, seed=seed, directed=False, relabel=True)
+    kcore(g)  # warm-up
+    res = kcore(g)
+    out = {"scale": scale, "nnz": int(g.nnz), "epochs": res.epochs, "seconds": round(res.seconds, 4),
+           "ms_per_epoch": round(res.seconds * 1e3 / max(res.epochs, 1), 3),
+           "max_coreness": int(res.coreness.max()), "modified_total": int(res.modified.sum())}
+    g.close()
+    s, d = oracle.rmat_edges(cpu_scale, 16, seed)
+    og = oracle.CSC(1 << cpu_scale, s, d, symmetrize=True)
+    t0 = time.perf_counter()
+    core, _, _ = oracle.kcore(og)
+    cpu_s = time.perf_counter() - t0
+    gs = Graph.rmat(ctx, cpu_scale, 16, seed=seed, directed=False)
+    rs = kcore(gs)
+    gs.close()
+    assert np.array_equal(rs.coreness, core)
+    out["cpu_baseline"] = {"scale": cpu_scale, "seconds": round(cpu_s, 3), "cores": 1, "kind": "port",
+                           "gpu_seconds_same_graph": round(rs.seconds, 4)}
+    return out
+
+
 def csv_text(src, dst):
     """`src,dst\n` lines with 10-digit zero-padded ids (22 bytes per edge), built with numpy."""
     E = src.size
@@ -341,6 +369,7 @@ def main():
     ap.add_argument("--bfs-scale", type=int, default=24)
     ap.add_argument("--bfs-roots", type=int, default=64)
     ap.add_argument("--no-bfs", action="store_true")
+    ap.add_argument("--no-kcore", action="store_true")
     ap.add_argument("--ppr-scale", type=int, default=24)
     ap.add_argument("--no-ppr", action="store_true")
     ap.add_argument("--ppr-block-only", action="store_true",
@@ -448,6 +477,9 @@ def main():
     if not args.no_bfs:
         log(f"[bench] bfs scale {args.bfs_scale}, {args.bfs_roots} roots")
         out["bfs"] = run_bfs(ctx, args.bfs_scale, args.bfs_roots)
+    if not args.no_kcore:
+        log(f"[bench] k-core scale {args.bfs_scale}")
+        out["kcore"] = run_kcore(ctx, args.bfs_scale)
     if not args.no_ppr:
         log(f"[bench] batched PPR scale {args.ppr_scale}, k = 64")
         out["ppr_batched"] = run_batched_ppr(ctx, args.ppr_scale, single_runs=not args.ppr_block_only)

@@ -313,6 +313,23 @@ int pe_trustrank_select(pe_graph_t* g, pe_graph_t* g_rev, const uint64_t* good_w
 /* The whole algorithm; out: host FP64 [V]; seeds_words optional. */
 int pe_trustrank(pe_graph_t* g, pe_graph_t* g_rev, const uint64_t* good_words, const pe_trustrank_opts_t* opts,
                  double* out, uint64_t* seeds_words, pe_rank_stats_t* stats);
+
+/* ---- k-core --------------------------------------------------------------- */
+
+typedef struct {
+  uint32_t epochs;    /* epochs run (the last one modifies nothing) */
+  uint64_t modified;  /* coreness decrements over all epochs */
+  double seconds;     /* device time of the whole loop, HIP events */
+} pe_kcore_stats_t;
+
+/* Replaces plato::algo::kcore_algo_t::compute_shell_index (plato/algo/kcore/kcore.hpp:80-268)
+ * over the graph's in-edges (the DCSC "incomings"), or its out-edges when out_edges is
+ * set (a BCSR passed as "incomings", as example/kcore_simple.cc does; needs
+ * PE_GRAPH_BUILD_CSR).  Writes coreness[V] in input ids (0xFFFFFFFF never remains: vertices
+ * without neighbours get 0), and per epoch e < cap the number of modified vertices and
+ * the size of the next active set (the reference's modified / need_modified log). */
+int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64_t* epoch_modified,
+             uint64_t* epoch_next, uint32_t cap, pe_kcore_stats_t* stats);
 /* A new graph with every edge of the directed graph g reversed (same input ids), for
  * pulls over out-edges such as inverse PageRank (the reference runs pagerank over its
  * bcsr_t out-edge structure, trustrank.hpp:101-103).  flags: PE_GRAPH_BUILD_CSR,

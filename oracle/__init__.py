@@ -67,6 +67,8 @@ def lib():
     L.orc_bfs.argtypes = [C.c_uint32, _u64p, _u32p, _u64p, _u32p, _u32p, C.c_uint64,
                           C.c_double, C.c_uint32, _i32p, _u8p, _u64p, _u64p, C.c_uint32,
                           C.POINTER(C.c_uint32)]
+    L.orc_kcore.argtypes = [C.c_uint32, _u64p, _u32p, _u32p, _u64p, _u64p, C.c_uint32]
+    L.orc_kcore.restype = C.c_int
     L.orc_parse_csv.argtypes = [C.c_char_p, C.c_uint64, C.c_int, C.c_uint64, _u32p, _u32p, C.c_void_p]
     L.orc_parse_csv.restype = C.c_int64
     L.orc_format_csv.argtypes = [C.c_uint64, _u32p, _u32p, C.c_void_p, C.c_uint64]
@@ -221,6 +223,17 @@ def bfs(g: CSC, root, threshold=0.05, max_epochs=4096):
     k = min(n, max_epochs)
     return dict(levels=levels, modes=modes[:k], frontier=frontier[:k], active_edges=active[:k],
                 epochs=n, visited=int(visited.value))
+
+
+def kcore(g: CSC, out_edges=False, max_epochs=4096):
+    """kcore_algo_t::compute_shell_index over the in-edges (CSC), or the out-edges (CSR)."""
+    ptr, idx = (g.row_ptr, g.col_idx) if out_edges else (g.col_ptr, g.row_idx)
+    core = np.zeros(g.V, np.uint32)
+    mod = np.zeros(max_epochs, np.uint64)
+    nxt = np.zeros(max_epochs, np.uint64)
+    n = lib().orc_kcore(g.V, ptr, np.ascontiguousarray(idx), core, mod, nxt, max_epochs)
+    k = min(n, max_epochs)
+    return core, mod[:k], nxt[:k]
 
 
 def topk(r, k):

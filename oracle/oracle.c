@@ -443,6 +443,73 @@ void orc_topk(uint32_t V, const double* r, uint32_t k, uint32_t* ids, double* va
   free(p);
 }
 
+/* ---- k-core (kcore.hpp:80-268) ------------------------------------------------ */
+int orc_kcore(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t* coreness,
+              uint64_t* epoch_modified, uint64_t* epoch_next, uint32_t cap) {
+  uint32_t* est = (uint32_t*)malloc(sizeof(uint32_t) * (V ? V : 1));
+  uint8_t* act = (uint8_t*)malloc(V ? V : 1);
+  uint8_t* nxt = (uint8_t*)malloc(V ? V : 1);
+  uint64_t maxlen = 0;
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint64_t len = ptr[v + 1] - ptr[v];
+    coreness[v] = len ? (uint32_t)len : 0xFFFFFFFFu;  /* kcore.hpp:128-136 */
+    if (len > maxlen) maxlen = len;
+    act[v] = 1;                                        /* active_current.fill() */
+  }
+  uint32_t* count = (uint32_t*)malloc(sizeof(uint32_t) * (maxlen + 1));
+  int epoch = 0;
+  uint64_t need = 0;
+  do {
+    /* broadcast: estimate = coreness (kcore.hpp:167-189; on one partition both modes
+       leave every estimate equal to its coreness) */
+    memcpy(est, coreness, sizeof(uint32_t) * V);
+    memset(nxt, 0, V);
+    uint64_t modified = 0;
+    for (uint32_t v = 0; v < V; ++v) {                 /* kcore.hpp:204-244 */
+      if (!act[v]) continue;
+      const uint64_t b = ptr[v], e = ptr[v + 1];
+      const uint32_t len = (uint32_t)(e - b);
+      if (len == 0) {
+        coreness[v] = 0;
+        continue;
+      }
+      const uint32_t cap_i = len < est[v] ? len : est[v];
+      memset(count, 0, sizeof(uint32_t) * (cap_i + 1));
+      for (uint64_t k = b; k < e; ++k) {
+        const uint32_t eu = est[idx[k]];
+        ++count[eu < cap_i ? eu : cap_i];
+      }
+      uint32_t sum = 0;
+      for (uint32_t i = cap_i; i > 0; --i) {
+        sum += count[i];
+        if (sum >= i) {
+          if (i < coreness[v]) {
+            coreness[v] = i;
+            for (uint64_t k = b; k < e; ++k) nxt[idx[k]] = 1;
+            ++modified;
+          }
+          break;
+        }
+      }
+    }
+    need = 0;
+    for (uint32_t v = 0; v < V; ++v) need += nxt[v];
+    if ((uint32_t)epoch < cap) {
+      epoch_modified[epoch] = modified;
+      epoch_next[epoch] = need;
+    }
+    uint8_t* t = act;
+    act = nxt;
+    nxt = t;
+    ++epoch;
+  } while (need);                                      /* kcore.hpp:264 */
+  free(count);
+  free(nxt);
+  free(act);
+  free(est);
+  return epoch;
+}
+
 /* ---- edge-list ingestion (parsers.hpp:225-293) ------------------------------ */
 int64_t orc_parse_csv(const char* bytes, uint64_t n, int weighted, uint64_t cap, uint32_t* src,
                       uint32_t* dst, float* w) {

@@ -117,6 +117,25 @@ int orc_bfs(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
             int32_t* levels, uint8_t* modes, uint64_t* frontier, uint64_t* active_edges,
             uint32_t max_epochs, uint32_t* visited);
 
+/* ---- k-core ----------------------------------------------------------------- */
+
+/* plato::algo::kcore_algo_t::compute_shell_index (kcore.hpp:80-268) on one partition,
+ * over the adjacency the caller passes as "incomings" (ptr[V+1], idx): a DCSC gives
+ * in-neighbours, a BCSR (as example/kcore_simple.cc passes) out-neighbours.
+ * coreness starts at the adjacency length, or 0xFFFFFFFF for vertices without one;
+ * every vertex is active in epoch 0.  Per epoch (Jacobi: every active vertex reads the
+ * estimates of the previous epoch): an active vertex with no neighbours gets 0; else
+ * est = min(len, estimate[v]), count[min(est, estimate[u])] over the neighbours u, and
+ * the largest i in [1, est] whose suffix count reaches i replaces coreness[v] when it is
+ * smaller, which marks v modified and activates its neighbours for the next epoch
+ * (when no such i exists coreness[v] is left as it is, as the reference's loop does).
+ * Stops after the epoch that activates nothing.  Returns the number of epochs and, per
+ * epoch e < cap, the modified count and the size of the next active set.
+ * Pinned: networkx core_number on simple undirected graphs; hand-worked directed
+ * cases (including a vertex left unchanged because no i qualifies). */
+int orc_kcore(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t* coreness,
+              uint64_t* epoch_modified, uint64_t* epoch_next, uint32_t cap);
+
 /* ---- edge-list ingestion ---------------------------------------------------- */
 
 /* Plato's csv_parser (parsers.hpp:225-293) over a byte buffer, with the same C

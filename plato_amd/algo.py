@@ -200,3 +200,25 @@ def trustrank(graph: Graph, opts: trustrank_opts_t = trustrank_opts_t(), good=No
                                  out.ctypes.data, words.ctypes.data, C.byref(st)), graph.ctx.h)
     res = RankResult(out, np.zeros(0), st.iterations_run, st.seconds, st.gteps)
     return res, _words_to_ids(words, graph.V)
+
+
+@dataclass
+class KcoreResult:
+    coreness: np.ndarray      # uint32 per vertex (input ids)
+    modified: np.ndarray      # per epoch: vertices whose coreness dropped
+    next_active: np.ndarray   # per epoch: size of the next active set
+    epochs: int
+    seconds: float
+
+
+def kcore(graph: Graph, out_edges: bool = False, max_epochs: int = 1 << 16) -> KcoreResult:
+    """kcore_algo_t::compute_shell_index (kcore.hpp:80-268) over the in-edges, or the
+    out-edges (a BCSR passed as the reference's "incomings")."""
+    core = np.zeros(graph.V, np.uint32)
+    mod = np.zeros(max_epochs, np.uint64)
+    nxt = np.zeros(max_epochs, np.uint64)
+    st = L.KcoreStats()
+    check(graph.lib.pe_kcore(graph.h, 1 if out_edges else 0, core.ctypes.data, mod.ctypes.data,
+                             nxt.ctypes.data, max_epochs, C.byref(st)), graph.ctx.h)
+    k = min(st.epochs, max_epochs)
+    return KcoreResult(core, mod[:k], nxt[:k], st.epochs, st.seconds)

@@ -311,6 +311,14 @@ __global__ __launch_bounds__(256) void k_frontier_compact(const uint64_t* __rest
   }
 }
 
+// Compacts the set bits of a bitmap (ids below V) into queue[*count ...]; *count
+// must be zero or hold the entries already queued.
+void frontier_compact(pe_ctx_s* ctx, const uint64_t* words, uint64_t nwords, uint32_t V, uint32_t* queue,
+                      unsigned long long* count) {
+  k_frontier_compact<<<grid_for(nwords, 256, 4096), 256, 0, ctx->stream>>>(words, nwords, V, queue, count);
+  PE_HIP(hipGetLastError());
+}
+
 __global__ void k_reached_edges(const uint32_t* __restrict__ level, const uint32_t* __restrict__ od,
                                 uint32_t V, unsigned long long* __restrict__ out,
                                 unsigned int* __restrict__ visited) {

@@ -71,10 +71,15 @@ class graph_facade {
   }
   const pe_graph_info_t& info() const { return g_->info; }
   std::shared_ptr<device_graph> shared() const { return g_; }
+  // true when the structure's adjacency lists are out-edges (a BCSR built with
+  // is_outgoing), false for in-edges (a DCSC, or a BCSR built over incoming edges)
+  bool outgoing() const { return outgoing_; }
+  void set_outgoing(bool o) { outgoing_ = o; }
 
  private:
   std::shared_ptr<device_graph> g_;
   std::shared_ptr<partition_t> part_;
+  bool outgoing_ = false;
 };
 }  // namespace detail
 
@@ -135,7 +140,24 @@ std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqd_from_path(graph_info_t* info, co
                                                           DECODER = nullptr) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
   auto g = detail::build(info, path, weighted, true, false);
-  return std::make_shared<bcsr_t<EDATA>>(g);
+  auto b = std::make_shared<bcsr_t<EDATA>>(g);
+  b->set_outgoing(true);
+  return b;
+}
+
+// create_bcsr_seqs_from_path (structure.hpp:727-739): is_outgoing = false keeps the
+// incoming edges in the structure (as example/kcore_simple.cc asks for).
+template <typename EDATA, typename DECODER = void*, typename ENCODER = std::nullptr_t>
+std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqs_from_path(graph_info_t* info, const std::string& path,
+                                                          edge_format_t = edge_format_t::CSV,
+                                                          DECODER = nullptr, int /*alpha*/ = -1,
+                                                          bool /*use_in_degree*/ = false,
+                                                          ENCODER = nullptr, bool is_outgoing = true) {
+  const bool weighted = !std::is_same<EDATA, empty_t>::value;
+  auto g = detail::build(info, path, weighted, is_outgoing, false);
+  auto b = std::make_shared<bcsr_t<EDATA>>(g);
+  b->set_outgoing(is_outgoing);
+  return b;
 }
 
 template <typename EDATA, typename DECODER = void*>
@@ -144,7 +166,9 @@ std::pair<std::shared_ptr<bcsr_t<EDATA>>, std::shared_ptr<dcsc_t<EDATA>>> create
     bool relabel = false) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
   auto g = detail::build(info, path, weighted, true, relabel);
-  return {std::make_shared<bcsr_t<EDATA>>(g), std::make_shared<dcsc_t<EDATA>>(g)};
+  auto out = std::make_shared<bcsr_t<EDATA>>(g);
+  out->set_outgoing(true);
+  return {out, std::make_shared<dcsc_t<EDATA>>(g)};
 }
 
 // out-degrees counted from the in-edges (structure.hpp:292-346)

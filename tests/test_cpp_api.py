@@ -16,7 +16,7 @@ def test_host_programs_build():
     r = subprocess.run(["make", "-s", "-f", "plato_amd/host/Makefile", f"ROOT={ROOT}"], cwd=ROOT,
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "test_plato_api"):
+    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "test_plato_api"):
         assert os.access(os.path.join(BIN, p), os.X_OK)
 
 
@@ -111,3 +111,32 @@ def test_trustrank_driver_matches_oracle(tmp_path):
     ref, _ = oracle.ppr(g, seeds, 30, 0.85, 0.0)
     assert np.array_equal(got[:, 0], np.arange(V))
     np.testing.assert_allclose(got[:, 1], ref, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.gpu
+def test_kcore_driver_matches_oracle(tmp_path):
+    """kcore_simple reads its input as a BCSR over incoming edges, like the reference
+    driver: vertex mode equals the oracle's coreness over the in-edges, and subgraph mode
+    writes the in-adjacency of the vertices of each coreness under <k>_core/."""
+    path = os.path.join(GOLDEN, "v100_e4300_da_c3.csv")
+    s, d = oracle.read_edge_csv(path)
+    g = oracle.CSC(100, s, d)
+    core, _, _ = oracle.kcore(g)
+    vdir = tmp_path / "v"
+    vdir.mkdir()
+    r = subprocess.run([os.path.join(BIN, "kcore_simple"), "--input", path, "--is_directed=true",
+                        "--type", "vertex", "--output", str(vdir)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(vdir / "0000_0.csv", delimiter=",", dtype=np.int64)
+    assert np.array_equal(got[:, 0], np.arange(100)) and np.array_equal(got[:, 1], core)
+    sdir = tmp_path / "s"
+    sdir.mkdir()
+    r = subprocess.run([os.path.join(BIN, "kcore_simple"), "--input", path, "--is_directed=true",
+                        "--output", str(sdir)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for k in range(int(core.max()) + 1):
+        f = sdir / f"{k}_core" / "0000_0.csv"
+        lines = f.read_text().split()
+        exp = [f"{v},{u}" for v in range(100) if core[v] == k
+               for u in g.row_idx[int(g.col_ptr[v]):int(g.col_ptr[v + 1])]]
+        assert lines == exp

@@ -299,3 +299,54 @@ def test_format_csv_round_trip():
     assert np.array_equal(s, s2) and np.array_equal(d, d2)
     assert oracle.format_csv(np.array([0, 4294967295], np.uint32), np.array([10, 7], np.uint32)) == \
         b"0,10\n4294967295,7\n"
+
+
+# ---- k-core (kcore.hpp:80-268) ---------------------------------------------------------------------
+def _simple_undirected(s, d):
+    """Distinct undirected pairs without self-loops, as src < dst arrays."""
+    a = np.minimum(s, d).astype(np.uint64)
+    b = np.maximum(s, d).astype(np.uint64)
+    keep = a != b
+    k = np.unique((a[keep] << np.uint64(32)) | b[keep])
+    return (k >> np.uint64(32)).astype(np.uint32), (k & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def test_kcore_matches_networkx_core_number():
+    """On simple undirected graphs the reference's h-index iteration converges to the
+    textbook coreness (networkx core_number, Batagelj-Zaversnik peeling)."""
+    import networkx as nx
+    cases = [oracle.rmat_edges(10, 8, 4), oracle.rmat_edges(12, 16, 5),
+             oracle.read_edge_csv(golden("v100_e2150_ua_c3.csv"))]
+    for s, d in cases:
+        a, b = _simple_undirected(s, d)
+        V = int(max(s.max(), d.max())) + 1
+        core, mod, nxt = oracle.kcore(oracle.CSC(V, a, b, symmetrize=True))
+        G = nx.Graph()
+        G.add_nodes_from(range(V))
+        G.add_edges_from(zip(a.tolist(), b.tolist()))
+        ref = nx.core_number(G)
+        assert np.array_equal(core, np.array([ref[v] for v in range(V)], np.uint32))
+        assert nxt[-1] == 0 and np.all(nxt[:-1] > 0) and mod[-1] == 0
+
+
+def test_kcore_hand_worked_cases():
+    # triangle 0-1-2 with a pendant 3 on vertex 2: cores 2, 2, 2, 1
+    s = np.array([0, 1, 2, 2], np.uint32)
+    d = np.array([1, 2, 0, 3], np.uint32)
+    core, mod, nxt = oracle.kcore(oracle.CSC(4, s, d, symmetrize=True))
+    assert core.tolist() == [2, 2, 2, 1]
+    # directed, in-edges: 0->2, 1->2, 2->3, 4->3, 5->4.  Epoch 0: 0, 1 and 5 have no
+    # in-edges (-> 0); vertex 3 (in-neighbours 2 with estimate 2, 4 with estimate 1)
+    # drops to 1 and activates 2 and 4; 2 and 4 keep 2 and 1 (their in-neighbours start
+    # at 0xFFFFFFFF).  Epoch 1: the in-neighbours of 2 and 4 now have estimate 0, so no
+    # i >= 1 qualifies and both keep their value; nothing is modified, the loop ends.
+    s = np.array([0, 1, 2, 4, 5], np.uint32)
+    d = np.array([2, 2, 3, 3, 4], np.uint32)
+    g = oracle.CSC(6, s, d)
+    core, mod, nxt = oracle.kcore(g)
+    assert core.tolist() == [0, 0, 2, 1, 1, 0]
+    assert mod.tolist() == [1, 0] and nxt.tolist() == [2, 0]
+    # the same edges read as out-adjacency (a BCSR "incomings", as kcore_simple passes):
+    # 3 has none (-> 0); 2's only out-neighbour 3 starts at 0xFFFFFFFF, so 2 keeps 1.
+    core, mod, nxt = oracle.kcore(g, out_edges=True)
+    assert core.tolist() == [1, 1, 1, 0, 1, 1] and mod.tolist() == [0] and nxt.tolist() == [0]
