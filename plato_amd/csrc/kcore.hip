@@ -16,7 +16,9 @@
 // most KC_SMALL neighbours counts them in a byte histogram of its own in LDS; longer
 // adjacencies go to a second queue handled one wave per vertex with a KC_HIST-bin LDS
 // histogram (values at or above KC_HIST share the last bin; when the answer reaches
-// it, a binary search over [KC_HIST, est] with wave-wide counts finishes the job).
+// it, a binary search over [KC_HIST, est] with wave-wide counts finishes the job), and
+// adjacencies above KC_HUGE to a third, one 1024-thread block per vertex with a
+// KC_HHIST-bin histogram.
 // Modified vertices set their neighbours' bits in the next bitmap (test, then atomic
 // OR, as the reference's get_bit / set_bit).  Then estimate = coreness for the queue.
 #include <algorithm>
@@ -31,10 +33,15 @@ static constexpr uint32_t KC_SMALL = 64;    // neighbours handled by one lane
 static constexpr uint32_t KC_HIST = 1024;   // histogram bins 0..KC_HIST of the wave-per-vertex pass
 static constexpr uint32_t KC_PER = 17;      // bins per lane in its scan (64 * 17 > KC_HIST)
 static constexpr int KC_THREADS = 256;
+static constexpr uint32_t KC_HUGE = 4096;    // longer adjacencies take one block each
+static constexpr int KC_HTHREADS = 1024;
+static constexpr uint32_t KC_HHIST = 16384;  // histogram bins 0..KC_HHIST of the block pass
+static constexpr uint32_t KC_HPER = 17;      // bins per thread in its scan (1024 * 17 > KC_HHIST)
 
 struct kc_counters {
   unsigned long long queued;    // active vertices this epoch
-  unsigned long long nbig;      // of them, with more than KC_SMALL neighbours
+  unsigned long long nbig;      // of them, with KC_SMALL < neighbours <= KC_HUGE
+  unsigned long long nhuge;     // of them, with more than KC_HUGE neighbours
   unsigned long long modified;  // vertices whose coreness dropped
 };
 
@@ -66,6 +73,7 @@ __global__ void k_kc_zero(uint64_t* __restrict__ w, uint64_t n) {
 __global__ void k_kc_reset(kc_counters* c) {
   c->queued = 0;
   c->nbig = 0;
+  c->nhuge = 0;
   c->modified = 0;
 }
 
@@ -80,15 +88,27 @@ __device__ __forceinline__ void kc_count_modified(bool mod, kc_counters* cnt) {
   if ((threadIdx.x & 63) == 0 && b) atomicAdd(&cnt->modified, (unsigned long long)__popcll(b));
 }
 
+// wave-aggregated append of the lanes with `take` to q[*n ...]
+__device__ __forceinline__ void kc_append(bool take, uint32_t v, uint32_t* q, unsigned long long* n) {
+  const uint64_t m = __ballot(take);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(n, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (take) q[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
 // One lane per queued vertex of at most KC_SMALL neighbours (byte counts suffice).
 __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
     const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
     const uint32_t* __restrict__ idx, const uint32_t* __restrict__ est, uint32_t* __restrict__ core,
-    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ big, kc_counters* __restrict__ cnt) {
+    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ big, uint32_t* __restrict__ huge,
+    kc_counters* __restrict__ cnt) {
   __shared__ uint8_t hist[KC_THREADS][KC_SMALL + 1];
   uint8_t* h = hist[threadIdx.x];
   const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  bool mod = false, is_big = false;
+  bool mod = false, is_big = false, is_huge = false;
   uint32_t v = 0;
   if (i < n) {
     v = queue[i];
@@ -96,6 +116,8 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
     const uint64_t len = e - b;
     if (len == 0) {
       core[v] = 0;  // kcore.hpp:214-218
+    } else if (len > KC_HUGE) {
+      is_huge = true;
     } else if (len > KC_SMALL) {
       is_big = true;
     } else {
@@ -120,14 +142,8 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
       }
     }
   }
-  const uint64_t bm = __ballot(is_big);
-  if (bm) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(&cnt->nbig, (unsigned long long)__popcll(bm));
-    base = __shfl(base, 0, 64);
-    if (is_big) big[base + __popcll(bm & ((1ull << lane) - 1ull))] = v;
-  }
+  kc_append(is_big, v, big, &cnt->nbig);
+  kc_append(is_huge, v, huge, &cnt->nhuge);
   kc_count_modified(mod, cnt);
 }
 
@@ -198,6 +214,87 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_big(
   if (lane == 0 && nmod) atomicAdd(&cnt->modified, (unsigned long long)nmod);
 }
 
+// One block per queued vertex with more than KC_HUGE neighbours (the hubs, whose
+// adjacency would otherwise keep one wave busy for the whole epoch).
+__global__ __launch_bounds__(KC_HTHREADS) void k_kc_huge(
+    const uint32_t* __restrict__ huge, const kc_counters* __restrict__ cnt_in, const uint64_t* __restrict__ ptr,
+    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ est, uint32_t* __restrict__ core,
+    uint64_t* __restrict__ next_bits, kc_counters* __restrict__ cnt) {
+  __shared__ uint32_t hist[KC_HTHREADS * KC_HPER];
+  __shared__ uint32_t s_wave[KC_HTHREADS / 64];
+  __shared__ uint32_t s_best;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int WAVES = KC_HTHREADS / 64;
+  const uint64_t nhuge = cnt_in->nhuge;
+  uint32_t nmod = 0;
+  for (uint64_t q = blockIdx.x; q < nhuge; q += gridDim.x) {
+    const uint32_t v = huge[q];
+    const uint64_t b = ptr[v], e = ptr[v + 1];
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(e - b, (uint64_t)est[v]);
+    const uint32_t top = std::min(cap, KC_HHIST);
+    for (uint32_t j = tid; j <= top; j += KC_HTHREADS) hist[j] = 0;
+    __syncthreads();
+    for (uint64_t k = b + tid; k < e; k += KC_HTHREADS) {
+      const uint32_t eu = est[idx[k]];
+      atomicAdd(&hist[eu < top ? eu : top], 1u);
+    }
+    __syncthreads();
+    // thread t owns bins [17 t, 17 t + 17); suffix sums over the threads above it
+    const uint32_t j0 = (uint32_t)tid * KC_HPER;
+    uint32_t own = 0;
+    for (uint32_t j = j0; j < j0 + KC_HPER && j <= top; ++j) own += hist[j];
+    uint32_t incl = own;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_down(incl, off, 64);
+      if (lane + off < 64) incl += t;
+    }
+    if (lane == 0) s_wave[wave] = incl;  // the wave's total
+    if (tid == 0) s_best = 0;
+    __syncthreads();
+    uint32_t sum = incl - own;
+    for (int w = wave + 1; w < WAVES; ++w) sum += s_wave[w];
+    uint32_t best = 0;
+    for (int j = (int)std::min(j0 + KC_HPER - 1, top); j >= (int)j0 && j >= 1; --j) {
+      sum += hist[j];
+      if (sum >= (uint32_t)j) {
+        best = (uint32_t)j;
+        break;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) best = std::max(best, (uint32_t)__shfl_xor(best, off, 64));
+    if (lane == 0 && best) atomicMax(&s_best, best);
+    __syncthreads();
+    best = s_best;
+    if (best == KC_HHIST && cap > KC_HHIST) {
+      // binary search on [KC_HHIST, cap] with block-wide counts
+      uint32_t lo = KC_HHIST, hi = cap;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        uint32_t c = 0;
+        for (uint64_t k = b + tid; k < e; k += KC_HTHREADS) c += est[idx[k]] >= mid ? 1u : 0u;
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        __syncthreads();
+        if (tid == 0) s_best = 0;
+        __syncthreads();
+        if (lane == 0 && c) atomicAdd(&s_best, c);
+        __syncthreads();
+        if (s_best >= mid) lo = mid;
+        else hi = mid - 1;
+      }
+      best = lo;
+    }
+    const bool mod = best && best < core[v];
+    __syncthreads();  // every thread has read core[v]
+    if (mod) {
+      if (tid == 0) core[v] = best;
+      for (uint64_t k = b + tid; k < e; k += KC_HTHREADS) kc_activate(next_bits, idx[k]);
+      ++nmod;
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && nmod) atomicAdd(&cnt->modified, (unsigned long long)nmod);
+}
+
 // estimate = coreness for the epoch's active vertices (kcore.hpp:248-258)
 __global__ void k_kc_sync(const uint32_t* __restrict__ queue, uint64_t n, const uint32_t* __restrict__ core,
                           uint32_t* __restrict__ est) {
@@ -223,7 +320,7 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
     const uint32_t* idx = out_edges ? g->csr_idx() : g->row_idx.p;
     const uint64_t nwords = ((uint64_t)V >> 6) + 1;
     const uint32_t Vs = std::max<uint32_t>(1, V);
-    dbuf<uint32_t> core(ctx, Vs), est(ctx, Vs), queue(ctx, Vs), big(ctx, Vs);
+    dbuf<uint32_t> core(ctx, Vs), est(ctx, Vs), queue(ctx, Vs), big(ctx, Vs), huge(ctx, Vs);
     dbuf<uint64_t> cur(ctx, nwords), nxt(ctx, nwords);
     dbuf<kc_counters> cnt(ctx, 1);
     kc_counters* h = static_cast<kc_counters*>(ctx->host_pinned);
@@ -232,6 +329,7 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
     k_kc_fill<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(cur.p, nwords, V);  // active_current.fill()
     PE_HIP(hipGetLastError());
     const unsigned big_grid = persistent_grid(ctx, 1u << 20, 4);
+    const unsigned huge_grid = persistent_grid(ctx, 1u << 20, 1);
     uint32_t epoch = 0;
     uint64_t total_mod = 0;
     for (;;) {
@@ -247,8 +345,9 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
       k_kc_zero<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(nxt.p, nwords);
       if (n) {
         k_kc_small<<<grid_for(n, KC_THREADS), KC_THREADS, 0, s>>>(queue.p, n, ptr, idx, est.p, core.p, nxt.p, big.p,
-                                                                  cnt.p);
+                                                                  huge.p, cnt.p);
         k_kc_big<<<big_grid, KC_THREADS, 0, s>>>(big.p, cnt.p, ptr, idx, est.p, core.p, nxt.p, cnt.p);
+        k_kc_huge<<<huge_grid, KC_HTHREADS, 0, s>>>(huge.p, cnt.p, ptr, idx, est.p, core.p, nxt.p, cnt.p);
         k_kc_sync<<<grid_for(n, 256, 8192), 256, 0, s>>>(queue.p, n, core.p, est.p);
       }
       PE_HIP(hipGetLastError());
