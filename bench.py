@@ -206,6 +206,34 @@ def run_kcore(ctx, scale, cpu_scale=20, seed=11):
     return out
 
 
+def run_cc(ctx, scale, cpu_scale=22, seed=11):
+    """Connected components (connected_component_t::compute) on the symmetrized RMAT graph
+    of the BFS leg; the oracle's search-based labelling is timed on a smaller graph of
+    the same family and checks the device labels there."""
+    import oracle  # CPU baseline (and the check of the device result on its graph)
+    from plato_amd import Graph
+    from plato_amd.algo import connected_components
+    g = Graph.rmat(ctx, scale, 16, seed=seed, directed=False, build_csr=True, relabel=True)
+    connected_components(g)  # warm-up
+    res = connected_components(g)
+    out = {"scale": scale, "nnz": int(g.nnz), "epochs": res.epochs, "seconds": round(res.seconds, 4),
+           "gteps": round(g.nnz / 2 / res.seconds / 1e9, 2), "components": res.components,
+           "modes": res.modes.tolist()}
+    g.close()
+    s, d = oracle.rmat_edges(cpu_scale, 16, seed)
+    og = oracle.CSC(1 << cpu_scale, s, d, symmetrize=True)
+    t0 = time.perf_counter()
+    lab = oracle.cc_labels(og)
+    cpu_s = time.perf_counter() - t0
+    gs = Graph.rmat(ctx, cpu_scale, 16, seed=seed, directed=False, build_csr=True)
+    rs = connected_components(gs)
+    gs.close()
+    assert np.array_equal(rs.labels, lab)
+    out["cpu_baseline"] = {"scale": cpu_scale, "seconds": round(cpu_s, 3), "cores": 1, "kind": "port",
+                           "gpu_seconds_same_graph": round(rs.seconds, 4)}
+    return out
+
+
 def csv_text(src, dst):
     """`src,dst\n` lines with 10-digit zero-padded ids (22 bytes per edge), built with numpy."""
     E = src.size
@@ -370,6 +398,7 @@ def main():
     ap.add_argument("--bfs-roots", type=int, default=64)
     ap.add_argument("--no-bfs", action="store_true")
     ap.add_argument("--no-kcore", action="store_true")
+    ap.add_argument("--no-cc", action="store_true")
     ap.add_argument("--ppr-scale", type=int, default=24)
     ap.add_argument("--no-ppr", action="store_true")
     ap.add_argument("--ppr-block-only", action="store_true",
@@ -477,6 +506,9 @@ def main():
     if not args.no_bfs:
         log(f"[bench] bfs scale {args.bfs_scale}, {args.bfs_roots} roots")
         out["bfs"] = run_bfs(ctx, args.bfs_scale, args.bfs_roots)
+    if not args.no_cc:
+        log(f"[bench] connected components scale {args.bfs_scale}")
+        out["connected_components"] = run_cc(ctx, args.bfs_scale)
     if not args.no_kcore:
         log(f"[bench] k-core scale {args.bfs_scale}")
         out["kcore"] = run_kcore(ctx, args.bfs_scale)

@@ -330,6 +330,24 @@ typedef struct {
  * the size of the next active set (the reference's modified / need_modified log). */
 int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64_t* epoch_modified,
              uint64_t* epoch_next, uint32_t cap, pe_kcore_stats_t* stats);
+
+/* ---- connected components -------------------------------------------------- */
+
+typedef struct {
+  uint32_t epochs;      /* epochs run (the last one lowers no label) */
+  uint32_t components;  /* distinct labels over the id space [0, V) */
+  double seconds;       /* device time of the whole loop, HIP events */
+} pe_cc_stats_t;
+
+/* Replaces plato::algo::connected_component_t::compute (plato/algo/cgm/connected_component.hpp:
+ * 170-234): label propagation with the engine's push/pull rule (push needs
+ * PE_GRAPH_BUILD_CSR; without it every epoch pulls).  Writes labels[V] in input ids:
+ * the smallest input id from which each vertex is reachable (on a symmetrized graph,
+ * the smallest id of its component), and per epoch e < cap the mode (1 push, 0 pull)
+ * and the sum of the slots' returns (the reference's logged actives; they depend on
+ * the order in which labels spread, the labels do not). */
+int pe_connected_components(pe_graph_t* g, uint32_t* labels, uint8_t* modes, uint64_t* actives, uint32_t cap,
+                            pe_cc_stats_t* stats);
 /* A new graph with every edge of the directed graph g reversed (same input ids), for
  * pulls over out-edges such as inverse PageRank (the reference runs pagerank over its
  * bcsr_t out-edge structure, trustrank.hpp:101-103).  flags: PE_GRAPH_BUILD_CSR,

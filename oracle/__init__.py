@@ -67,6 +67,8 @@ def lib():
     L.orc_bfs.argtypes = [C.c_uint32, _u64p, _u32p, _u64p, _u32p, _u32p, C.c_uint64,
                           C.c_double, C.c_uint32, _i32p, _u8p, _u64p, _u64p, C.c_uint32,
                           C.POINTER(C.c_uint32)]
+    L.orc_cc_labels.argtypes = [C.c_uint32, _u64p, _u32p, _u32p]
+    L.orc_cc_labels.restype = None
     L.orc_kcore.argtypes = [C.c_uint32, _u64p, _u32p, _u32p, _u64p, _u64p, C.c_uint32]
     L.orc_kcore.restype = C.c_int
     L.orc_parse_csv.argtypes = [C.c_char_p, C.c_uint64, C.c_int, C.c_uint64, _u32p, _u32p, C.c_void_p]
@@ -223,6 +225,13 @@ def bfs(g: CSC, root, threshold=0.05, max_epochs=4096):
     k = min(n, max_epochs)
     return dict(levels=levels, modes=modes[:k], frontier=frontier[:k], active_edges=active[:k],
                 epochs=n, visited=int(visited.value))
+
+
+def cc_labels(g: CSC):
+    """connected_component_t's fixed point: the smallest id that reaches each vertex."""
+    labels = np.zeros(g.V, np.uint32)
+    lib().orc_cc_labels(g.V, g.row_ptr, np.ascontiguousarray(g.col_idx), labels)
+    return labels
 
 
 def kcore(g: CSC, out_edges=False, max_epochs=4096):

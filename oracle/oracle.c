@@ -510,6 +510,28 @@ int orc_kcore(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t* co
   return epoch;
 }
 
+/* ---- connected components (cgm/connected_component.hpp:170-234) --------------- */
+void orc_cc_labels(uint32_t V, const uint64_t* row_ptr, const uint32_t* col_idx, uint32_t* labels) {
+  uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * (V ? V : 1));
+  for (uint32_t v = 0; v < V; ++v) labels[v] = 0xFFFFFFFFu;
+  for (uint32_t u = 0; u < V; ++u) {
+    if (labels[u] != 0xFFFFFFFFu) continue;
+    uint64_t top = 0;
+    labels[u] = u;
+    stack[top++] = u;
+    while (top) {
+      const uint32_t w = stack[--top];
+      for (uint64_t e = row_ptr[w]; e < row_ptr[w + 1]; ++e) {
+        const uint32_t x = col_idx[e];
+        if (labels[x] != 0xFFFFFFFFu) continue;
+        labels[x] = u;
+        stack[top++] = x;
+      }
+    }
+  }
+  free(stack);
+}
+
 /* ---- edge-list ingestion (parsers.hpp:225-293) ------------------------------ */
 int64_t orc_parse_csv(const char* bytes, uint64_t n, int weighted, uint64_t cap, uint32_t* src,
                       uint32_t* dst, float* w) {

@@ -136,6 +136,19 @@ int orc_bfs(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
 int orc_kcore(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t* coreness,
               uint64_t* epoch_modified, uint64_t* epoch_next, uint32_t cap);
 
+/* ---- connected components ---------------------------------------------------- */
+
+/* The labels plato::algo::connected_component_t::compute (cgm/connected_component.hpp:
+ * 170-234) converges to: labels start at the vertex ids and only ever take the minimum
+ * of a label pushed along an out-edge or pulled over an in-edge, so at the fixed point
+ * label(v) = the smallest id u from which v is reachable (u = v included); on an
+ * undirected graph, the smallest id of v's component.  Computed here independently of
+ * the iteration: a search from every u in ascending order over out-edges labels what it
+ * reaches first (searches stop at vertices already labelled, whose reach is labelled).
+ * Pinned: the reference test's own BFS labels (cgm/test/test_cgm.cc:83-139) on
+ * v100_e2150_ua_c3.csv, scipy connected_components, brute-force reachability. */
+void orc_cc_labels(uint32_t V, const uint64_t* row_ptr, const uint32_t* col_idx, uint32_t* labels);
+
 /* ---- edge-list ingestion ---------------------------------------------------- */
 
 /* Plato's csv_parser (parsers.hpp:225-293) over a byte buffer, with the same C

@@ -222,3 +222,25 @@ def kcore(graph: Graph, out_edges: bool = False, max_epochs: int = 1 << 16) -> K
                              nxt.ctypes.data, max_epochs, C.byref(st)), graph.ctx.h)
     k = min(st.epochs, max_epochs)
     return KcoreResult(core, mod[:k], nxt[:k], st.epochs, st.seconds)
+
+
+@dataclass
+class CcResult:
+    labels: np.ndarray     # uint32 per vertex: smallest input id that reaches it
+    modes: np.ndarray      # per epoch: 1 push, 0 pull
+    actives: np.ndarray    # per epoch: sum of the slots' returns
+    epochs: int
+    components: int
+    seconds: float
+
+
+def connected_components(graph: Graph, max_epochs: int = 1 << 16) -> CcResult:
+    """connected_component_t::compute (cgm/connected_component.hpp:170-234)."""
+    labels = np.zeros(graph.V, np.uint32)
+    modes = np.zeros(max_epochs, np.uint8)
+    act = np.zeros(max_epochs, np.uint64)
+    st = L.CcStats()
+    check(graph.lib.pe_connected_components(graph.h, labels.ctypes.data, modes.ctypes.data, act.ctypes.data,
+                                            max_epochs, C.byref(st)), graph.ctx.h)
+    k = min(st.epochs, max_epochs)
+    return CcResult(labels, modes[:k], act[:k], st.epochs, st.components, st.seconds)

@@ -243,6 +243,8 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts);
 // arrays downloaded in input id order (elem = 4 or 8 bytes)
 void to_internal_ids(pe_graph_s* g, const uint32_t* host_ids, uint32_t n, uint32_t* host_out);
 void download_external(pe_graph_s* g, const void* dev, void* host, size_t elem);
+// sum of out-degrees over a bitmap's vertices (rank.hip)
+uint64_t active_edges_dev(pe_graph_s* g, const uint64_t* words, uint64_t nwords);
 // bitmap -> queue of its set ids below V, appended at *count (bfs.hip)
 void frontier_compact(pe_ctx_s* ctx, const uint64_t* words, uint64_t nwords, uint32_t V, uint32_t* queue,
                       unsigned long long* count);

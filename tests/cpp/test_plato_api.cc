@@ -10,6 +10,7 @@
 #include "plato/algo/pagerank/pagerank.hpp"
 #include "plato/algo/ppr/personalized_pagerank.hpp"
 #include "plato/algo/trustrank/trustrank.hpp"
+#include "plato/algo/cgm/connected_component.hpp"
 
 static int failures = 0;
 #define EXPECT(cond)                                                   \
@@ -164,6 +165,49 @@ static void TrustRank_Chain() {
   EXPECT(pairs.size() == 10 && pairs[9].v_i == 9 && pairs[9].rank > pairs[0].rank);
 }
 
+// plato/algo/cgm/test/test_cgm.cc:83-139: on v100_e2150_ua_c3.csv (undirected) every
+// label equals the BFS label of the test (the smallest id of the component); the file
+// has three components, and the edges of all components add up to the input edges.
+static void CGM_CalcComponent() {
+  plato::graph_info_t graph_info(false);
+  auto graph = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/v100_e2150_ua_c3.csv");
+  using dcsc_spec_t = plato::dcsc_t<plato::empty_t>;
+  using bcsr_spec_t = plato::bcsr_t<plato::empty_t>;
+  plato::dualmode_engine_t<dcsc_spec_t, bcsr_spec_t> engine(graph.second, graph.first, graph_info);
+  plato::algo::connected_component_t<dcsc_spec_t, bcsr_spec_t> cc(&engine, graph_info);
+  cc.compute();
+  // the test's reference labels, by BFS over the file's edges
+  std::vector<std::vector<int>> adj(100);
+  {
+    FILE* f = std::fopen((dir + "/v100_e2150_ua_c3.csv").c_str(), "r");
+    int a, b;
+    while (f && std::fscanf(f, "%d,%d", &a, &b) == 2) {
+      adj[a].push_back(b);
+      adj[b].push_back(a);
+    }
+    if (f) std::fclose(f);
+  }
+  std::vector<int> label(100, -1);
+  for (int i = 0; i < 100; ++i) {
+    if (label[i] != -1) continue;
+    label[i] = i;
+    std::vector<int> q{i};
+    for (size_t h = 0; h < q.size(); ++h)
+      for (int v : adj[q[h]])
+        if (label[v] == -1) {
+          label[v] = i;
+          q.push_back(v);
+        }
+  }
+  auto* labels = cc.get_labels();
+  for (int v = 0; v < 100; ++v) EXPECT((int)(*labels)[v] == label[v]);
+  EXPECT(cc.get_num_components() == 3);
+  uint64_t edges = 0;
+  for (auto& c : *cc.get_components_info()) edges += c.second.edges_;
+  EXPECT(edges == 2150);
+  EXPECT(cc.get_major_vertices() == cc.get_components_info()->at(cc.get_major_label()).vertices_);
+}
+
 int main(int argc, char** argv) {
   dir = argc > 1 ? argv[1] : "tests/golden";
   struct { const char* name; void (*fn)(); } cases[] = {
@@ -174,6 +218,7 @@ int main(int argc, char** argv) {
       {"PPR.ChainClosedForm", PPR_ChainClosedForm},
       {"BFS.BinaryTree", BFS_BinaryTree},
       {"TrustRank.Chain", TrustRank_Chain},
+      {"CGM.CalcComponent", CGM_CalcComponent},
   };
   for (auto& c : cases) {
     const int before = failures;

@@ -16,7 +16,8 @@ def test_host_programs_build():
     r = subprocess.run(["make", "-s", "-f", "plato_amd/host/Makefile", f"ROOT={ROOT}"], cwd=ROOT,
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "test_plato_api"):
+    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "cgm_simple",
+              "test_plato_api"):
         assert os.access(os.path.join(BIN, p), os.X_OK)
 
 
@@ -38,7 +39,7 @@ def test_cpp_api_known_answers():
                        text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("  OK  ") == 7
+    assert r.stdout.count("  OK  ") == 8
 
 
 @pytest.mark.gpu
@@ -140,3 +141,27 @@ def test_kcore_driver_matches_oracle(tmp_path):
         exp = [f"{v},{u}" for v in range(100) if core[v] == k
                for u in g.row_idx[int(g.col_ptr[v]):int(g.col_ptr[v + 1])]]
         assert lines == exp
+
+
+@pytest.mark.gpu
+def test_cgm_driver_outputs(tmp_path):
+    """cgm_simple: all_vertices writes `v,label` (the oracle's labels); the default
+    sub_graph_by_label writes the largest component's edges, src < dst."""
+    path = os.path.join(GOLDEN, "v100_e2150_ua_c3.csv")
+    s, d = oracle.read_edge_csv(path)
+    g = oracle.CSC(100, s, d, symmetrize=True)
+    lab = oracle.cc_labels(g)
+    r = subprocess.run([os.path.join(BIN, "cgm_simple"), "--input", path, "--output", str(tmp_path / "v"),
+                        "--output_method", "all_vertices"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "There are 3 components." in r.stdout
+    got = np.loadtxt(tmp_path / "v" / "part_0000_0.csv", delimiter=",", dtype=np.int64)
+    assert np.array_equal(got[:, 0], np.arange(100)) and np.array_equal(got[:, 1], lab)
+    r = subprocess.run([os.path.join(BIN, "cgm_simple"), "--input", path, "--output", str(tmp_path / "c")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    major = np.bincount(lab).argmax()
+    lines = (tmp_path / "c" / "part_0000_0.csv").read_text().split()
+    exp = [f"{int(u)},{v}" for v in range(100) for u in g.row_idx[int(g.col_ptr[v]):int(g.col_ptr[v + 1])]
+           if lab[u] == major and u < v]
+    assert lines == exp

@@ -350,3 +350,60 @@ def test_kcore_hand_worked_cases():
     # 3 has none (-> 0); 2's only out-neighbour 3 starts at 0xFFFFFFFF, so 2 keeps 1.
     core, mod, nxt = oracle.kcore(g, out_edges=True)
     assert core.tolist() == [1, 1, 1, 0, 1, 1] and mod.tolist() == [0] and nxt.tolist() == [0]
+
+
+# ---- connected components (cgm/connected_component.hpp) --------------------------------------------
+def _bfs_labels_like_reference_test(s, d, V):
+    """cgm/test/test_cgm.cc:83-104: undirected adjacency, BFS from each unlabelled id in
+    ascending order, every reached vertex labelled with the root."""
+    adj = [[] for _ in range(V)]
+    for a, b in zip(s.tolist(), d.tolist()):
+        adj[a].append(b)
+        adj[b].append(a)
+    label = [-1] * V
+    for i in range(V):
+        if label[i] != -1:
+            continue
+        label[i] = i
+        q = [i]
+        while q:
+            u = q.pop(0)
+            for v in adj[u]:
+                if label[v] == -1:
+                    label[v] = i
+                    q.append(v)
+    return np.array(label, np.uint32)
+
+
+def test_cc_labels_match_reference_test_and_scipy():
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    s, d = oracle.read_edge_csv(golden("v100_e2150_ua_c3.csv"))
+    g = oracle.CSC(100, s, d, symmetrize=True)
+    lab = oracle.cc_labels(g)
+    assert np.array_equal(lab, _bfs_labels_like_reference_test(s, d, 100))
+    assert len(set(lab.tolist())) == 3  # the file's three components
+    for scale in (10, 14):
+        s, d = oracle.rmat_edges(scale, 4, 3)
+        V = 1 << scale
+        lab = oracle.cc_labels(oracle.CSC(V, s, d, symmetrize=True))
+        n, comp = connected_components(coo_matrix((np.ones(s.size), (s, d)), shape=(V, V)), directed=False)
+        assert len(set(lab.tolist())) == n
+        # the label is the smallest id of the component
+        mins = {}
+        for v, c in enumerate(comp.tolist()):
+            mins.setdefault(c, v)
+        assert np.array_equal(lab, np.array([mins[c] for c in comp.tolist()], np.uint32))
+
+
+def test_cc_labels_directed_reachability():
+    """Directed: the smallest id that reaches v, by brute force over reachability."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import shortest_path
+    s, d = oracle.rmat_edges(7, 2, 5)
+    V = 1 << 7
+    lab = oracle.cc_labels(oracle.CSC(V, s, d))
+    reach = np.isfinite(shortest_path(coo_matrix((np.ones(s.size), (s, d)), shape=(V, V)), directed=True,
+                                      unweighted=True))
+    exp = np.array([np.flatnonzero(reach[:, v])[0] for v in range(V)], np.uint32)
+    assert np.array_equal(lab, exp)
