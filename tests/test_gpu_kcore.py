@@ -40,7 +40,7 @@ def test_hand_worked(ctx):
 def test_reference_fixtures(ctx):
     from plato_amd import Graph
     for name in ("v100_e2150_ua_c3.csv", "v100_e4300_da_c3.csv", "graph_10_9.csv",
-                 "complete_binary_tree_10_9.csv"):
+                 "complete_binary_tree_10_9.csv", "kcore_test_7_11.csv"):
         s, d = oracle.read_edge_csv(golden(name))
         V = int(max(s.max(), d.max())) + 1
         for directed in (True, False):

@@ -316,7 +316,7 @@ def test_kcore_matches_networkx_core_number():
     textbook coreness (networkx core_number, Batagelj-Zaversnik peeling)."""
     import networkx as nx
     cases = [oracle.rmat_edges(10, 8, 4), oracle.rmat_edges(12, 16, 5),
-             oracle.read_edge_csv(golden("v100_e2150_ua_c3.csv"))]
+             oracle.read_edge_csv(golden("v100_e2150_ua_c3.csv")), oracle.read_edge_csv(golden("kcore_test_7_11.csv"))]
     for s, d in cases:
         a, b = _simple_undirected(s, d)
         V = int(max(s.max(), d.max())) + 1
