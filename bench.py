@@ -234,6 +234,36 @@ def run_cc(ctx, scale, cpu_scale=22, seed=11):
     return out
 
 
+def run_hyperanf(ctx, scale, bits=6, cpu_scale=16, seed=11):
+    """HyperANF (plato::algo::hyperanf) on the symmetrized RMAT graph of the BFS leg with
+    2^bits-register counters (the reference driver's default width); the oracle runs the
+    same iteration on a smaller graph of the same family and checks the device there."""
+    import oracle  # CPU baseline (and the check of the device result on its graph)
+    from plato_amd import Graph
+    from plato_amd.algo import hyperanf, hyperanf_opts_t
+    g = Graph.rmat(ctx, scale, 16, seed=seed, directed=False, relabel=True)
+    hyperanf(g, hyperanf_opts_t(20, bits))  # warm-up
+    res = hyperanf(g, hyperanf_opts_t(20, bits))
+    iters = max(res.anfs.size, 1)
+    out = {"scale": scale, "bits": bits, "nnz": int(g.nnz), "anfs": int(res.anfs.size),
+           "avg_distance": res.avg_distance, "seconds": round(res.seconds, 4),
+           "counter_bytes_per_edge_iteration": 1 << bits,
+           "gteps": round(g.nnz * iters / res.seconds / 1e9, 2)}
+    g.close()
+    s, d = oracle.rmat_edges(cpu_scale, 16, seed)
+    og = oracle.CSC(1 << cpu_scale, s, d, symmetrize=True)
+    t0 = time.perf_counter()
+    anfs, avg, regs = oracle.hyperanf(og, bits=bits, iters=20)
+    cpu_s = time.perf_counter() - t0
+    gs = Graph.rmat(ctx, cpu_scale, 16, seed=seed, directed=False)
+    rs = hyperanf(gs, hyperanf_opts_t(20, bits), registers=True)
+    gs.close()
+    assert np.array_equal(rs.registers, regs)
+    out["cpu_baseline"] = {"scale": cpu_scale, "seconds": round(cpu_s, 3), "cores": 1, "kind": "port",
+                           "gpu_seconds_same_graph": round(rs.seconds, 4)}
+    return out
+
+
 def csv_text(src, dst):
     """`src,dst\n` lines with 10-digit zero-padded ids (22 bytes per edge), built with numpy."""
     E = src.size
@@ -399,6 +429,7 @@ def main():
     ap.add_argument("--no-bfs", action="store_true")
     ap.add_argument("--no-kcore", action="store_true")
     ap.add_argument("--no-cc", action="store_true")
+    ap.add_argument("--no-hyperanf", action="store_true")
     ap.add_argument("--ppr-scale", type=int, default=24)
     ap.add_argument("--no-ppr", action="store_true")
     ap.add_argument("--ppr-block-only", action="store_true",
@@ -509,6 +540,9 @@ def main():
     if not args.no_cc:
         log(f"[bench] connected components scale {args.bfs_scale}")
         out["connected_components"] = run_cc(ctx, args.bfs_scale)
+    if not args.no_hyperanf:
+        log(f"[bench] hyperanf scale {args.bfs_scale}")
+        out["hyperanf"] = run_hyperanf(ctx, args.bfs_scale)
     if not args.no_kcore:
         log(f"[bench] k-core scale {args.bfs_scale}")
         out["kcore"] = run_kcore(ctx, args.bfs_scale)

@@ -348,6 +348,21 @@ typedef struct {
  * the order in which labels spread, the labels do not). */
 int pe_connected_components(pe_graph_t* g, uint32_t* labels, uint8_t* modes, uint64_t* actives, uint32_t cap,
                             pe_cc_stats_t* stats);
+
+/* ---- HyperANF ---------------------------------------------------------------- */
+
+typedef struct {
+  uint32_t iterations;  /* hyperanf_opts_t::iteration_ (20) */
+  uint32_t bits;        /* HyperLogLog registers per counter = 2^bits, 6..16 (the BitWidth argument) */
+} pe_hyperanf_opts_t;
+
+/* Replaces plato::algo::hyperanf (plato/algo/hyperanf/hyperanf.hpp:80-197) over the
+ * in-edges: anfs[0..cap) gets the neighbourhood-function estimates (the initial one
+ * first), *nanfs their number, *avg_distance GetAvgDistance of them (the reference's
+ * return value), registers (optional, [V << bits] bytes in input-id order) the final
+ * counters, *seconds the device time. */
+int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double* anfs, uint32_t cap, uint32_t* nanfs,
+                double* avg_distance, uint8_t* registers, double* seconds);
 /* A new graph with every edge of the directed graph g reversed (same input ids), for
  * pulls over out-edges such as inverse PageRank (the reference runs pagerank over its
  * bcsr_t out-edge structure, trustrank.hpp:101-103).  flags: PE_GRAPH_BUILD_CSR,

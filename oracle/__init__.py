@@ -67,6 +67,14 @@ def lib():
     L.orc_bfs.argtypes = [C.c_uint32, _u64p, _u32p, _u64p, _u32p, _u32p, C.c_uint64,
                           C.c_double, C.c_uint32, _i32p, _u8p, _u64p, _u64p, C.c_uint32,
                           C.POINTER(C.c_uint32)]
+    L.orc_murmur3_32.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32]
+    L.orc_murmur3_32.restype = C.c_uint32
+    L.orc_hll_estimate.argtypes = [_u8p, C.c_uint32]
+    L.orc_hll_estimate.restype = C.c_double
+    L.orc_hyperanf.argtypes = [C.c_uint32, _u64p, _u32p, C.c_uint32, C.c_uint32, _u8p, _f64p, C.c_uint32]
+    L.orc_hyperanf.restype = C.c_int
+    L.orc_anf_avg_distance.argtypes = [_f64p, C.c_uint32]
+    L.orc_anf_avg_distance.restype = C.c_double
     L.orc_cc_labels.argtypes = [C.c_uint32, _u64p, _u32p, _u32p]
     L.orc_cc_labels.restype = None
     L.orc_kcore.argtypes = [C.c_uint32, _u64p, _u32p, _u32p, _u64p, _u64p, C.c_uint32]
@@ -225,6 +233,24 @@ def bfs(g: CSC, root, threshold=0.05, max_epochs=4096):
     k = min(n, max_epochs)
     return dict(levels=levels, modes=modes[:k], frontier=frontier[:k], active_edges=active[:k],
                 epochs=n, visited=int(visited.value))
+
+
+def murmur3_32(data: bytes, seed: int) -> int:
+    return int(lib().orc_murmur3_32(data, len(data), seed))
+
+
+def hll_estimate(regs) -> float:
+    regs = np.ascontiguousarray(regs, np.uint8)
+    return float(lib().orc_hll_estimate(regs, int(regs.size).bit_length() - 1))
+
+
+def hyperanf(g: CSC, bits=12, iters=20, max_anfs=4096):
+    """hyperanf over the in-edges: (anfs, average distance, registers[V][2^bits])."""
+    regs = np.zeros((g.V, 1 << bits), np.uint8)
+    anfs = np.zeros(max_anfs, np.float64)
+    n = lib().orc_hyperanf(g.V, g.col_ptr, np.ascontiguousarray(g.row_idx), bits, iters, regs, anfs, max_anfs)
+    anfs = anfs[:min(n, max_anfs)]
+    return anfs, float(lib().orc_anf_avg_distance(anfs, anfs.size)), regs
 
 
 def cc_labels(g: CSC):

@@ -8,6 +8,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -530,6 +531,118 @@ void orc_cc_labels(uint32_t V, const uint64_t* row_ptr, const uint32_t* col_idx,
     }
   }
   free(stack);
+}
+
+/* ---- HyperANF (hyperanf.hpp:80-197, hyperloglog.hpp:32-142) -------------------- */
+static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+uint32_t orc_murmur3_32(const void* key, uint32_t len, uint32_t seed) {
+  const uint8_t* data = (const uint8_t*)key;
+  const uint32_t c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+  uint32_t h = seed;
+  const uint32_t nblocks = len / 4;
+  for (uint32_t i = 0; i < nblocks; ++i) {
+    uint32_t k;
+    memcpy(&k, data + 4 * i, 4);  /* little-endian blocks */
+    k *= c1;
+    k = rotl32(k, 15);
+    k *= c2;
+    h ^= k;
+    h = rotl32(h, 13);
+    h = h * 5 + 0xe6546b64u;
+  }
+  const uint8_t* tail = data + 4 * nblocks;
+  uint32_t k = 0;
+  switch (len & 3) {
+    case 3: k ^= (uint32_t)tail[2] << 16; /* fall through */
+    case 2: k ^= (uint32_t)tail[1] << 8;  /* fall through */
+    case 1:
+      k ^= tail[0];
+      k *= c1;
+      k = rotl32(k, 15);
+      k *= c2;
+      h ^= k;
+  }
+  h ^= len;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+double orc_hll_estimate(const uint8_t* regs, uint32_t P) {
+  const uint32_t m = 1u << P;
+  double alpha_mm;
+  switch (P) {
+    case 4: alpha_mm = 0.673; break;
+    case 5: alpha_mm = 0.697; break;
+    case 6: alpha_mm = 0.709; break;
+    default: alpha_mm = 0.7213 / (1.0 + 1.079 / (double)m); break;
+  }
+  alpha_mm *= (double)m * (double)m;
+  double sum = 0.0;
+  for (uint32_t i = 0; i < m; ++i) sum += 1.0 / (double)(1u << regs[i]);
+  double est = alpha_mm / sum;
+  if (est <= 5.0 / 2.0 * (double)m) {
+    uint32_t zeros = 0;
+    for (uint32_t i = 0; i < m; ++i) zeros += regs[i] == 0;
+    if (zeros) est = (double)m * log((double)m / zeros);
+  } else if (est > (1.0 / 30.0) * 4294967296.0) {
+    est = -4294967296.0 * log(1.0 - (est / 4294967296.0));
+  }
+  return est;
+}
+
+int orc_hyperanf(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx, uint32_t P, uint32_t iters,
+                 uint8_t* regs, double* anfs, uint32_t cap) {
+  const uint64_t m = 1ull << P;
+  uint8_t* nxt = (uint8_t*)malloc((size_t)V * m + 1);
+  memset(regs, 0, (size_t)V * m);
+  double anf = 0.0;
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint32_t h = orc_murmur3_32(&v, 4, 0x5f61767au);
+    const uint32_t idx = h >> (32 - P);
+    uint32_t w = h << P;
+    uint8_t rank = 1;
+    while (rank <= 32 - P && !(w & 0x80000000u)) {
+      ++rank;
+      w <<= 1;
+    }
+    if (rank > regs[(size_t)v * m + idx]) regs[(size_t)v * m + idx] = rank;
+    anf += orc_hll_estimate(regs + (size_t)v * m, P);
+  }
+  int n = 0;
+  if ((uint32_t)n < cap) anfs[n] = anf;
+  ++n;
+  double last = anf;
+  for (uint32_t it = 0; it < iters; ++it) {
+    memcpy(nxt, regs, (size_t)V * m);
+    for (uint32_t v = 0; v < V; ++v) {
+      uint8_t* dst = nxt + (size_t)v * m;
+      for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
+        const uint8_t* src = regs + (size_t)row_idx[e] * m;
+        for (uint64_t i = 0; i < m; ++i)
+          if (src[i] > dst[i]) dst[i] = src[i];
+      }
+    }
+    memcpy(regs, nxt, (size_t)V * m);
+    anf = 0.0;
+    for (uint32_t v = 0; v < V; ++v) anf += orc_hll_estimate(regs + (size_t)v * m, P);
+    if (anf - last < DBL_EPSILON) break;
+    if ((uint32_t)n < cap) anfs[n] = anf;
+    ++n;
+    last = anf;
+  }
+  free(nxt);
+  return n;
+}
+
+double orc_anf_avg_distance(const double* anfs, uint32_t n) {
+  double avg = 0.0;
+  for (uint32_t i = 1; i < n; ++i) avg += i * (anfs[i] - anfs[i - 1]);
+  return avg / (anfs[n - 1] - anfs[0]);
 }
 
 /* ---- edge-list ingestion (parsers.hpp:225-293) ------------------------------ */

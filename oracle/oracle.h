@@ -149,6 +149,31 @@ int orc_kcore(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t* co
  * v100_e2150_ua_c3.csv, scipy connected_components, brute-force reachability. */
 void orc_cc_labels(uint32_t V, const uint64_t* row_ptr, const uint32_t* col_idx, uint32_t* labels);
 
+/* ---- HyperANF ---------------------------------------------------------------- */
+
+/* MurmurHash3_x86_32 (A. Appleby, public domain), the hash of plato/util/hash.hpp:33-96.
+ * Pinned: the published test vectors of the algorithm. */
+uint32_t orc_murmur3_32(const void* key, uint32_t len, uint32_t seed);
+
+/* hyperloglog_t<P>::estimate (plato/util/hyperloglog.hpp:77-116) of 2^P registers. */
+double orc_hll_estimate(const uint8_t* regs, uint32_t P);
+
+/* plato::algo::hyperanf (hyperanf.hpp:80-197) with 2^P-register counters
+ * (hyperloglog.hpp:32-142): each vertex starts with add(&v, 4) (murmur3 of its 4-byte id,
+ * seed 0x5f61767a; register hash >> (32-P), rank 1 + leading zeros of hash << P, at most
+ * 33-P); an iteration gives every vertex the register-wise max of its counter and its
+ * in-neighbours' counters of the previous iteration (push and pull deliver the same);
+ * anf = sum of the estimates; the loop stops after `iters` iterations or when anf grows
+ * by less than DBL_EPSILON (that anf is not kept).  Writes the anfs (first: the initial
+ * one) and returns their number; regs[V << P] gets the final counters.
+ * Pinned: exact BFS average distance within 0.05 on raw_graph_7_7.csv (the reference
+ * test), counters equal to the HLL of each vertex's in-reachable set within t steps. */
+int orc_hyperanf(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx, uint32_t P, uint32_t iters,
+                 uint8_t* regs, double* anfs, uint32_t cap);
+
+/* GetAvgDistance (hyperanf.hpp:58-67) of n anfs. */
+double orc_anf_avg_distance(const double* anfs, uint32_t n);
+
 /* ---- edge-list ingestion ---------------------------------------------------- */
 
 /* Plato's csv_parser (parsers.hpp:225-293) over a byte buffer, with the same C

@@ -58,6 +58,10 @@ class RankOpts(C.Structure):
                 ("precision", C.c_int32), ("weighted", C.c_int32), ("l1_norm_tol", C.c_double)]
 
 
+class HyperAnfOpts(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("bits", C.c_uint32)]
+
+
 class CcStats(C.Structure):
     _fields_ = [("epochs", C.c_uint32), ("components", C.c_uint32), ("seconds", C.c_double)]
 
@@ -125,6 +129,7 @@ SIGNATURES = {
     "pe_trustrank": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pe_kcore": [_vp, C.c_int, _vp, _vp, _vp, _u32, _vp],
     "pe_connected_components": [_vp, _vp, _vp, _vp, _u32, _vp],
+    "pe_hyperanf": [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp],
     "pe_graph_create_rmat": [_vp, _u32, _u32, _u64, _dbl, _dbl, _dbl, _u32, _P(_vp)],
     "pe_rmat_edges": [_vp, _u32, _u64, _dbl, _dbl, _dbl, _u64, _u64, _vp, _vp, _vp],
     "pe_graph_destroy": [_vp],

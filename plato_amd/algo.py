@@ -244,3 +244,32 @@ def connected_components(graph: Graph, max_epochs: int = 1 << 16) -> CcResult:
                                             max_epochs, C.byref(st)), graph.ctx.h)
     k = min(st.epochs, max_epochs)
     return CcResult(labels, modes[:k], act[:k], st.epochs, st.components, st.seconds)
+
+
+@dataclass
+class hyperanf_opts_t:
+    """plato/algo/hyperanf/hyperanf.hpp:39-42 (bits: the BitWidth template argument)."""
+    iteration_: int = 20
+    bits: int = 12
+
+
+@dataclass
+class HyperAnfResult:
+    avg_distance: float
+    anfs: np.ndarray
+    registers: np.ndarray | None  # [V][2^bits] uint8 in input ids, when asked for
+    seconds: float
+
+
+def hyperanf(graph: Graph, opts: hyperanf_opts_t = hyperanf_opts_t(), registers: bool = False,
+             max_anfs: int = 4096) -> HyperAnfResult:
+    """plato::algo::hyperanf (hyperanf.hpp:80-197): the average distance estimate."""
+    o = L.HyperAnfOpts(opts.iteration_, opts.bits)
+    anfs = np.zeros(max_anfs, np.float64)
+    n = C.c_uint32()
+    avg = C.c_double()
+    sec = C.c_double()
+    regs = np.zeros((graph.V, 1 << opts.bits), np.uint8) if registers else None
+    check(graph.lib.pe_hyperanf(graph.h, C.byref(o), anfs.ctypes.data, max_anfs, C.byref(n), C.byref(avg),
+                                regs.ctypes.data if registers else None, C.byref(sec)), graph.ctx.h)
+    return HyperAnfResult(avg.value, anfs[:min(n.value, max_anfs)], regs, sec.value)

@@ -1,0 +1,257 @@
+// HyperANF: plato::algo::hyperanf (plato/algo/hyperanf/hyperanf.hpp:80-197) on the device.
+//
+// Every vertex holds a HyperLogLog counter of 2^P one-byte registers
+// (plato/util/hyperloglog.hpp:32-142), started with its own id (MurmurHash3_x86_32 of
+// the 4-byte id, seed 0x5f61767a: register = hash >> (32 - P), rank = 1 + leading
+// zeros of hash << P, at most 33 - P).  An iteration gives every vertex the
+// register-wise max of its counter and its in-neighbours' counters of the previous
+// iteration: the reference's push (active vertices merge into their out-neighbours'
+// deltas) and pull (every vertex merges its in-neighbours) deliver exactly that, since
+// a counter that did not change was already merged downstream.  Register max is
+// order-independent, so the counters are bit-exact; anf is the FP64 sum of the
+// per-vertex estimates, and the loop stops after `iterations` or when anf grows by less
+// than DBL_EPSILON (that anf is dropped).  The average distance is GetAvgDistance over
+// the anfs (hyperanf.hpp:58-67).
+//
+// Layout: counters [V][2^P] bytes, double-buffered (Jacobi).  Pull: one wave per
+// vertex; each lane owns 2^P / 64 consecutive registers (bytes, or 32-bit words of four
+// registers when P >= 8) and the wave walks the in-edges, one 2^P-byte row per edge.
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+#include "pe_internal.hpp"
+
+namespace pe {
+
+__device__ __forceinline__ uint32_t hanf_rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+// MurmurHash3_x86_32 of one little-endian 32-bit word (len 4), the reference's hash of
+// a vertex id (plato/util/hash.hpp:33-96, A. Appleby's published algorithm)
+__device__ __forceinline__ uint32_t hanf_murmur_u32(uint32_t key, uint32_t seed) {
+  uint32_t k = key * 0xcc9e2d51u;
+  k = hanf_rotl(k, 15);
+  k *= 0x1b873593u;
+  uint32_t h = seed ^ k;
+  h = hanf_rotl(h, 13);
+  h = h * 5 + 0xe6546b64u;
+  h ^= 4u;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+__global__ void k_hanf_init(uint8_t* __restrict__ regs, uint32_t V, int P, const uint32_t* __restrict__ perm) {
+  const uint64_t m = 1ull << P;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t id = perm ? perm[v] : (uint32_t)v;  // counters hash input ids
+    const uint32_t h = hanf_murmur_u32(id, 0x5f61767au);
+    const uint32_t w = h << P;
+    const uint32_t q = 32u - (uint32_t)P;
+    const uint32_t rank = w ? (uint32_t)__clz(w) + 1u : q + 1u;
+    regs[v * m + (h >> q)] = (uint8_t)(rank < q + 1u ? rank : q + 1u);
+  }
+}
+
+__global__ void k_hanf_zero(uint32_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__device__ __forceinline__ uint32_t byte_max4(uint32_t a, uint32_t b) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < 32; i += 8) r |= max((a >> i) & 0xFFu, (b >> i) & 0xFFu) << i;
+  return r;
+}
+
+// Pull with W 32-bit words (4 W registers) per lane and chunk, P >= 8: a counter row
+// of `row` words is handled in chunks of 64 W words (larger counters walk the in-edges
+// once per chunk).
+template <int W>
+__global__ __launch_bounds__(256) void k_hanf_pull_words(const uint64_t* __restrict__ col_ptr,
+                                                         const uint32_t* __restrict__ row_idx, uint32_t V,
+                                                         uint64_t row, const uint32_t* __restrict__ cur,
+                                                         uint32_t* __restrict__ nxt) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  for (uint64_t v = wave_id; v < V; v += nwaves) {
+    const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    for (uint64_t c0 = 0; c0 < row; c0 += 64 * W) {
+      uint32_t acc[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[j] = cur[v * row + c0 + (uint64_t)j * 64 + lane];
+      for (uint64_t k = b; k < e; ++k) {
+        const uint64_t u = row_idx[k];
+#pragma unroll
+        for (int j = 0; j < W; ++j) acc[j] = byte_max4(acc[j], cur[u * row + c0 + (uint64_t)j * 64 + lane]);
+      }
+#pragma unroll
+      for (int j = 0; j < W; ++j) nxt[v * row + c0 + (uint64_t)j * 64 + lane] = acc[j];
+    }
+  }
+}
+
+// Pull with B bytes per lane (P = 6 or 7).
+template <int B>
+__global__ __launch_bounds__(256) void k_hanf_pull_bytes(const uint64_t* __restrict__ col_ptr,
+                                                         const uint32_t* __restrict__ row_idx, uint32_t V,
+                                                         const uint8_t* __restrict__ cur, uint8_t* __restrict__ nxt) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  constexpr uint64_t ROW = 64 * B;
+  for (uint64_t v = wave_id; v < V; v += nwaves) {
+    uint32_t acc[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) acc[j] = cur[v * ROW + (uint64_t)lane * B + j];
+    const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    for (uint64_t k = b; k < e; ++k) {
+      const uint64_t u = row_idx[k];
+#pragma unroll
+      for (int j = 0; j < B; ++j) acc[j] = max(acc[j], (uint32_t)cur[u * ROW + (uint64_t)lane * B + j]);
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) nxt[v * ROW + (uint64_t)lane * B + j] = (uint8_t)acc[j];
+  }
+}
+
+// Per-vertex estimate (hyperloglog.hpp:77-116), summed per block in a fixed order.
+// The register sum of 2^-reg is exact in FP64 (at most 2^16 terms of 2^-k, k <= 27).
+__global__ __launch_bounds__(256) void k_hanf_estimate(const uint8_t* __restrict__ regs, uint32_t V, int P,
+                                                       double* __restrict__ part) {
+  __shared__ double s_w[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t m = 1ull << P;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  double alpha;
+  switch (P) {
+    case 4: alpha = 0.673; break;
+    case 5: alpha = 0.697; break;
+    case 6: alpha = 0.709; break;
+    default: alpha = 0.7213 / (1.0 + 1.079 / (double)m); break;
+  }
+  alpha *= (double)m * (double)m;
+  double total = 0.0;  // lane 0's running sum over this wave's vertices, in order
+  for (uint64_t v = wave_id; v < V; v += nwaves) {
+    double s = 0.0;
+    uint32_t zeros = 0;
+    for (uint64_t i = lane; i < m; i += 64) {
+      const uint32_t r = regs[v * m + i];
+      s += 1.0 / (double)(1u << r);
+      zeros += r == 0 ? 1u : 0u;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      s += __shfl_xor(s, off, 64);
+      zeros += __shfl_xor(zeros, off, 64);
+    }
+    double est = alpha / s;
+    if (est <= 5.0 / 2.0 * (double)m) {
+      if (zeros) est = (double)m * log((double)m / zeros);
+    } else if (est > (1.0 / 30.0) * 4294967296.0) {
+      est = -4294967296.0 * log(1.0 - (est / 4294967296.0));
+    }
+    total += est;
+  }
+  if (lane == 0) s_w[wave] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// counters in input-id order
+__global__ void k_hanf_rows_external(const uint8_t* __restrict__ in, const uint32_t* __restrict__ rank_of,
+                                     uint32_t V, uint64_t m, uint8_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)V * m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = i / m, j = i % m;
+    out[i] = in[(uint64_t)rank_of[v] * m + j];
+  }
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double* anfs, uint32_t cap,
+                           uint32_t* nanfs, double* avg_distance, uint8_t* registers, double* seconds) {
+  if (!g || !opts || !anfs || cap == 0) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    const int P = (int)opts->bits;
+    PE_REQUIRE(P >= 6 && P <= 16, PE_ERR_INVALID, "bits must be in [6, 16] (hyperanf_test supported range)");
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const uint32_t V = g->V;
+    const uint64_t m = 1ull << P, bytes = std::max<uint64_t>(4, (uint64_t)V * m);
+    dbuf<uint8_t> a(ctx, bytes), b(ctx, bytes);
+    const unsigned est_grid = persistent_grid(ctx, grid_for(V, 4), 4);
+    dbuf<double> part(ctx, est_grid);
+    std::vector<double> hpart(est_grid);
+    const unsigned pull_grid = persistent_grid(ctx, grid_for(V, 4), 8);
+    const uint32_t* perm = g->relabeled ? g->perm.p : nullptr;
+    auto anf_of = [&](const uint8_t* regs) {
+      k_hanf_estimate<<<est_grid, 256, 0, s>>>(regs, V, P, part.p);
+      PE_HIP(hipGetLastError());
+      PE_HIP(hipMemcpyAsync(hpart.data(), part.p, est_grid * sizeof(double), hipMemcpyDeviceToHost, s));
+      PE_HIP(hipStreamSynchronize(s));
+      double t = 0.0;
+      for (double x : hpart) t += x;
+      return t;
+    };
+    auto pull = [&](const uint8_t* cur, uint8_t* nxt) {
+      switch (P) {
+        case 6: k_hanf_pull_bytes<1><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, cur, nxt); break;
+        case 7: k_hanf_pull_bytes<2><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, cur, nxt); break;
+#define HANF_W(PP, W)                                                                                       \
+  case PP:                                                                                                  \
+    k_hanf_pull_words<W><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, m / 4, (const uint32_t*)cur, \
+                                                   (uint32_t*)nxt);                                         \
+    break;
+        HANF_W(8, 1) HANF_W(9, 2) HANF_W(10, 4) HANF_W(11, 8) HANF_W(12, 16) HANF_W(13, 16) HANF_W(14, 16)
+        HANF_W(15, 16) HANF_W(16, 16)
+#undef HANF_W
+      }
+      PE_HIP(hipGetLastError());
+    };
+    PE_HIP(hipEventRecord(ctx->t0, s));
+    k_hanf_zero<<<grid_for(bytes / 4, 256, 65536), 256, 0, s>>>((uint32_t*)a.p, bytes / 4);
+    k_hanf_init<<<grid_for(V, 256, 8192), 256, 0, s>>>(a.p, V, P, perm);
+    PE_HIP(hipGetLastError());
+    std::vector<double> out;
+    double last = anf_of(a.p);
+    out.push_back(last);
+    uint8_t* cur = a.p;
+    uint8_t* nxt = b.p;
+    for (uint32_t it = 0; it < opts->iterations; ++it) {
+      pull(cur, nxt);
+      const double anf = anf_of(nxt);
+      std::swap(cur, nxt);
+      if (anf - last < DBL_EPSILON) break;  // hyperanf.hpp:189-191
+      out.push_back(anf);
+      last = anf;
+    }
+    PE_HIP(hipEventRecord(ctx->t1, s));
+    PE_HIP(hipEventSynchronize(ctx->t1));
+    float ms = 0.f;
+    PE_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+    for (size_t i = 0; i < out.size() && i < cap; ++i) anfs[i] = out[i];
+    if (nanfs) *nanfs = (uint32_t)out.size();
+    if (avg_distance) {  // GetAvgDistance (hyperanf.hpp:58-67)
+      double avg = 0.0;
+      for (size_t i = 1; i < out.size(); ++i) avg += i * (out[i] - out[i - 1]);
+      *avg_distance = avg / (out.back() - out.front());
+    }
+    if (registers) {
+      if (g->relabeled) {
+        k_hanf_rows_external<<<grid_for((uint64_t)V * m, 256, 65536), 256, 0, s>>>(cur, g->rank_of.p, V, m, nxt);
+        PE_HIP(hipGetLastError());
+        PE_HIP(hipMemcpyAsync(registers, nxt, (uint64_t)V * m, hipMemcpyDeviceToHost, s));
+      } else {
+        PE_HIP(hipMemcpyAsync(registers, cur, (uint64_t)V * m, hipMemcpyDeviceToHost, s));
+      }
+      PE_HIP(hipStreamSynchronize(s));
+    }
+    if (seconds) *seconds = ms * 1e-3;
+  });
+}

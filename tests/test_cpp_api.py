@@ -16,7 +16,7 @@ def test_host_programs_build():
     r = subprocess.run(["make", "-s", "-f", "plato_amd/host/Makefile", f"ROOT={ROOT}"], cwd=ROOT,
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "cgm_simple",
+    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "cgm_simple", "hyperanf",
               "test_plato_api"):
         assert os.access(os.path.join(BIN, p), os.X_OK)
 
@@ -165,3 +165,18 @@ def test_cgm_driver_outputs(tmp_path):
     exp = [f"{int(u)},{v}" for v in range(100) for u in g.row_idx[int(g.col_ptr[v]):int(g.col_ptr[v + 1])]
            if lab[u] == major and u < v]
     assert lines == exp
+
+
+@pytest.mark.gpu
+def test_hyperanf_driver_reference_case():
+    """The reference test's case through the driver: raw_graph_7_7.csv, bits 6, 20
+    iterations; the printed average distance is the oracle's (FP64, 1e-9)."""
+    path = os.path.join(GOLDEN, "raw_graph_7_7.csv")
+    s, d = oracle.read_edge_csv(path)
+    V = int(max(s.max(), d.max())) + 1
+    _, avg, _ = oracle.hyperanf(oracle.CSC(V, s, d, symmetrize=True), bits=6, iters=20)
+    r = subprocess.run([os.path.join(BIN, "hyperanf"), "--input", path, "--bits", "6"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = float(r.stdout.split("avg_distance: ")[1].split(",")[0])
+    assert abs(got - avg) <= 1e-9 * abs(avg)

@@ -407,3 +407,71 @@ def test_cc_labels_directed_reachability():
                                       unweighted=True))
     exp = np.array([np.flatnonzero(reach[:, v])[0] for v in range(V)], np.uint32)
     assert np.array_equal(lab, exp)
+
+
+# ---- HyperANF (hyperanf.hpp:80-197, hyperloglog.hpp, hash.hpp) ----------------------------------------
+def test_murmur3_published_vectors():
+    """MurmurHash3_x86_32's published test vectors (empty input, every tail length, seeds)."""
+    vec = [(b"", 0, 0x00000000), (b"", 1, 0x514E28B7), (b"", 0xffffffff, 0x81F16F39),
+           (b"\xff\xff\xff\xff", 0, 0x76293B50), (b"!Ce\x87", 0, 0xF55B516B), (b"!Ce\x87", 0x5082EDEE, 0x2362F9DE),
+           (b"!Ce", 0, 0x7E4A8634), (b"!C", 0, 0xA0F7B07A), (b"!", 0, 0x72661CF4),
+           (b"\x00\x00\x00\x00", 0, 0x2362F9DE), (b"\x00\x00\x00", 0, 0x85F0B427), (b"\x00\x00", 0, 0x30F4C306),
+           (b"\x00", 0, 0x514E28B7), (b"Hello, world!", 0x9747b28c, 0x24884CBA),
+           (b"The quick brown fox jumps over the lazy dog", 0x9747b28c, 0x2FA826CD), (b"aaaa", 0x9747b28c, 0x5A97808A)]
+    for data, seed, exp in vec:
+        assert oracle.murmur3_32(data, seed) == exp, (data, seed)
+
+
+def _hll_of(ids, bits):
+    """hyperloglog_t::add over the ids (4-byte little-endian, seed 0x5f61767a)."""
+    regs = np.zeros(1 << bits, np.uint8)
+    for v in ids:
+        h = oracle.murmur3_32(int(v).to_bytes(4, "little"), 0x5f61767a)
+        idx = h >> (32 - bits)
+        w = (h << bits) & 0xFFFFFFFF
+        rank = 1
+        while rank <= 32 - bits and not (w & 0x80000000):
+            rank += 1
+            w = (w << 1) & 0xFFFFFFFF
+        regs[idx] = max(regs[idx], rank)
+    return regs
+
+
+def test_hyperanf_counters_are_hll_of_balls():
+    """After t iterations a vertex's counter is the HLL of the vertices that reach it in
+    at most t steps (register-wise max is the union of sketches)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import shortest_path
+    s, d = oracle.rmat_edges(6, 4, 9)
+    V = 1 << 6
+    g = oracle.CSC(V, s, d)
+    dist = shortest_path(coo_matrix((np.ones(s.size), (s, d)), shape=(V, V)), directed=True, unweighted=True)
+    for t in (0, 1, 3):
+        anfs, _, regs = oracle.hyperanf(g, bits=6, iters=t)
+        if len(anfs) <= t:  # stopped early: nothing changes any more
+            t = len(anfs) - 1
+        for v in range(V):
+            ball = np.flatnonzero(dist[:, v] <= t)
+            assert np.array_equal(regs[v], _hll_of(ball, 6)), (t, v)
+    # one element: one register set, so the small-range estimate m * ln(m / (m - 1))
+    r = _hll_of([7], 6)
+    assert abs(oracle.hll_estimate(r) - 64 * np.log(64 / 63)) < 1e-12
+
+
+def test_hyperanf_reference_test_average_distance():
+    """hyperanf/test/test_hyperanf.cc:133-219: raw_graph_7_7.csv undirected, bits 6, 20
+    iterations; the estimate is within 0.05 of the exact BFS average distance."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import shortest_path
+    s, d = oracle.read_edge_csv(golden("raw_graph_7_7.csv"))
+    V = int(max(s.max(), d.max())) + 1
+    anfs, avg, _ = oracle.hyperanf(oracle.CSC(V, s, d, symmetrize=True), bits=6, iters=20)
+    dist = shortest_path(coo_matrix((np.ones(s.size), (s, d)), shape=(V, V)), directed=False, unweighted=True)
+    real = 0.0
+    for root in range(V):  # number_of_friends + get_avg_distance_by_src of the test
+        dd = dist[root]
+        mx = int(dd[np.isfinite(dd)].max())
+        friends = np.array([np.sum(dd <= i) for i in range(mx + 1)], np.float64)
+        real += sum(i * (friends[i] - friends[i - 1]) for i in range(1, mx + 1)) / (friends[-1] - friends[0])
+    real /= V
+    assert abs(real - avg) <= 0.05, (real, avg)
