@@ -14,8 +14,9 @@
 // the anfs (hyperanf.hpp:58-67).
 //
 // Layout: counters [V][2^P] bytes, double-buffered (Jacobi).  Pull: one wave per
-// vertex; each lane owns 2^P / 64 consecutive registers (bytes, or 32-bit words of four
-// registers when P >= 8) and the wave walks the in-edges, one 2^P-byte row per edge.
+// vertex, registers handled as 32-bit words of four; 64- and 128-register counters take
+// 4 or 2 in-edges per wave step (a row is 16 or 32 lanes), larger ones one edge per step
+// with each lane owning 2^P / 256 words; several rows are loaded before they are merged.
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -68,10 +69,38 @@ __device__ __forceinline__ uint32_t byte_max4(uint32_t a, uint32_t b) {
   return r;
 }
 
+// Pull for counters of 64 or 128 registers (16 or 32 words): a wave takes EPS in-edges
+// per step (64 / EPS lanes per row, one word each), U steps in flight, and folds the
+// EPS partial rows together with shuffles at the end.  0 is the identity of max.
+template <int EPS, int U>
+__global__ __launch_bounds__(256) void k_hanf_pull_small(const uint64_t* __restrict__ col_ptr,
+                                                         const uint32_t* __restrict__ row_idx, uint32_t V,
+                                                         const uint32_t* __restrict__ cur, uint32_t* __restrict__ nxt) {
+  constexpr int WPR = 64 / EPS;  // words per counter
+  const int lane = threadIdx.x & 63, slot = lane / WPR, w = lane % WPR;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  for (uint64_t v = wave_id; v < V; v += nwaves) {
+    uint32_t acc = slot == 0 ? cur[v * WPR + w] : 0u;
+    const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    for (uint64_t k0 = b; k0 < e; k0 += (uint64_t)EPS * U) {
+      uint32_t val[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t k = k0 + (uint64_t)u * EPS + slot;
+        val[u] = k < e ? cur[(uint64_t)row_idx[k] * WPR + w] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = byte_max4(acc, val[u]);
+    }
+    for (int off = WPR; off < 64; off <<= 1) acc = byte_max4(acc, (uint32_t)__shfl_xor(acc, off, 64));
+    if (slot == 0) nxt[v * WPR + w] = acc;
+  }
+}
+
 // Pull with W 32-bit words (4 W registers) per lane and chunk, P >= 8: a counter row
 // of `row` words is handled in chunks of 64 W words (larger counters walk the in-edges
-// once per chunk).
-template <int W>
+// once per chunk), U in-edges in flight.
+template <int W, int U>
 __global__ __launch_bounds__(256) void k_hanf_pull_words(const uint64_t* __restrict__ col_ptr,
                                                          const uint32_t* __restrict__ row_idx, uint32_t V,
                                                          uint64_t row, const uint32_t* __restrict__ cur,
@@ -84,37 +113,23 @@ __global__ __launch_bounds__(256) void k_hanf_pull_words(const uint64_t* __restr
       uint32_t acc[W];
 #pragma unroll
       for (int j = 0; j < W; ++j) acc[j] = cur[v * row + c0 + (uint64_t)j * 64 + lane];
-      for (uint64_t k = b; k < e; ++k) {
-        const uint64_t u = row_idx[k];
+      for (uint64_t k0 = b; k0 < e; k0 += U) {
+        uint32_t val[U][W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) acc[j] = byte_max4(acc[j], cur[u * row + c0 + (uint64_t)j * 64 + lane]);
+        for (int u = 0; u < U; ++u) {
+          const bool ok = k0 + u < e;
+          const uint64_t urow = ok ? (uint64_t)row_idx[k0 + u] * row + c0 : 0;
+#pragma unroll
+          for (int j = 0; j < W; ++j) val[u][j] = ok ? cur[urow + (uint64_t)j * 64 + lane] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < W; ++j) acc[j] = byte_max4(acc[j], val[u][j]);
       }
 #pragma unroll
       for (int j = 0; j < W; ++j) nxt[v * row + c0 + (uint64_t)j * 64 + lane] = acc[j];
     }
-  }
-}
-
-// Pull with B bytes per lane (P = 6 or 7).
-template <int B>
-__global__ __launch_bounds__(256) void k_hanf_pull_bytes(const uint64_t* __restrict__ col_ptr,
-                                                         const uint32_t* __restrict__ row_idx, uint32_t V,
-                                                         const uint8_t* __restrict__ cur, uint8_t* __restrict__ nxt) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
-  constexpr uint64_t ROW = 64 * B;
-  for (uint64_t v = wave_id; v < V; v += nwaves) {
-    uint32_t acc[B];
-#pragma unroll
-    for (int j = 0; j < B; ++j) acc[j] = cur[v * ROW + (uint64_t)lane * B + j];
-    const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
-    for (uint64_t k = b; k < e; ++k) {
-      const uint64_t u = row_idx[k];
-#pragma unroll
-      for (int j = 0; j < B; ++j) acc[j] = max(acc[j], (uint32_t)cur[u * ROW + (uint64_t)lane * B + j]);
-    }
-#pragma unroll
-    for (int j = 0; j < B; ++j) nxt[v * ROW + (uint64_t)lane * B + j] = (uint8_t)acc[j];
   }
 }
 
@@ -201,15 +216,21 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
     };
     auto pull = [&](const uint8_t* cur, uint8_t* nxt) {
       switch (P) {
-        case 6: k_hanf_pull_bytes<1><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, cur, nxt); break;
-        case 7: k_hanf_pull_bytes<2><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, cur, nxt); break;
-#define HANF_W(PP, W)                                                                                       \
-  case PP:                                                                                                  \
-    k_hanf_pull_words<W><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, m / 4, (const uint32_t*)cur, \
-                                                   (uint32_t*)nxt);                                         \
+        case 6:
+          k_hanf_pull_small<4, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, (const uint32_t*)cur,
+                                                            (uint32_t*)nxt);
+          break;
+        case 7:
+          k_hanf_pull_small<2, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, (const uint32_t*)cur,
+                                                            (uint32_t*)nxt);
+          break;
+#define HANF_W(PP, W, U)                                                                                      \
+  case PP:                                                                                                    \
+    k_hanf_pull_words<W, U><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, m / 4,                  \
+                                                      (const uint32_t*)cur, (uint32_t*)nxt);                   \
     break;
-        HANF_W(8, 1) HANF_W(9, 2) HANF_W(10, 4) HANF_W(11, 8) HANF_W(12, 16) HANF_W(13, 16) HANF_W(14, 16)
-        HANF_W(15, 16) HANF_W(16, 16)
+        HANF_W(8, 1, 8) HANF_W(9, 2, 4) HANF_W(10, 4, 4) HANF_W(11, 8, 2) HANF_W(12, 16, 2) HANF_W(13, 16, 2)
+        HANF_W(14, 16, 2) HANF_W(15, 16, 2) HANF_W(16, 16, 2)
 #undef HANF_W
       }
       PE_HIP(hipGetLastError());
