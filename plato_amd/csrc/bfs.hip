@@ -201,14 +201,14 @@ static constexpr uint32_t PULL_THREADS = 1024;
 __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
     uint64_t nwords, uint32_t* __restrict__ level, const uint64_t* __restrict__ cur_bits, uint32_t L,
-    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ next_queue,
-    uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+    uint64_t* __restrict__ next_bits, uint64_t* __restrict__ heavy_bits, bfs_counters* __restrict__ cnt,
+    const uint32_t* __restrict__ od) {
   __shared__ uint64_t hot[PULL_HOT_WORDS];
   const uint64_t nh = nwords < (uint64_t)PULL_HOT_WORDS ? nwords : (uint64_t)PULL_HOT_WORDS;
   for (uint64_t i = threadIdx.x; i < nh; i += blockDim.x) hot[i] = cur_bits[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  unsigned long long active = 0;
+  unsigned long long active = 0, found = 0;
   for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
        word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     const uint64_t v = word * 64 + lane;
@@ -223,32 +223,30 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
       }
       is_heavy = !claimed && e > lim;
     }
+    // the next frontier and the vertices left for the wave-per-vertex pass as bitmap
+    // words (one store per wave, no atomics); counts are summed once per wave at the end
     const uint64_t m = __ballot(claimed);
-    if (lane == 0) next_bits[word] = m;
-    if (m) {
-      const unsigned long long pos = wave_append(m, &cnt->next_count);
-      if (claimed) {
-        level[v] = L;
-        next_queue[pos] = (uint32_t)v;
-        active += od[v];
-      }
-    }
     const uint64_t hm = __ballot(is_heavy);
-    if (hm) {
-      const unsigned long long pos = wave_append(hm, &cnt->heavy_count);
-      if (is_heavy) heavy[pos] = (uint32_t)v;
+    if (lane == 0) {
+      next_bits[word] = m;
+      heavy_bits[word] = hm;
+      found += (unsigned long long)__popcll(m);
+    }
+    if (claimed) {
+      level[v] = L;
+      active += od[v];
     }
   }
   for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
+  if (lane == 0 && found) atomicAdd(&cnt->next_count, found);
 }
 
 // second pull pass: one wave per heavy vertex, 64 in-edges per step, stop at the first hit
 __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
     uint32_t* __restrict__ level, const uint64_t* __restrict__ cur_bits, uint32_t L,
-    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ next_queue,
-    const uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt,
+    uint64_t* __restrict__ next_bits, const uint32_t* __restrict__ heavy, bfs_counters* __restrict__ cnt,
     const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave_id = grid_wave_id();
@@ -271,8 +269,7 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
     if (found && lane == 0) {
       level[v] = L;
       atomicOr((unsigned long long*)&next_bits[v >> 6], 1ull << (v & 63));
-      const unsigned long long pos = atomicAdd(&cnt->next_count, 1ull);
-      next_queue[pos] = v;
+      atomicAdd(&cnt->next_count, 1ull);
       active += od[v];
     }
   }
@@ -345,6 +342,7 @@ struct bfs_state {
   pe_graph_s* g;
   uint64_t nwords;
   dbuf<uint32_t> level, q0, q1, heavy;
+  dbuf<uint64_t> hb;  // vertices left for the wave-per-vertex pull pass, as a bitmap
   dbuf<uint64_t> b0, b1;
   dbuf<bfs_counters> cnt;
   dbuf<unsigned long long> red;
@@ -361,6 +359,7 @@ struct bfs_state {
     q0.alloc(ctx, V);
     q1.alloc(ctx, V);
     heavy.alloc(ctx, V);
+    hb.alloc(ctx, nwords);
     b0.alloc(ctx, nwords);
     b1.alloc(ctx, nwords);
     cnt.alloc(ctx, 1);
@@ -394,6 +393,7 @@ struct bfs_state {
     uint64_t* bc = b0.p;
     uint64_t* bn = b1.p;
     int epoch = 0;
+    bool queued = true;  // qc holds the current frontier as a queue
     while (n != 0) {
       const bool push = ((double)A / (double)Ep) < threshold;  // dualmode.hpp:279
       if ((uint32_t)epoch < max_epochs) {
@@ -407,6 +407,8 @@ struct bfs_state {
         PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
         k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(bn, nwords);
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
+        // a pull step leaves the frontier as a bitmap only: queue it (heavy_count as scratch)
+        if (!queued) frontier_compact(ctx, bc, nwords, g->V, qc, &cnt.p->heavy_count);
         if (A > LB_MEAN_DEGREE * n) {
           // high mean degree (hub frontiers): spread the edges evenly over the waves
           k_frontier_degrees<<<grid_for(n + 1, 256, 65536), 256, 0, s>>>(qc, n, g->csr_ptr(), deg.p);
@@ -424,10 +426,12 @@ struct bfs_state {
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
         k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,
-                                             qn, heavy.p, cnt.p, g->od.p);
-        k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, qn,
-                                                    heavy.p, cnt.p, g->od.p);
+                                             hb.p, cnt.p, g->od.p);
+        frontier_compact(ctx, hb.p, nwords, g->V, heavy.p, &cnt.p->heavy_count);
+        k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, heavy.p,
+                                                    cnt.p, g->od.p);
       }
+      queued = push;
       PE_HIP(hipGetLastError());
       PE_HIP(hipMemcpyAsync(hcnt, cnt.p, sizeof(bfs_counters), hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
@@ -524,11 +528,12 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
           st.cnt.p, g->od.p);
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
-      k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords,
-                                                                       st.level.p, current->p, L, next->p, st.q1.p,
-                                                                       st.heavy.p, st.cnt.p, g->od.p);
+      k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0,
+                   s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, st.level.p, current->p, L, next->p, st.hb.p,
+                        st.cnt.p, g->od.p);
+      frontier_compact(ctx, st.hb.p, nwords, g->V, st.heavy.p, &st.cnt.p->heavy_count);
       k_bfs_pull_heavy<<<persistent_grid(ctx, 1u << 20, 4), 256, 0, s>>>(
-          g->col_ptr.p, g->row_idx.p, st.level.p, current->p, L, next->p, st.q1.p, st.heavy.p, st.cnt.p, g->od.p);
+          g->col_ptr.p, g->row_idx.p, st.level.p, current->p, L, next->p, st.heavy.p, st.cnt.p, g->od.p);
     }
     PE_HIP(hipGetLastError());
     bfs_counters h{};
