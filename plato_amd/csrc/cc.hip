@@ -83,12 +83,13 @@ __global__ __launch_bounds__(256) void k_cc_push(const uint32_t* __restrict__ qu
 }
 
 // Pull: one lane per vertex, 64 consecutive vertices per wave (the wave owns one word of
-// the next bitmap); vertices with more than CC_PULL_CAP in-edges go to k_cc_pull_heavy.
+// the next bitmap); vertices with more than CC_PULL_CAP in-edges are marked in a second
+// bitmap, queued by compaction and finished by k_cc_pull_heavy.
 // Only v's own lane or wave writes label[v] in a pull epoch.
 __global__ __launch_bounds__(256) void k_cc_pull(const uint64_t* __restrict__ col_ptr,
                                                  const uint32_t* __restrict__ row_idx, uint32_t V, uint64_t nwords,
                                                  const uint32_t* __restrict__ perm, uint32_t* __restrict__ label,
-                                                 uint64_t* __restrict__ next_bits, uint32_t* __restrict__ heavy,
+                                                 uint64_t* __restrict__ next_bits, uint64_t* __restrict__ heavy_bits,
                                                  cc_counters* __restrict__ cnt) {
   const int lane = threadIdx.x & 63;
   unsigned long long act = 0;
@@ -109,16 +110,13 @@ __global__ __launch_bounds__(256) void k_cc_pull(const uint64_t* __restrict__ co
         }
       }
     }
-    const uint64_t m = __ballot(upd);
-    if (lane == 0) next_bits[word] = m;
-    act += upd ? 1 : 0;
-    const uint64_t hm = __ballot(is_heavy);
-    if (hm) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&cnt->heavy, (unsigned long long)__popcll(hm));
-      base = __shfl(base, 0, 64);
-      if (is_heavy) heavy[base + __popcll(hm & ((1ull << lane) - 1ull))] = (uint32_t)v;
+    // bitmap words, one store per wave: no per-word atomics on a shared counter
+    const uint64_t m = __ballot(upd), hm = __ballot(is_heavy);
+    if (lane == 0) {
+      next_bits[word] = m;
+      heavy_bits[word] = hm;
     }
+    act += upd ? 1 : 0;
   }
   cc_add(act, &cnt->activated);
 }
@@ -161,7 +159,7 @@ extern "C" int pe_connected_components(pe_graph_t* g, uint32_t* labels, uint8_t*
     const uint64_t nwords = ((uint64_t)V >> 6) + 1;
     const uint32_t Vs = std::max<uint32_t>(1, V);
     dbuf<uint32_t> label(ctx, Vs), queue(ctx, Vs), heavy(ctx, Vs);
-    dbuf<uint64_t> cur(ctx, nwords), nxt(ctx, nwords);
+    dbuf<uint64_t> cur(ctx, nwords), nxt(ctx, nwords), hbits(ctx, nwords);
     dbuf<cc_counters> cnt(ctx, 1);
     cc_counters* h = static_cast<cc_counters*>(ctx->host_pinned);
     const uint32_t* perm = g->relabeled ? g->perm.p : nullptr;
@@ -182,8 +180,9 @@ extern "C" int pe_connected_components(pe_graph_t* g, uint32_t* labels, uint8_t*
         frontier_compact(ctx, cur.p, nwords, V, queue.p, &cnt.p->queued);
         k_cc_push<<<push_grid, 256, 0, s>>>(queue.p, cnt.p, g->csr_ptr(), g->csr_idx(), label.p, nxt.p, cnt.p);
       } else {
-        k_cc_pull<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, nwords, perm, label.p, nxt.p, heavy.p,
+        k_cc_pull<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, nwords, perm, label.p, nxt.p, hbits.p,
                                             cnt.p);
+        frontier_compact(ctx, hbits.p, nwords, V, heavy.p, &cnt.p->heavy);
         k_cc_pull_heavy<<<push_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, perm, label.p, nxt.p, heavy.p, cnt.p);
       }
       PE_HIP(hipGetLastError());

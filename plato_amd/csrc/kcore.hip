@@ -83,43 +83,27 @@ __device__ __forceinline__ void kc_activate(uint64_t* next_bits, uint32_t u) {
   if (!(next_bits[u >> 6] & m)) atomicOr((unsigned long long*)&next_bits[u >> 6], (unsigned long long)m);
 }
 
-__device__ __forceinline__ void kc_count_modified(bool mod, kc_counters* cnt) {
-  const uint64_t b = __ballot(mod);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(&cnt->modified, (unsigned long long)__popcll(b));
-}
-
-// wave-aggregated append of the lanes with `take` to q[*n ...]
-__device__ __forceinline__ void kc_append(bool take, uint32_t v, uint32_t* q, unsigned long long* n) {
-  const uint64_t m = __ballot(take);
-  if (!m) return;
-  const int lane = threadIdx.x & 63;
-  unsigned long long base = 0;
-  if (lane == 0) base = atomicAdd(n, (unsigned long long)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (take) q[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
-}
-
 // One lane per queued vertex of at most KC_SMALL neighbours (byte counts suffice).
+// Grid-stride over the queue; longer adjacencies are marked in the big / huge bitmaps
+// (one atomic OR per marked vertex, on its own word) and queued by compaction after the
+// pass, and the modified count is summed once per wave: no per-wave atomics on a shared
+// counter.
 __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
     const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
     const uint32_t* __restrict__ idx, const uint32_t* __restrict__ est, uint32_t* __restrict__ core,
-    uint64_t* __restrict__ next_bits, uint32_t* __restrict__ big, uint32_t* __restrict__ huge,
+    uint64_t* __restrict__ next_bits, uint64_t* __restrict__ big_bits, uint64_t* __restrict__ huge_bits,
     kc_counters* __restrict__ cnt) {
   __shared__ uint8_t hist[KC_THREADS][KC_SMALL + 1];
   uint8_t* h = hist[threadIdx.x];
-  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  bool mod = false, is_big = false, is_huge = false;
-  uint32_t v = 0;
-  if (i < n) {
-    v = queue[i];
+  unsigned long long nmod = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = queue[i];
     const uint64_t b = ptr[v], e = ptr[v + 1];
     const uint64_t len = e - b;
     if (len == 0) {
       core[v] = 0;  // kcore.hpp:214-218
-    } else if (len > KC_HUGE) {
-      is_huge = true;
     } else if (len > KC_SMALL) {
-      is_big = true;
+      atomicOr((unsigned long long*)&(len > KC_HUGE ? huge_bits : big_bits)[v >> 6], 1ull << (v & 63));
     } else {
       const uint32_t cap = std::min<uint32_t>((uint32_t)len, est[v]);
       for (uint32_t j = 0; j <= cap; ++j) h[j] = 0;
@@ -137,14 +121,13 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
       }
       if (best && best < core[v]) {
         core[v] = best;
-        mod = true;
+        ++nmod;
         for (uint64_t k = b; k < e; ++k) kc_activate(next_bits, idx[k]);
       }
     }
   }
-  kc_append(is_big, v, big, &cnt->nbig);
-  kc_append(is_huge, v, huge, &cnt->nhuge);
-  kc_count_modified(mod, cnt);
+  for (int off = 32; off > 0; off >>= 1) nmod += __shfl_down(nmod, off, 64);
+  if ((threadIdx.x & 63) == 0 && nmod) atomicAdd(&cnt->modified, nmod);
 }
 
 // One wave per queued vertex with more than KC_SMALL neighbours.
@@ -321,7 +304,7 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
     const uint64_t nwords = ((uint64_t)V >> 6) + 1;
     const uint32_t Vs = std::max<uint32_t>(1, V);
     dbuf<uint32_t> core(ctx, Vs), est(ctx, Vs), queue(ctx, Vs), big(ctx, Vs), huge(ctx, Vs);
-    dbuf<uint64_t> cur(ctx, nwords), nxt(ctx, nwords);
+    dbuf<uint64_t> cur(ctx, nwords), nxt(ctx, nwords), big_bits(ctx, nwords), huge_bits(ctx, nwords);
     dbuf<kc_counters> cnt(ctx, 1);
     kc_counters* h = static_cast<kc_counters*>(ctx->host_pinned);
     PE_HIP(hipEventRecord(ctx->t0, s));
@@ -329,6 +312,7 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
     k_kc_fill<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(cur.p, nwords, V);  // active_current.fill()
     PE_HIP(hipGetLastError());
     const unsigned big_grid = persistent_grid(ctx, 1u << 20, 4);
+    const unsigned small_grid = persistent_grid(ctx, 1u << 20, 8);
     const unsigned huge_grid = persistent_grid(ctx, 1u << 20, 1);
     uint32_t epoch = 0;
     uint64_t total_mod = 0;
@@ -344,8 +328,12 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
       }
       k_kc_zero<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(nxt.p, nwords);
       if (n) {
-        k_kc_small<<<grid_for(n, KC_THREADS), KC_THREADS, 0, s>>>(queue.p, n, ptr, idx, est.p, core.p, nxt.p, big.p,
-                                                                  huge.p, cnt.p);
+        k_kc_zero<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(big_bits.p, nwords);
+        k_kc_zero<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(huge_bits.p, nwords);
+        k_kc_small<<<std::min<unsigned>(small_grid, grid_for(n, KC_THREADS)), KC_THREADS, 0, s>>>(
+            queue.p, n, ptr, idx, est.p, core.p, nxt.p, big_bits.p, huge_bits.p, cnt.p);
+        frontier_compact(ctx, big_bits.p, nwords, V, big.p, &cnt.p->nbig);
+        frontier_compact(ctx, huge_bits.p, nwords, V, huge.p, &cnt.p->nhuge);
         k_kc_big<<<big_grid, KC_THREADS, 0, s>>>(big.p, cnt.p, ptr, idx, est.p, core.p, nxt.p, cnt.p);
         k_kc_huge<<<huge_grid, KC_HTHREADS, 0, s>>>(huge.p, cnt.p, ptr, idx, est.p, core.p, nxt.p, cnt.p);
         k_kc_sync<<<grid_for(n, 256, 8192), 256, 0, s>>>(queue.p, n, core.p, est.p);
