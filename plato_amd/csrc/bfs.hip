@@ -4,19 +4,21 @@
 // with the engine's push/pull choice (plato/engine/dualmode.hpp:261-279):
 // push iff (double)A / (double)E' < threshold, A = sum of od over the frontier.
 //
-//  push (top-down): the frontier queue is walked one wave per vertex over its
-//       out-edges; an unvisited neighbour is claimed with an integer atomicMin
-//       on its level (the claim that the reference does with cas on parent,
-//       bfs.hpp:104-116), and appended to the next queue with one
-//       wave-aggregated atomic (ballot + mbcnt prefix).
+//  push (top-down): an unvisited out-neighbour of a frontier vertex is claimed with
+//       an integer atomicMin on its level (the claim that the reference does with
+//       cas on parent, bfs.hpp:104-116).  Frontiers with few active edges are walked
+//       one wave per vertex and append their claims to the next queue (one
+//       wave-aggregated atomic per step); larger ones are cut into equal ranges of
+//       edges, one per wave, and set their claims in the next bitmap only.
 //  pull (bottom-up): one lane per vertex, 64 consecutive vertices per wave, so
 //       a wave owns one 64-bit word of the next-frontier bitmap and writes it
 //       with a ballot, no atomics; an unvisited vertex scans its in-edges for
 //       the first neighbour in the current frontier bitmap and stops there
 //       (bfs.hpp:119-128).  Vertices with more than CAP in-edges that found
 //       nothing in their first CAP continue in a second kernel, one wave each.
-// Both steps produce the next frontier as a queue and as a bitmap, and count
-// its size and active edges on the device, so the next decision needs no scan.
+// Every step leaves the next frontier as a bitmap and counts its size and active
+// edges on the device, so the next decision needs no scan; a queue is built from the
+// bitmap by compaction only when a top-down step needs one and none was appended.
 #include <algorithm>
 #include <memory>
 
@@ -81,30 +83,6 @@ __device__ __forceinline__ unsigned long long wave_append(uint64_t mask,
   return base + wave_prefix(mask);
 }
 
-// Claims the unvisited target of out-edge i at level L and appends it to the next
-// queue (wave-aggregated; the lanes of the wave call it together).
-__device__ __forceinline__ void push_edge(uint64_t i, bool live, const uint32_t* __restrict__ idx,
-                                          uint32_t* __restrict__ level, uint32_t L,
-                                          uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
-                                          bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od,
-                                          unsigned long long& active) {
-  bool claimed = false;
-  uint32_t w = 0;
-  if (live) {
-    w = idx[i];
-    if (level[w] == UNVIS) claimed = (atomicMin(&level[w], L) == UNVIS);
-  }
-  const uint64_t m = __ballot(claimed);
-  if (m) {
-    const unsigned long long pos = wave_append(m, &cnt->next_count);
-    if (claimed) {
-      next_queue[pos] = w;
-      atomicOr((unsigned long long*)&next_bits[w >> 6], 1ull << (w & 63));
-      active += od[w];
-    }
-  }
-}
-
 // Out-degree of every frontier vertex (deg[n] = 0), scanned into edge offsets.
 __global__ void k_frontier_degrees(const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
                                    uint64_t* __restrict__ deg) {
@@ -112,54 +90,108 @@ __global__ void k_frontier_degrees(const uint32_t* __restrict__ queue, uint64_t 
     deg[q] = q < n ? ptr[queue[q] + 1] - ptr[queue[q]] : 0ull;
 }
 
-// Load-balanced top-down step for frontiers of high mean degree: the frontier's edges
-// (offsets pref[0..n], pref[n] = total) are cut into chunks of 256 edges, one chunk
-// per wave at a time, so a hub's edges spread over many waves instead of one.
-__global__ __launch_bounds__(256) void k_bfs_push_lb(
+// Edge-balanced top-down step.  The frontier's out-edges, numbered 0..T-1 through the
+// exclusive scan pref[0..n] of the queued vertices' degrees (pref[n] = T), are cut into
+// one contiguous range per wave, so a hub's edges spread over many waves and a run of
+// low-degree vertices fills every lane.  A wave finds the vertex of its first edge with
+// a 64-way search, then keeps a window of 64 queued vertices in registers (lane t: the
+// first edge number of vertex wbase+t and its CSR offset) and resolves each lane's
+// vertex with a 6-step shuffle search; the window moves only when the step's edges run
+// past it.  Claims are set in the next bitmap and counted once per wave (no queue).
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(256) void k_bfs_push_eb(
     const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ pref,
     const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
-    uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits, bfs_counters* __restrict__ cnt,
-    const uint32_t* __restrict__ od) {
+    uint64_t* __restrict__ next_bits, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
   const uint64_t T = pref[n];
-  unsigned long long active = 0;
-  for (uint64_t c = wave_id; c * 256 < T; c += nwaves) {
-    const uint64_t e0 = c * 256;
-    uint64_t lo = 0, hi = n - 1;  // last q with pref[q] <= e0
+  const uint64_t per = ((T + nwaves - 1) / nwaves + 63) & ~63ull;
+  const uint64_t e0 = wave_id * per;
+  const uint64_t e1 = std::min<uint64_t>(T, e0 + per);
+  unsigned long long active = 0, found = 0;
+  if (e0 < e1) {
+    // the vertex of edge e0: the last q in [0, n) with pref[q] <= e0 (pref[0] = 0)
+    uint64_t lo = 0, hi = n - 1;
     while (lo < hi) {
-      const uint64_t mid = (lo + hi + 1) >> 1;
-      if (pref[mid] <= e0) lo = mid;
-      else hi = mid - 1;
+      const uint64_t step = (hi - lo + 64) >> 6;
+      const uint64_t p = lo + (uint64_t)(lane + 1) * step;
+      const bool ok = p <= hi && pref[p] <= e0;
+      const uint64_t nlo = lo + (uint64_t)__popcll(__ballot(ok)) * step;
+      hi = std::min<uint64_t>(hi, nlo + step - 1);
+      lo = nlo;
     }
-    uint64_t q = lo;
-#pragma unroll 1
-    for (int k = 0; k < 4; ++k) {
-      const uint64_t j = e0 + (uint64_t)k * 64 + lane;
-      const bool live = j < T;
-      uint64_t i = 0;
-      if (live) {
-        while (pref[q + 1] <= j) ++q;
-        i = ptr[queue[q]] + (j - pref[q]);
+    uint64_t wbase = lo;
+    uint64_t wpref = 0, wsrc = 0, wlast = 0;  // pref[wbase+lane] (T past n), ptr[queue[.]], pref[wbase+64]
+    auto load_window = [&] {
+      const uint64_t q = wbase + lane;
+      wpref = q < n ? pref[q] : T;
+      wsrc = q < n ? ptr[queue[q]] : 0ull;
+      wlast = pref[std::min<uint64_t>(wbase + 64, n)];
+    };
+    load_window();
+    for (uint64_t j0 = e0; j0 < e1; j0 += 64) {
+      if (j0 + 63 >= wlast && wlast < T) {
+        // move the window to the vertex of edge j0 (in steps of up to 63 slots while j0
+        // lies past it; zero-degree vertices can make a window hold fewer than 64 edges)
+        do {
+          const int t0 = __popcll(__ballot(wpref <= j0)) - 1;
+          wbase += (uint64_t)t0;
+          load_window();
+        } while (j0 >= wlast);
       }
-      push_edge(i, live, idx, level, L, next_queue, next_bits, cnt, od, active);
+      const uint64_t j = j0 + lane;
+      // window slot of edge j: the last t with wpref[t] <= j (all lanes take part)
+      int t = 0;
+#pragma unroll
+      for (int sft = 32; sft > 0; sft >>= 1) {
+        const uint64_t pv = shfl64(wpref, t + sft);
+        if (pv <= j) t += sft;
+      }
+      const uint64_t tp = shfl64(wpref, t), ts = shfl64(wsrc, t);
+      bool claimed = false;
+      uint32_t w = 0;
+      if (j < e1) {
+        uint64_t i;
+        if (j < wlast) {
+          i = ts + (j - tp);
+        } else {  // past the window (zero-degree vertices in it): walk on
+          uint64_t q = wbase + 63;
+          while (pref[q + 1] <= j) ++q;
+          i = ptr[queue[q]] + (j - pref[q]);
+        }
+        w = idx[i];
+        if (level[w] == UNVIS) claimed = (atomicMin(&level[w], L) == UNVIS);
+      }
+      if (claimed) {
+        atomicOr((unsigned long long*)&next_bits[w >> 6], 1ull << (w & 63));
+        active += od[w];
+        found += 1;
+      }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
+  for (int off = 32; off > 0; off >>= 1) {
+    active += __shfl_down(active, off, 64);
+    found += __shfl_down(found, off, 64);
+  }
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
+  if (lane == 0 && found) atomicAdd(&cnt->next_count, found);
 }
 
 __global__ __launch_bounds__(256) void k_bfs_push(
-    const uint32_t* __restrict__ queue, uint64_t n_host, const unsigned long long* __restrict__ n_dev,
-    const uint64_t* __restrict__ ptr,
+    const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
     const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
     uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
     bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
-  const uint64_t n = n_dev ? (uint64_t)*n_dev : n_host;
   unsigned long long active = 0;
   for (uint64_t q = wave_id; q < n; q += nwaves) {
     const uint32_t v = queue[q];
@@ -335,8 +367,12 @@ __global__ void k_reached_edges(const uint32_t* __restrict__ level, const uint32
 }
 
 // Device state of one BFS engine, reused across roots.
-// frontiers whose active edges exceed this many per vertex use the load-balanced push
-static constexpr uint64_t LB_MEAN_DEGREE = 32;
+// top-down levels with at most this many active edges append to a queue (one wave per
+// vertex); larger ones run edge-balanced and leave the next frontier as a bitmap
+#ifndef PUSH_APPEND_EDGES
+#define PUSH_APPEND_EDGES 4096
+#endif
+static constexpr uint64_t EB_WAVE_EDGES = 512;  // fewest edges an edge-balanced wave takes
 
 struct bfs_state {
   pe_graph_s* g;
@@ -370,6 +406,30 @@ struct bfs_state {
                                    rocprim::plus<uint64_t>(), ctx->stream));
     scan_tmp.alloc(ctx, std::max<size_t>(1, scan_bytes));
     hcnt = static_cast<bfs_counters*>(ctx->host_pinned);
+  }
+
+  // One top-down level from the queue qc (n vertices, A active edges; A = 0 if unknown)
+  // at level L.  Returns whether the next frontier was queued into qn as well as set in bn.
+  bool push_level(const uint32_t* qc, uint64_t n, uint64_t A, uint32_t L, uint32_t* qn, uint64_t* bn) {
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
+    if (A != 0 && A <= PUSH_APPEND_EDGES) {
+      const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
+      k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
+                                      g->od.p);
+      PE_HIP(hipGetLastError());
+      return true;
+    }
+    k_frontier_degrees<<<grid_for(n + 1, 256, 65536), 256, 0, s>>>(qc, n, g->csr_ptr(), deg.p);
+    size_t tb = scan_bytes;
+    PE_HIP(rocprim::exclusive_scan(scan_tmp.p, tb, deg.p, pref.p, (uint64_t)0, (size_t)n + 1,
+                                   rocprim::plus<uint64_t>(), s));
+    const uint64_t waves = A ? std::max<uint64_t>(1, (A + EB_WAVE_EDGES - 1) / EB_WAVE_EDGES) : (uint64_t)push_grid * 4;
+    const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (waves + 3) / 4);
+    k_bfs_push_eb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), level.p, L, bn, cnt.p, g->od.p);
+    PE_HIP(hipGetLastError());
+    return false;
   }
 
   int run(uint32_t root, double threshold, uint8_t* modes, uint64_t* frontier, uint64_t* actv,
@@ -409,20 +469,7 @@ struct bfs_state {
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
         // a pull step leaves the frontier as a bitmap only: queue it (heavy_count as scratch)
         if (!queued) frontier_compact(ctx, bc, nwords, g->V, qc, &cnt.p->heavy_count);
-        if (A > LB_MEAN_DEGREE * n) {
-          // high mean degree (hub frontiers): spread the edges evenly over the waves
-          k_frontier_degrees<<<grid_for(n + 1, 256, 65536), 256, 0, s>>>(qc, n, g->csr_ptr(), deg.p);
-          size_t tb = scan_bytes;
-          PE_HIP(rocprim::exclusive_scan(scan_tmp.p, tb, deg.p, pref.p, (uint64_t)0, (size_t)n + 1,
-                                         rocprim::plus<uint64_t>(), s));
-          const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, std::max<uint64_t>(1, (A + 1023) / 1024));
-          k_bfs_push_lb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
-                                             g->od.p);
-        } else {
-          const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
-          k_bfs_push<<<grid, 256, 0, s>>>(qc, n, nullptr, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
-                                          g->od.p);
-        }
+        queued = push_level(qc, n, A, L, qn, bn);
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
         k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,
@@ -430,8 +477,8 @@ struct bfs_state {
         frontier_compact(ctx, hb.p, nwords, g->V, heavy.p, &cnt.p->heavy_count);
         k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, heavy.p,
                                                     cnt.p, g->od.p);
+        queued = false;
       }
-      queued = push;
       PE_HIP(hipGetLastError());
       PE_HIP(hipMemcpyAsync(hcnt, cnt.p, sizeof(bfs_counters), hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
@@ -522,10 +569,12 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
       k_frontier_compact<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(current->p, nwords, g->V, st.q0.p,
                                                                      &st.cnt.p->heavy_count);
       k_zero_words<<<grid_for(next->words, 256, 4096), 256, 0, s>>>(next->p, next->words);
+      PE_HIP(hipGetLastError());
+      bfs_counters h{};
+      PE_HIP(hipMemcpyAsync(&h, st.cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+      PE_HIP(hipStreamSynchronize(s));
       ktime_scope kt(ctx, PE_KT_BFS_PUSH);
-      k_bfs_push<<<persistent_grid(ctx, 1u << 20, 8), 256, 0, s>>>(
-          st.q0.p, 0, &st.cnt.p->heavy_count, g->csr_ptr(), g->csr_idx(), st.level.p, L, st.q1.p, next->p,
-          st.cnt.p, g->od.p);
+      st.push_level(st.q0.p, h.heavy_count, 0, L, st.q1.p, next->p);
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
       k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0,
