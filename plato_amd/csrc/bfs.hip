@@ -45,10 +45,39 @@ __device__ __forceinline__ unsigned wave_prefix(uint64_t mask) {
   return __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1ull));
 }
 
-__global__ void k_bfs_init(uint32_t* __restrict__ level, uint32_t V, uint32_t root) {
+// levels UNVIS but the root's 0; the visited bitmap empty (the root is in the frontier)
+__global__ void k_bfs_init(uint32_t* __restrict__ level, uint64_t* __restrict__ vis, uint32_t V, uint64_t nwords,
+                           uint32_t root) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x)
     level[v] = (v == root) ? 0u : UNVIS;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nwords;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    vis[i] = 0ull;
+}
+
+// Before a top-down step: the current frontier joins the visited bitmap, the next is cleared.
+__global__ void k_bfs_push_prep(uint64_t* __restrict__ vis, const uint64_t* __restrict__ cur, uint64_t nwords,
+                                uint64_t* __restrict__ next, uint64_t next_words) {
+  const uint64_t n = nwords > next_words ? nwords : next_words;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < nwords) {
+      const uint64_t c = cur[i];
+      if (c) vis[i] |= c;
+    }
+    if (i < next_words) next[i] = 0ull;
+  }
+}
+
+// Claims w at level L unless it was visited before this step.  The visited bitmap (one
+// bit per vertex, read-only during the step, small enough to stay in L2) rules out
+// nearly every edge of a late top-down step without touching the level array; a
+// target not in it is claimed with an atomicMin on its level, as the reference claims
+// with a cas on the parent (bfs.hpp:104-116).
+__device__ __forceinline__ bool bfs_claim(uint32_t w, const uint64_t* __restrict__ vis, uint32_t* __restrict__ level,
+                                          uint32_t L) {
+  if ((vis[w >> 6] >> (w & 63)) & 1ull) return false;
+  return level[w] == UNVIS && atomicMin(&level[w], L) == UNVIS;
 }
 
 __global__ void k_bfs_seed(uint32_t* __restrict__ queue, uint64_t* __restrict__ bitmap, uint32_t root,
@@ -106,8 +135,9 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
 
 __global__ __launch_bounds__(256) void k_bfs_push_eb(
     const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ pref,
-    const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
-    uint64_t* __restrict__ next_bits, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+    const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx, const uint64_t* __restrict__ vis,
+    uint32_t* __restrict__ level, uint32_t L, uint64_t* __restrict__ next_bits, bfs_counters* __restrict__ cnt,
+    const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
@@ -167,7 +197,7 @@ __global__ __launch_bounds__(256) void k_bfs_push_eb(
           i = ptr[queue[q]] + (j - pref[q]);
         }
         w = idx[i];
-        if (level[w] == UNVIS) claimed = (atomicMin(&level[w], L) == UNVIS);
+        claimed = bfs_claim(w, vis, level, L);
       }
       if (claimed) {
         atomicOr((unsigned long long*)&next_bits[w >> 6], 1ull << (w & 63));
@@ -176,17 +206,12 @@ __global__ __launch_bounds__(256) void k_bfs_push_eb(
       }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    active += __shfl_down(active, off, 64);
-    found += __shfl_down(found, off, 64);
-  }
-  if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
-  if (lane == 0 && found) atomicAdd(&cnt->next_count, found);
+  block_add2(active, found, &cnt->next_active, &cnt->next_count);
 }
 
 __global__ __launch_bounds__(256) void k_bfs_push(
     const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
-    const uint32_t* __restrict__ idx, uint32_t* __restrict__ level, uint32_t L,
+    const uint32_t* __restrict__ idx, const uint64_t* __restrict__ vis, uint32_t* __restrict__ level, uint32_t L,
     uint32_t* __restrict__ next_queue, uint64_t* __restrict__ next_bits,
     bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   const int lane = threadIdx.x & 63;
@@ -202,7 +227,7 @@ __global__ __launch_bounds__(256) void k_bfs_push(
       uint32_t w = 0;
       if (i < e) {
         w = idx[i];
-        if (level[w] == UNVIS) claimed = (atomicMin(&level[w], L) == UNVIS);
+        claimed = bfs_claim(w, vis, level, L);
       }
       const uint64_t m = __ballot(claimed);
       if (m) {
@@ -232,9 +257,9 @@ static constexpr uint32_t PULL_THREADS = 1024;
 
 __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
-    uint64_t nwords, uint32_t* __restrict__ level, const uint64_t* __restrict__ cur_bits, uint32_t L,
-    uint64_t* __restrict__ next_bits, uint64_t* __restrict__ heavy_bits, bfs_counters* __restrict__ cnt,
-    const uint32_t* __restrict__ od) {
+    uint64_t nwords, uint64_t* __restrict__ vis, uint32_t* __restrict__ level,
+    const uint64_t* __restrict__ cur_bits, uint32_t L, uint64_t* __restrict__ next_bits,
+    uint64_t* __restrict__ heavy_bits, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   __shared__ uint64_t hot[PULL_HOT_WORDS];
   const uint64_t nh = nwords < (uint64_t)PULL_HOT_WORDS ? nwords : (uint64_t)PULL_HOT_WORDS;
   for (uint64_t i = threadIdx.x; i < nh; i += blockDim.x) hot[i] = cur_bits[i];
@@ -244,8 +269,12 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
   for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
        word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     const uint64_t v = word * 64 + lane;
+    // visited before this step: the visited bitmap and the current frontier; the frontier
+    // joins the bitmap here (this wave alone owns the word in this kernel)
+    const uint64_t cw = word < nh ? hot[word] : cur_bits[word];
+    const uint64_t seen = vis[word] | cw;
     bool claimed = false, is_heavy = false;
-    if (v < V && level[v] == UNVIS) {
+    if (v < V && !((seen >> lane) & 1ull)) {
       const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
       const uint64_t lim = (e - b > PULL_CAP) ? b + PULL_CAP : e;
       for (uint64_t i = b; i < lim; ++i) {
@@ -262,6 +291,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     if (lane == 0) {
       next_bits[word] = m;
       heavy_bits[word] = hm;
+      if (cw) vis[word] = seen;
       found += (unsigned long long)__popcll(m);
     }
     if (claimed) {
@@ -269,9 +299,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
       active += od[v];
     }
   }
-  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
-  if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
-  if (lane == 0 && found) atomicAdd(&cnt->next_count, found);
+  block_add2(active, found, &cnt->next_active, &cnt->next_count);
 }
 
 // second pull pass: one wave per heavy vertex, 64 in-edges per step, stop at the first hit
@@ -348,22 +376,14 @@ void frontier_compact(pe_ctx_s* ctx, const uint64_t* words, uint64_t nwords, uin
   PE_HIP(hipGetLastError());
 }
 
-__global__ void k_reached_edges(const uint32_t* __restrict__ level, const uint32_t* __restrict__ od,
-                                uint32_t V, unsigned long long* __restrict__ out,
-                                unsigned int* __restrict__ visited) {
-  unsigned long long acc = 0;
-  unsigned int cntv = 0;
+__global__ __launch_bounds__(256) void k_reached_edges(const uint32_t* __restrict__ level,
+                                                       const uint32_t* __restrict__ od, uint32_t V,
+                                                       unsigned long long* __restrict__ out) {
+  unsigned long long acc = 0, cntv = 0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x)
     if (level[v] != UNVIS) { acc += od[v]; ++cntv; }
-  for (int off = 32; off > 0; off >>= 1) {
-    acc += __shfl_down(acc, off, 64);
-    cntv += __shfl_down(cntv, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (acc) atomicAdd(out, acc);
-    if (cntv) atomicAdd(visited, cntv);
-  }
+  block_add2(acc, cntv, out, out + 1);
 }
 
 // Device state of one BFS engine, reused across roots.
@@ -380,6 +400,9 @@ struct bfs_state {
   dbuf<uint32_t> level, q0, q1, heavy;
   dbuf<uint64_t> hb;  // vertices left for the wave-per-vertex pull pass, as a bitmap
   dbuf<uint64_t> b0, b1;
+  // vertices visited before the current frontier (with it: all visited so far);
+  // the frontier joins it at the start of each step
+  dbuf<uint64_t> vis;
   dbuf<bfs_counters> cnt;
   dbuf<unsigned long long> red;
   dbuf<uint64_t> deg, pref;  // frontier degrees and their scan (load-balanced push)
@@ -398,6 +421,7 @@ struct bfs_state {
     hb.alloc(ctx, nwords);
     b0.alloc(ctx, nwords);
     b1.alloc(ctx, nwords);
+    vis.alloc(ctx, nwords);
     cnt.alloc(ctx, 1);
     red.alloc(ctx, 2);
     deg.alloc(ctx, (uint64_t)V + 1);
@@ -416,7 +440,7 @@ struct bfs_state {
     const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
     if (A != 0 && A <= PUSH_APPEND_EDGES) {
       const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
-      k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), level.p, L, qn, bn, cnt.p,
+      k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, qn, bn, cnt.p,
                                       g->od.p);
       PE_HIP(hipGetLastError());
       return true;
@@ -427,7 +451,8 @@ struct bfs_state {
                                    rocprim::plus<uint64_t>(), s));
     const uint64_t waves = A ? std::max<uint64_t>(1, (A + EB_WAVE_EDGES - 1) / EB_WAVE_EDGES) : (uint64_t)push_grid * 4;
     const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (waves + 3) / 4);
-    k_bfs_push_eb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), level.p, L, bn, cnt.p, g->od.p);
+    k_bfs_push_eb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, bn, cnt.p,
+                                       g->od.p);
     PE_HIP(hipGetLastError());
     return false;
   }
@@ -441,7 +466,7 @@ struct bfs_state {
     const unsigned pull_grid = persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU);
     const unsigned heavy_grid = persistent_grid(ctx, 1u << 20, 4);
     PE_HIP(hipEventRecord(ctx->t0, s));
-    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, g->V, root);
+    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, vis.p, g->V, nwords, root);
     k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(b0.p, nwords);
     k_bfs_seed<<<1, 1, 0, s>>>(q0.p, b0.p, root, cnt.p, g->od.p);
     PE_HIP(hipGetLastError());
@@ -465,15 +490,15 @@ struct bfs_state {
       k_bfs_reset<<<1, 1, 0, s>>>(cnt.p);
       if (push) {
         PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
-        k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(bn, nwords);
+        k_bfs_push_prep<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(vis.p, bc, nwords, bn, nwords);
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
         // a pull step leaves the frontier as a bitmap only: queue it (heavy_count as scratch)
         if (!queued) frontier_compact(ctx, bc, nwords, g->V, qc, &cnt.p->heavy_count);
         queued = push_level(qc, n, A, L, qn, bn);
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
-        k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, level.p, bc, L, bn,
-                                             hb.p, cnt.p, g->od.p);
+        k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, vis.p, level.p, bc,
+                                                      L, bn, hb.p, cnt.p, g->od.p);
         frontier_compact(ctx, hb.p, nwords, g->V, heavy.p, &cnt.p->heavy_count);
         k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, heavy.p,
                                                     cnt.p, g->od.p);
@@ -541,7 +566,7 @@ int pe_bfs_reset(pe_bfs_t* b, uint32_t root, pe_bitmap_t* frontier) {
     uint32_t iroot = root;
     to_internal_ids(g, &root, 1, &iroot);
     hipStream_t s = g->ctx->stream;
-    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(b->st->level.p, g->V, iroot);
+    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(b->st->level.p, b->st->vis.p, g->V, b->st->nwords, iroot);
     k_zero_words<<<grid_for(frontier->words, 256, 4096), 256, 0, s>>>(frontier->p, frontier->words);
     k_bfs_seed<<<1, 1, 0, s>>>(b->st->q0.p, frontier->p, iroot, b->st->cnt.p, g->od.p);
     PE_HIP(hipGetLastError());
@@ -568,7 +593,8 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
       // compact the frontier bitmap into a queue (count in heavy_count, reused as scratch)
       k_frontier_compact<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(current->p, nwords, g->V, st.q0.p,
                                                                      &st.cnt.p->heavy_count);
-      k_zero_words<<<grid_for(next->words, 256, 4096), 256, 0, s>>>(next->p, next->words);
+      k_bfs_push_prep<<<grid_for(std::max(nwords, next->words), 256, 4096), 256, 0, s>>>(st.vis.p, current->p, nwords,
+                                                                                          next->p, next->words);
       PE_HIP(hipGetLastError());
       bfs_counters h{};
       PE_HIP(hipMemcpyAsync(&h, st.cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -578,8 +604,8 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
       k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0,
-                   s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, st.level.p, current->p, L, next->p, st.hb.p,
-                        st.cnt.p, g->od.p);
+                   s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, st.vis.p, st.level.p, current->p, L, next->p,
+                        st.hb.p, st.cnt.p, g->od.p);
       frontier_compact(ctx, st.hb.p, nwords, g->V, st.heavy.p, &st.cnt.p->heavy_count);
       k_bfs_pull_heavy<<<persistent_grid(ctx, 1u << 20, 4), 256, 0, s>>>(
           g->col_ptr.p, g->row_idx.p, st.level.p, current->p, L, next->p, st.heavy.p, st.cnt.p, g->od.p);
@@ -612,8 +638,7 @@ int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint
     hipStream_t s = g->ctx->stream;
     if (levels) download_external(g, st.level.p, levels, 4);
     PE_HIP(hipMemsetAsync(st.red.p, 0, 16, s));
-    k_reached_edges<<<grid_for(g->V, 256, 4096), 256, 0, s>>>(
-        st.level.p, g->od.p, g->V, st.red.p, reinterpret_cast<unsigned int*>(st.red.p + 1));
+    k_reached_edges<<<grid_for(g->V, 256, 1024), 256, 0, s>>>(st.level.p, g->od.p, g->V, st.red.p);
     PE_HIP(hipGetLastError());
     unsigned long long r[2] = {0, 0};
     PE_HIP(hipMemcpyAsync(r, st.red.p, 16, hipMemcpyDeviceToHost, s));

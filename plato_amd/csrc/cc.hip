@@ -50,9 +50,9 @@ __global__ void k_cc_reset(cc_counters* c) {
   c->heavy = 0;
 }
 
+// one atomic per block; every thread of the block calls it
 __device__ __forceinline__ void cc_add(unsigned long long act, unsigned long long* dst) {
-  for (int off = 32; off > 0; off >>= 1) act += __shfl_down(act, off, 64);
-  if ((threadIdx.x & 63) == 0 && act) atomicAdd(dst, act);
+  block_add2(act, 0, dst, dst);
 }
 
 // Push: one wave per frontier vertex u over its out-edges; a neighbour whose label is

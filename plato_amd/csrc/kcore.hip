@@ -126,8 +126,7 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
       }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) nmod += __shfl_down(nmod, off, 64);
-  if ((threadIdx.x & 63) == 0 && nmod) atomicAdd(&cnt->modified, nmod);
+  block_add2(nmod, 0, &cnt->modified, &cnt->modified);
 }
 
 // One wave per queued vertex with more than KC_SMALL neighbours.

@@ -134,6 +134,34 @@ __device__ __forceinline__ uint64_t grid_wave_id() {
 }
 __device__ __forceinline__ uint64_t grid_waves() { return ((uint64_t)gridDim.x * blockDim.x) >> 6; }
 
+// Adds a and b over the block into *da and *db with one atomic each from thread 0 (none
+// for a zero sum).  Every thread of the block must call it; blockDim.x is a multiple of
+// 64, at most 1024.  Counters that every wave of a large grid bumps are serialized in
+// the memory-side atomic unit, so kernels sum per block first.
+__device__ __forceinline__ void block_add2(unsigned long long a, unsigned long long b, unsigned long long* da,
+                                           unsigned long long* db) {
+  __shared__ unsigned long long part[2][16];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[0][wave] = a;
+    part[1][wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sa = 0, sb = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      sa += part[0][w];
+      sb += part[1][w];
+    }
+    if (sa) atomicAdd(da, sa);
+    if (sb) atomicAdd(db, sb);
+  }
+}
+
 }  // namespace pe
 
 // ---- binned pull layout (pull_binned.hip) -------------------------------------------
