@@ -403,12 +403,11 @@ struct bfs_state {
   // vertices visited before the current frontier (with it: all visited so far);
   // the frontier joins it at the start of each step
   dbuf<uint64_t> vis;
-  dbuf<bfs_counters> cnt;
+  dbuf<bfs_counters> cnt;  // two sets: consecutive steps alternate
   dbuf<unsigned long long> red;
   dbuf<uint64_t> deg, pref;  // frontier degrees and their scan (load-balanced push)
   dbuf<uint8_t> scan_tmp;
   size_t scan_bytes = 0;
-  bfs_counters* hcnt;  // pinned host mirror
 
   explicit bfs_state(pe_graph_s* graph) : g(graph) {
     pe_ctx_s* ctx = g->ctx;
@@ -422,25 +421,25 @@ struct bfs_state {
     b0.alloc(ctx, nwords);
     b1.alloc(ctx, nwords);
     vis.alloc(ctx, nwords);
-    cnt.alloc(ctx, 1);
+    cnt.alloc(ctx, 2);
     red.alloc(ctx, 2);
     deg.alloc(ctx, (uint64_t)V + 1);
     pref.alloc(ctx, (uint64_t)V + 1);
     PE_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, deg.p, pref.p, (uint64_t)0, (size_t)V + 1,
                                    rocprim::plus<uint64_t>(), ctx->stream));
     scan_tmp.alloc(ctx, std::max<size_t>(1, scan_bytes));
-    hcnt = static_cast<bfs_counters*>(ctx->host_pinned);
   }
 
   // One top-down level from the queue qc (n vertices, A active edges; A = 0 if unknown)
   // at level L.  Returns whether the next frontier was queued into qn as well as set in bn.
-  bool push_level(const uint32_t* qc, uint64_t n, uint64_t A, uint32_t L, uint32_t* qn, uint64_t* bn) {
+  bool push_level(const uint32_t* qc, uint64_t n, uint64_t A, uint32_t L, uint32_t* qn, uint64_t* bn,
+                  bfs_counters* c) {
     pe_ctx_s* ctx = g->ctx;
     hipStream_t s = ctx->stream;
     const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
     if (A != 0 && A <= PUSH_APPEND_EDGES) {
       const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (n + 3) / 4);
-      k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, qn, bn, cnt.p,
+      k_bfs_push<<<grid, 256, 0, s>>>(qc, n, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, qn, bn, c,
                                       g->od.p);
       PE_HIP(hipGetLastError());
       return true;
@@ -451,7 +450,7 @@ struct bfs_state {
                                    rocprim::plus<uint64_t>(), s));
     const uint64_t waves = A ? std::max<uint64_t>(1, (A + EB_WAVE_EDGES - 1) / EB_WAVE_EDGES) : (uint64_t)push_grid * 4;
     const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (waves + 3) / 4);
-    k_bfs_push_eb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, bn, cnt.p,
+    k_bfs_push_eb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, bn, c,
                                        g->od.p);
     PE_HIP(hipGetLastError());
     return false;
@@ -468,11 +467,16 @@ struct bfs_state {
     PE_HIP(hipEventRecord(ctx->t0, s));
     k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, vis.p, g->V, nwords, root);
     k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(b0.p, nwords);
-    k_bfs_seed<<<1, 1, 0, s>>>(q0.p, b0.p, root, cnt.p, g->od.p);
+    // counters: step e adds into set (e + 1) & 1 (the seed into set 0); each step's
+    // counters are mailed to the host, which zeroes the next step's set on the way
+    auto set = [&](int e) { return cnt.p + ((e + 1) & 1); };
+    constexpr uint32_t CW = sizeof(bfs_counters) / 8;
+    k_bfs_seed<<<1, 1, 0, s>>>(q0.p, b0.p, root, set(-1), g->od.p);
     PE_HIP(hipGetLastError());
-    PE_HIP(hipMemcpyAsync(hcnt, cnt.p, sizeof(bfs_counters), hipMemcpyDeviceToHost, s));
-    PE_HIP(hipStreamSynchronize(s));
-    uint64_t n = hcnt->next_count, A = hcnt->next_active;
+    const bfs_counters* h = reinterpret_cast<const bfs_counters*>(
+        mail_post(ctx, reinterpret_cast<const unsigned long long*>(set(-1)), CW,
+                  reinterpret_cast<unsigned long long*>(set(0)), CW));
+    uint64_t n = h->next_count, A = h->next_active;
     uint32_t* qc = q0.p;
     uint32_t* qn = q1.p;
     uint64_t* bc = b0.p;
@@ -487,28 +491,28 @@ struct bfs_state {
         if (actv) actv[epoch] = A;
       }
       const uint32_t L = (uint32_t)epoch + 1;
-      k_bfs_reset<<<1, 1, 0, s>>>(cnt.p);
+      bfs_counters* c = set(epoch);
       if (push) {
         PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
         k_bfs_push_prep<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(vis.p, bc, nwords, bn, nwords);
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
         // a pull step leaves the frontier as a bitmap only: queue it (heavy_count as scratch)
-        if (!queued) frontier_compact(ctx, bc, nwords, g->V, qc, &cnt.p->heavy_count);
-        queued = push_level(qc, n, A, L, qn, bn);
+        if (!queued) frontier_compact(ctx, bc, nwords, g->V, qc, &c->heavy_count);
+        queued = push_level(qc, n, A, L, qn, bn, c);
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
         k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, vis.p, level.p, bc,
-                                                      L, bn, hb.p, cnt.p, g->od.p);
-        frontier_compact(ctx, hb.p, nwords, g->V, heavy.p, &cnt.p->heavy_count);
-        k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, heavy.p,
-                                                    cnt.p, g->od.p);
+                                                      L, bn, hb.p, c, g->od.p);
+        frontier_compact(ctx, hb.p, nwords, g->V, heavy.p, &c->heavy_count);
+        k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, heavy.p, c,
+                                                    g->od.p);
         queued = false;
       }
       PE_HIP(hipGetLastError());
-      PE_HIP(hipMemcpyAsync(hcnt, cnt.p, sizeof(bfs_counters), hipMemcpyDeviceToHost, s));
-      PE_HIP(hipStreamSynchronize(s));
-      n = hcnt->next_count;
-      A = hcnt->next_active;
+      h = reinterpret_cast<const bfs_counters*>(mail_post(ctx, reinterpret_cast<const unsigned long long*>(c), CW,
+                                                          reinterpret_cast<unsigned long long*>(set(epoch + 1)), CW));
+      n = h->next_count;
+      A = h->next_active;
       std::swap(qc, qn);
       std::swap(bc, bn);
       ++epoch;
@@ -600,7 +604,7 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
       PE_HIP(hipMemcpyAsync(&h, st.cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
       ktime_scope kt(ctx, PE_KT_BFS_PUSH);
-      st.push_level(st.q0.p, h.heavy_count, 0, L, st.q1.p, next->p);
+      st.push_level(st.q0.p, h.heavy_count, 0, L, st.q1.p, next->p, st.cnt.p);
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
       k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0,

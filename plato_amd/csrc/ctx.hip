@@ -172,6 +172,40 @@ __global__ void k_bitmap_emit(const uint64_t* __restrict__ w, uint64_t words,
   }
 }
 
+// One thread: the step's counters to the host words, the next step's counters zeroed,
+// then the sequence number, released at system scope after the words.
+__global__ void k_mail_post(const unsigned long long* __restrict__ src, uint32_t n,
+                            unsigned long long* __restrict__ clear, uint32_t nclear,
+                            unsigned long long* __restrict__ mail, unsigned long long seq) {
+  for (uint32_t i = 0; i < n; ++i) mail[1 + i] = src[i];
+  for (uint32_t i = 0; i < nclear; ++i) clear[i] = 0ull;
+  __threadfence_system();
+  __hip_atomic_store(&mail[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+const unsigned long long* mail_post(pe_ctx_s* ctx, const unsigned long long* src, uint32_t n,
+                                    unsigned long long* clear, uint32_t nclear) {
+  PE_REQUIRE(n <= 64, PE_ERR_INTERNAL, "mail_post: more than 64 words");
+  const unsigned long long seq = ++ctx->mail_seq;
+  k_mail_post<<<1, 1, 0, ctx->stream>>>(src, n, clear, nclear, ctx->host_mail, seq);
+  PE_HIP(hipGetLastError());
+  // poll the sequence number; every 1024 polls ask the stream, so that a failed kernel
+  // ends the wait with its error instead of spinning
+  for (uint64_t k = 1;; ++k) {
+    if (__atomic_load_n(&ctx->host_mail[0], __ATOMIC_ACQUIRE) == seq) break;
+    if ((k & 1023) == 0) {
+      const hipError_t e = hipStreamQuery(ctx->stream);
+      if (e == hipSuccess) {
+        PE_REQUIRE(__atomic_load_n(&ctx->host_mail[0], __ATOMIC_ACQUIRE) == seq, PE_ERR_INTERNAL,
+                   "mail_post: stream finished without the sequence number");
+        break;
+      }
+      if (e != hipErrorNotReady) PE_HIP(e);
+    }
+  }
+  return ctx->host_mail + 1;
+}
+
 }  // namespace pe
 
 using namespace pe;
@@ -200,6 +234,8 @@ int pe_ctx_create(int device, pe_ctx_t** out) {
     PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     ctx->num_cus = cus;
     PE_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+    PE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_mail), 4096, hipHostMallocCoherent));
+    ctx->host_mail[0] = 0ull;
   });
   if (rc != PE_OK) {
     std::fprintf(stderr, "pe_ctx_create: %s\n", ctx->last_error.c_str());
@@ -222,6 +258,7 @@ int pe_ctx_destroy(pe_ctx_t* ctx) {
   if (ctx->t0) (void)hipEventDestroy(ctx->t0);
   if (ctx->t1) (void)hipEventDestroy(ctx->t1);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->host_mail) (void)hipHostFree(ctx->host_mail);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PE_OK;

@@ -59,9 +59,19 @@ struct pe_ctx_s {
   int num_cus = 0;
   // device scalars used by reductions; pinned host mirror for cheap readback
   void* host_pinned = nullptr;  // 4 KiB
+  // coherent pinned host words that a one-thread kernel writes at the end of a step and
+  // the host polls: a step's counters reach the host without a copy or a stream wait
+  unsigned long long* host_mail = nullptr;  // 4 KiB; [0] = sequence number, [1..] = words
+  unsigned long long mail_seq = 0;
 };
 
 namespace pe {
+
+// Copies n (at most 64) device words to the host and zeroes nclear device words after
+// the work queued on the stream, then waits for them: returns the host copy, valid until
+// the next call on this context.
+const unsigned long long* mail_post(pe_ctx_s* ctx, const unsigned long long* src, uint32_t n,
+                                    unsigned long long* clear, uint32_t nclear);
 
 // Allocation through the context keeps a ledger (admission control reads it).
 void* dmalloc(pe_ctx_s* ctx, size_t bytes);
