@@ -45,15 +45,25 @@ __device__ __forceinline__ unsigned wave_prefix(uint64_t mask) {
   return __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1ull));
 }
 
-// levels UNVIS but the root's 0; the visited bitmap empty (the root is in the frontier)
+// A search from root: levels UNVIS but the root's 0, the visited bitmap empty, the
+// frontier {root} as a bitmap of fwords words and as a queue, its size and active edges
+// in *cnt.
 __global__ void k_bfs_init(uint32_t* __restrict__ level, uint64_t* __restrict__ vis, uint32_t V, uint64_t nwords,
-                           uint32_t root) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x)
-    level[v] = (v == root) ? 0u : UNVIS;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nwords;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    vis[i] = 0ull;
+                           uint32_t root, uint64_t* __restrict__ fbits, uint64_t fwords, uint32_t* __restrict__ queue,
+                           bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
+  const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = tid; v < V; v += stride) level[v] = (v == root) ? 0u : UNVIS;
+  const uint64_t nw = nwords > fwords ? nwords : fwords;
+  for (uint64_t i = tid; i < nw; i += stride) {
+    if (i < nwords) vis[i] = 0ull;
+    if (i < fwords) fbits[i] = (i == (root >> 6)) ? 1ull << (root & 63) : 0ull;
+  }
+  if (tid == 0) {
+    queue[0] = root;
+    cnt->next_count = 1;
+    cnt->next_active = od[root];
+    cnt->heavy_count = 0;
+  }
 }
 
 // Before a top-down step: the current frontier joins the visited bitmap, the next is cleared.
@@ -78,15 +88,6 @@ __device__ __forceinline__ bool bfs_claim(uint32_t w, const uint64_t* __restrict
                                           uint32_t L) {
   if ((vis[w >> 6] >> (w & 63)) & 1ull) return false;
   return level[w] == UNVIS && atomicMin(&level[w], L) == UNVIS;
-}
-
-__global__ void k_bfs_seed(uint32_t* __restrict__ queue, uint64_t* __restrict__ bitmap, uint32_t root,
-                           bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
-  queue[0] = root;
-  bitmap[root >> 6] |= 1ull << (root & 63);
-  cnt->next_count = 1;
-  cnt->next_active = od[root];
-  cnt->heavy_count = 0;
 }
 
 __global__ void k_bfs_reset(bfs_counters* __restrict__ cnt) {
@@ -405,7 +406,7 @@ struct bfs_state {
   dbuf<uint64_t> vis;
   dbuf<bfs_counters> cnt;  // two sets: consecutive steps alternate
   dbuf<unsigned long long> red;
-  dbuf<uint64_t> deg, pref;  // frontier degrees and their scan (load-balanced push)
+  dbuf<uint64_t> deg, pref;  // frontier degrees and their scan (edge-balanced push)
   dbuf<uint8_t> scan_tmp;
   size_t scan_bytes = 0;
 
@@ -465,13 +466,12 @@ struct bfs_state {
     const unsigned pull_grid = persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU);
     const unsigned heavy_grid = persistent_grid(ctx, 1u << 20, 4);
     PE_HIP(hipEventRecord(ctx->t0, s));
-    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, vis.p, g->V, nwords, root);
-    k_zero_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(b0.p, nwords);
-    // counters: step e adds into set (e + 1) & 1 (the seed into set 0); each step's
-    // counters are mailed to the host, which zeroes the next step's set on the way
+    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(level.p, vis.p, g->V, nwords, root, b0.p, nwords, q0.p,
+                                                         cnt.p, g->od.p);
+    // counters: step e adds into set (e + 1) & 1 (the root's are in set 0); the kernel that
+    // mails a step's counters to the host zeroes the next step's set
     auto set = [&](int e) { return cnt.p + ((e + 1) & 1); };
     constexpr uint32_t CW = sizeof(bfs_counters) / 8;
-    k_bfs_seed<<<1, 1, 0, s>>>(q0.p, b0.p, root, set(-1), g->od.p);
     PE_HIP(hipGetLastError());
     const bfs_counters* h = reinterpret_cast<const bfs_counters*>(
         mail_post(ctx, reinterpret_cast<const unsigned long long*>(set(-1)), CW,
@@ -496,7 +496,7 @@ struct bfs_state {
         PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
         k_bfs_push_prep<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(vis.p, bc, nwords, bn, nwords);
         ktime_scope kt(ctx, PE_KT_BFS_PUSH);
-        // a pull step leaves the frontier as a bitmap only: queue it (heavy_count as scratch)
+        // a step that left the frontier as a bitmap only: queue it (heavy_count as scratch)
         if (!queued) frontier_compact(ctx, bc, nwords, g->V, qc, &c->heavy_count);
         queued = push_level(qc, n, A, L, qn, bn, c);
       } else {
@@ -570,9 +570,9 @@ int pe_bfs_reset(pe_bfs_t* b, uint32_t root, pe_bitmap_t* frontier) {
     uint32_t iroot = root;
     to_internal_ids(g, &root, 1, &iroot);
     hipStream_t s = g->ctx->stream;
-    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(b->st->level.p, b->st->vis.p, g->V, b->st->nwords, iroot);
-    k_zero_words<<<grid_for(frontier->words, 256, 4096), 256, 0, s>>>(frontier->p, frontier->words);
-    k_bfs_seed<<<1, 1, 0, s>>>(b->st->q0.p, frontier->p, iroot, b->st->cnt.p, g->od.p);
+    k_bfs_init<<<grid_for(g->V, 256, 8192), 256, 0, s>>>(b->st->level.p, b->st->vis.p, g->V, b->st->nwords, iroot,
+                                                         frontier->p, frontier->words, b->st->q0.p, b->st->cnt.p,
+                                                         g->od.p);
     PE_HIP(hipGetLastError());
     PE_HIP(hipStreamSynchronize(s));
     b->epoch = 0;
