@@ -112,3 +112,54 @@ def test_engine_step_loop_mirrors_reference(ctx):
             epoch += 1
         assert modes == ref["modes"].tolist() and visited == ref["visited"]
         assert np.array_equal(st.levels(), ref["levels"])
+
+
+def _check_bfs(g, og, root, threshold):
+    from plato_amd.algo import bfs_opts_t, breadth_first_search
+    ref = oracle.bfs(og, int(root), threshold=threshold)
+    res = breadth_first_search(g, bfs_opts_t(int(root)), push_threshold=threshold)
+    assert np.array_equal(res.levels, ref["levels"]), f"root {root} threshold {threshold}"
+    assert np.array_equal(res.modes, ref["modes"])
+    assert np.array_equal(res.frontier, ref["frontier"])
+    assert np.array_equal(res.active_edges, ref["active_edges"])
+    assert res.visited == ref["visited"]
+    return res
+
+
+def test_bfs_edge_balanced_push_zero_degree_runs(ctx):
+    """Top-down steps over a queue with long runs of zero-degree vertices between
+    vertices of 300 out-edges (more than 4096 active edges: the edge-balanced step,
+    whose 64-vertex window then holds fewer than 64 edges), push forced."""
+    from plato_amd import Graph
+    n, hub_every, hub_deg = 20000, 97, 300
+    src = [np.zeros(n, np.uint32)]
+    dst = [np.arange(1, n + 1, dtype=np.uint32)]
+    nxt = n + 1
+    for v in range(1, n + 1):
+        if v % hub_every == 0:
+            src.append(np.full(hub_deg, v, np.uint32))
+            dst.append(np.arange(nxt, nxt + hub_deg, dtype=np.uint32))
+            nxt += hub_deg
+    s, d = np.concatenate(src), np.concatenate(dst)
+    V = nxt
+    print(f"case: bfs zero-degree runs V={V} E={len(s)}", flush=True)
+    g = Graph.from_edges(ctx, s, d, V, build_csr=True)
+    og = oracle.CSC(V, s, d, symmetrize=False)
+    res = _check_bfs(g, og, 0, 1e9)
+    assert res.modes.tolist() == [1, 1, 1] and int(res.active_edges[1]) > 4096
+
+
+@pytest.mark.parametrize("threshold", [1e9, 0.0])
+def test_bfs_forced_direction_rmat(ctx, threshold):
+    """Every level top-down (threshold 1e9: edge-balanced steps over hub frontiers) or
+    every level bottom-up (threshold 0), directed and symmetrized."""
+    from plato_amd import Graph
+    for directed in (True, False):
+        print(f"case: bfs forced threshold={threshold} directed={directed}", flush=True)
+        g = Graph.rmat(ctx, 13, 16, seed=19, directed=directed, build_csr=True)
+        s, d = oracle.rmat_edges(13, 16, 19)
+        og = oracle.CSC(1 << 13, s, d, symmetrize=not directed)
+        cand = np.flatnonzero(og.od > 0)
+        for root in [int(cand[0]), int(cand[len(cand) // 2]), int(np.argmax(og.od))]:
+            res = _check_bfs(g, og, root, threshold)
+            assert set(res.modes.tolist()) == ({1} if threshold > 1 else {0})
