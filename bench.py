@@ -450,6 +450,9 @@ def main():
     from plato_amd.algo import pagerank, pagerank_opts_t
 
     ctx = Context(0)
+    # what the device streams (the ceiling the roofline fraction is read against)
+    stream = ctx.stream_rates(1 << 31, 5)
+    log(f"[bench] stream rates over 2 GiB: read {stream['read_gbs']:.0f} GB/s, copy {stream['copy_gbs']:.0f} GB/s")
     log(f"[bench] building RMAT scale {args.scale} ef {args.edge_factor} on the device")
     t0 = time.time()
     g = Graph.rmat(ctx, args.scale, args.edge_factor, seed=args.seed, relabel=not args.no_relabel)
@@ -492,6 +495,10 @@ def main():
             "kernel": head["kernel"], "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": round(head["kernel_ms"], 4)}
     roof.update(head["extra"])
+    roof["stream"] = {"read_gbs": round(stream["read_gbs"], 1), "copy_gbs": round(stream["copy_gbs"], 1),
+                      "kernels": "k_read_words, k_copy_words over 2 GiB, 8 blocks of 256 per CU"}
+    if traffic:
+        roof["traffic_gbs"] = round(traffic / kern_s / 1e9, 1)
     out = {
         "metric": METRIC,
         "value": round(gteps, 2),

@@ -79,6 +79,13 @@ int pe_ctx_set_kernel_timing(pe_ctx_t* ctx, int enable);
 int pe_ctx_kernel_time(pe_ctx_t* ctx, int which, double* total_ms, uint64_t* launches);
 /* multiprocessor count of the device */
 int pe_ctx_num_cus(pe_ctx_t* ctx, int* cus);
+/* Streaming rates of the device (no counterpart in the reference: the ceiling that
+ * roofline figures are read against).  Times `reps` passes of a read-only kernel
+ * over `bytes` of device memory and of a copy of `bytes` into a second buffer
+ * (16-byte loads, 8 blocks of 256 threads per CU), after one untimed pass of each;
+ * read_gbs = bytes / time, copy_gbs = 2 * bytes / time, in 1e9 bytes per second.
+ * Allocates and frees 2 * bytes; bytes is a positive multiple of 16. */
+int pe_ctx_stream_rates(pe_ctx_t* ctx, uint64_t bytes, int reps, double* read_gbs, double* copy_gbs);
 
 /* ===========================================================================
  * Graph.  Replaces dcsc_t (plato/graph/structure/dcsc.hpp:58-185) and

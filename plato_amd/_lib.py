@@ -121,6 +121,7 @@ SIGNATURES = {
     "pe_ctx_set_kernel_timing": [_vp, _i32],
     "pe_ctx_kernel_time": [_vp, _i32, _P(_dbl), _P(_u64)],
     "pe_ctx_num_cus": [_vp, _P(_i32)],
+    "pe_ctx_stream_rates": [_vp, _u64, _i32, _P(_dbl), _P(_dbl)],
     "pe_graph_create": [_vp, _vp, _vp, _vp, _u64, _u32, _u32, _P(_vp)],
     "pe_graph_create_csv": [_vp, C.c_char_p, _u64, _u32, _P(_vp)],
     "pe_graph_vertex_ids": [_vp, _vp],

@@ -92,6 +92,19 @@ __global__ __launch_bounds__(256) void k_copy_words(const uint4* __restrict__ sr
     dst[i] = src[i];
 }
 
+// read-only stream (the read rate): every word is loaded and folded into a value that
+// is stored only if it equals a constant, so the loads cannot be dropped
+__global__ __launch_bounds__(256) void k_read_words(const uint4* __restrict__ src, uint64_t n16,
+                                                   uint32_t* __restrict__ sink) {
+  uint32_t acc = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 v = src[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;
+}
+
 __global__ void k_copy_tail(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -306,6 +319,35 @@ int pe_ctx_set_kernel_timing(pe_ctx_t* ctx, int enable) {
   if (!ctx) return PE_ERR_INVALID;
   ctx->timing = enable ? 1 : 0;
   return PE_OK;
+}
+
+int pe_ctx_stream_rates(pe_ctx_t* ctx, uint64_t bytes, int reps, double* read_gbs, double* copy_gbs) {
+  if (!ctx || bytes == 0 || bytes % 16 || reps < 1) return PE_ERR_INVALID;
+  return guarded(ctx, [&] {
+    const uint64_t n16 = bytes / 16;
+    dbuf<uint4> a(ctx, n16), b(ctx, n16);
+    dbuf<uint32_t> sink(ctx, 1);
+    hipStream_t s = ctx->stream;
+    PE_HIP(hipMemsetAsync(a.p, 1, bytes, s));
+    PE_HIP(hipMemsetAsync(b.p, 0, bytes, s));
+    const unsigned grid = (unsigned)std::max(1, ctx->num_cus) * 8u;
+    float ms = 0.f;
+    k_read_words<<<grid, 256, 0, s>>>(a.p, n16, sink.p);
+    PE_HIP(hipEventRecord(ctx->t0, s));
+    for (int r = 0; r < reps; ++r) k_read_words<<<grid, 256, 0, s>>>(a.p, n16, sink.p);
+    PE_HIP(hipEventRecord(ctx->t1, s));
+    PE_HIP(hipEventSynchronize(ctx->t1));
+    PE_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+    if (read_gbs) *read_gbs = (double)bytes * reps / (ms * 1e-3) / 1e9;
+    k_copy_words<<<grid, 256, 0, s>>>(a.p, b.p, n16);
+    PE_HIP(hipEventRecord(ctx->t0, s));
+    for (int r = 0; r < reps; ++r) k_copy_words<<<grid, 256, 0, s>>>(a.p, b.p, n16);
+    PE_HIP(hipEventRecord(ctx->t1, s));
+    PE_HIP(hipEventSynchronize(ctx->t1));
+    PE_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+    if (copy_gbs) *copy_gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    PE_HIP(hipGetLastError());
+  });
 }
 
 int pe_ctx_kernel_time(pe_ctx_t* ctx, int which, double* total_ms, uint64_t* launches) {

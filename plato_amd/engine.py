@@ -69,6 +69,12 @@ class Context:
         check(self.lib.pe_ctx_num_cus(self.h, C.byref(n)), self.h)
         return n.value
 
+    def stream_rates(self, nbytes: int = 1 << 31, reps: int = 5):
+        """Read-only and copy streaming rates of the device (GB/s) over nbytes buffers."""
+        r, c = C.c_double(), C.c_double()
+        check(self.lib.pe_ctx_stream_rates(self.h, int(nbytes), int(reps), C.byref(r), C.byref(c)), self.h)
+        return dict(read_gbs=r.value, copy_gbs=c.value, bytes=int(nbytes))
+
     def set_kernel_timing(self, on: bool):
         check(self.lib.pe_ctx_set_kernel_timing(self.h, int(on)), self.h)
 
