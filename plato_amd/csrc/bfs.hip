@@ -245,6 +245,16 @@ __global__ __launch_bounds__(256) void k_bfs_push(
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
 }
 
+// first_in[v] = lowest in-neighbour of v (the first of its ascending in-edge list:
+// with degree relabeling, its highest-degree one), UNVIS for none
+__global__ void k_first_in(const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
+                           uint32_t* __restrict__ first_in) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = col_ptr[v];
+    first_in[v] = b < col_ptr[v + 1] ? row_idx[b] : UNVIS;
+  }
+}
+
 // Bottom-up step.  The first PULL_HOT_WORDS words of the frontier bitmap (ids below
 // 64 * PULL_HOT_WORDS) are staged in LDS once per block: with degree relabeling those
 // are the hubs, which most in-edge lists start with, so most membership tests hit LDS.
@@ -257,8 +267,9 @@ static constexpr uint32_t PULL_THREADS = 1024;
 #endif
 
 __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
-    const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx, uint32_t V,
-    uint64_t nwords, uint64_t* __restrict__ vis, uint32_t* __restrict__ level,
+    const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
+    const uint32_t* __restrict__ first_in, uint32_t V, uint64_t nwords, uint64_t* __restrict__ vis,
+    uint32_t* __restrict__ level,
     const uint64_t* __restrict__ cur_bits, uint32_t L, uint64_t* __restrict__ next_bits,
     uint64_t* __restrict__ heavy_bits, bfs_counters* __restrict__ cnt, const uint32_t* __restrict__ od) {
   __shared__ uint64_t hot[PULL_HOT_WORDS];
@@ -276,14 +287,24 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
     const uint64_t seen = vis[word] | cw;
     bool claimed = false, is_heavy = false;
     if (v < V && !((seen >> lane) & 1ull)) {
-      const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
-      const uint64_t lim = (e - b > PULL_CAP) ? b + PULL_CAP : e;
-      for (uint64_t i = b; i < lim; ++i) {
-        const uint32_t u = row_idx[i];
-        const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
-        if ((wd >> (u & 63)) & 1ull) { claimed = true; break; }
+      // the first in-edge from the dense first_in array (one coalesced word per lane);
+      // the list is read only when that neighbour is not in the frontier
+      const uint32_t f = first_in[v];
+      if (f != UNVIS) {
+        const uint64_t fw = (f >> 6) < nh ? hot[f >> 6] : cur_bits[f >> 6];
+        if ((fw >> (f & 63)) & 1ull) {
+          claimed = true;
+        } else {
+          const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+          const uint64_t lim = (e - b > PULL_CAP) ? b + PULL_CAP : e;
+          for (uint64_t i = b + 1; i < lim; ++i) {
+            const uint32_t u = row_idx[i];
+            const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
+            if ((wd >> (u & 63)) & 1ull) { claimed = true; break; }
+          }
+          is_heavy = !claimed && e > lim;
+        }
       }
-      is_heavy = !claimed && e > lim;
     }
     // the next frontier and the vertices left for the wave-per-vertex pass as bitmap
     // words (one store per wave, no atomics); counts are summed once per wave at the end
@@ -412,6 +433,12 @@ struct bfs_state {
 
   explicit bfs_state(pe_graph_s* graph) : g(graph) {
     pe_ctx_s* ctx = g->ctx;
+    if (!g->first_in.p) {  // kept with the graph for later searches
+      g->first_in.alloc(ctx, std::max<uint32_t>(1, g->V));
+      k_first_in<<<grid_for(g->V, 256, 8192), 256, 0, ctx->stream>>>(g->col_ptr.p, g->row_idx.p, g->V,
+                                                                      g->first_in.p);
+      PE_HIP(hipGetLastError());
+    }
     const uint32_t V = std::max<uint32_t>(1, g->V);
     nwords = ((uint64_t)g->V >> 6) + 1;
     level.alloc(ctx, V);
@@ -501,8 +528,8 @@ struct bfs_state {
         queued = push_level(qc, n, A, L, qn, bn, c);
       } else {
         ktime_scope kt(ctx, PE_KT_BFS_PULL);
-        k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, vis.p, level.p, bc,
-                                                      L, bn, hb.p, c, g->od.p);
+        k_bfs_pull<<<pull_grid, PULL_THREADS, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->first_in.p, g->V, nwords,
+                                                      vis.p, level.p, bc, L, bn, hb.p, c, g->od.p);
         frontier_compact(ctx, hb.p, nwords, g->V, heavy.p, &c->heavy_count);
         k_bfs_pull_heavy<<<heavy_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, level.p, bc, L, bn, heavy.p, c,
                                                     g->od.p);
@@ -608,8 +635,8 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
     } else {
       ktime_scope kt(ctx, PE_KT_BFS_PULL);
       k_bfs_pull<<<persistent_grid(ctx, grid_for(nwords * 64, PULL_THREADS), PULL_BLOCKS_PER_CU), PULL_THREADS, 0,
-                   s>>>(g->col_ptr.p, g->row_idx.p, g->V, nwords, st.vis.p, st.level.p, current->p, L, next->p,
-                        st.hb.p, st.cnt.p, g->od.p);
+                   s>>>(g->col_ptr.p, g->row_idx.p, g->first_in.p, g->V, nwords, st.vis.p, st.level.p, current->p,
+                        L, next->p, st.hb.p, st.cnt.p, g->od.p);
       frontier_compact(ctx, st.hb.p, nwords, g->V, st.heavy.p, &st.cnt.p->heavy_count);
       k_bfs_pull_heavy<<<persistent_grid(ctx, 1u << 20, 4), 256, 0, s>>>(
           g->col_ptr.p, g->row_idx.p, st.level.p, current->p, L, next->p, st.heavy.p, st.cnt.p, g->od.p);

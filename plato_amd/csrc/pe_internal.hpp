@@ -229,6 +229,7 @@ struct pe_graph_s {
   pe::dbuf<uint32_t> row_idx;   // [nnz], ascending source id inside each column
   pe::dbuf<float> wn;           // [nnz] weight / sum of the source's out-weights
   pe::dbuf<uint32_t> od;        // [V] out-degree counted from the in-edges
+  pe::dbuf<uint32_t> first_in;  // [V] lowest in-neighbour (0xFFFFFFFF: none); built by the first BFS
   pe::dbuf<uint64_t> row_ptr;   // [V+1] CSR (out-edges)
   pe::dbuf<uint32_t> col_idx;   // [nnz]
 
