@@ -358,35 +358,62 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
 }
 
-// Frontier bitmap -> queue: one lane per 64-bit word, a wave covers 64 words;
-// the wave's popcounts are prefix-summed with shuffles, one atomic reserves the
-// wave's slice of the queue, and each lane writes its word's ids.
+// Frontier bitmap -> queue.  Each block owns one contiguous range of words: it counts
+// the range's set bits, reserves that many queue slots with one atomic, then writes the
+// ids tile by tile (256 words, one per thread) at offsets from a block-wide scan.  One
+// atomic per block: a counter that every wave bumps serialises in the memory-side atomic
+// unit.  Ids come out ascending inside each block's slice.
 __global__ __launch_bounds__(256) void k_frontier_compact(const uint64_t* __restrict__ words, uint64_t nwords,
                                                           uint32_t V, uint32_t* __restrict__ queue,
                                                           unsigned long long* __restrict__ count) {
-  const int lane = threadIdx.x & 63;
-  for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6) * 64; base < nwords;
-       base += ((uint64_t)gridDim.x * blockDim.x)) {
-    const uint64_t wi = base + lane;
-    uint64_t w = wi < nwords ? words[wi] : 0ull;
-    if (wi == nwords - 1 && (V & 63)) w &= (1ull << (V & 63)) - 1ull;  // bits >= V are not vertices
-    if (wi >= (uint64_t)(V >> 6) + ((V & 63) ? 1 : 0)) w = 0ull;
-    const unsigned c = __popcll(w);
-    unsigned incl = c;
+  __shared__ unsigned long long s_part[4];
+  __shared__ unsigned s_wave[4];
+  __shared__ unsigned long long s_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t vwords = std::min<uint64_t>(nwords, (uint64_t)(V >> 6) + ((V & 63) ? 1 : 0));
+  const uint64_t per = (vwords + gridDim.x - 1) / gridDim.x;
+  const uint64_t w0 = std::min<uint64_t>(vwords, blockIdx.x * per), w1 = std::min<uint64_t>(vwords, w0 + per);
+  auto word_at = [&](uint64_t wi) -> uint64_t {
+    if (wi >= w1) return 0ull;
+    uint64_t w = words[wi];
+    if (wi == (uint64_t)(V >> 6) && (V & 63)) w &= (1ull << (V & 63)) - 1ull;  // bits >= V are not vertices
+    return w;
+  };
+  unsigned long long c = 0;
+  for (uint64_t wi = w0 + threadIdx.x; wi < w1; wi += 256) c += (unsigned long long)__popcll(word_at(wi));
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if (lane == 0) s_part[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long total = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    s_base = total ? atomicAdd(count, total) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long pos = s_base;
+  for (uint64_t t0 = w0; t0 < w1; t0 += 256) {
+    const uint64_t wi = t0 + threadIdx.x;
+    uint64_t w = word_at(wi);
+    const unsigned cnt = (unsigned)__popcll(w);
+    unsigned incl = cnt;
     for (int off = 1; off < 64; off <<= 1) {
       const unsigned t = __shfl_up(incl, off, 64);
       if (lane >= off) incl += t;
     }
-    const unsigned total = __shfl(incl, 63, 64);
-    unsigned long long start = 0;
-    if (lane == 0 && total) start = atomicAdd(count, (unsigned long long)total);
-    start = __shfl(start, 0, 64);
-    uint64_t pos = start + incl - c;
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    unsigned before = 0, tile = 0;
+    for (int k = 0; k < 4; ++k) {
+      const unsigned x = s_wave[k];
+      before += k < wave ? x : 0u;
+      tile += x;
+    }
+    __syncthreads();  // s_wave is rewritten by the next tile
+    uint64_t p = pos + before + incl - cnt;
     while (w) {
-      const int b = __ffsll((unsigned long long)w) - 1;
-      queue[pos++] = (uint32_t)(wi * 64 + b);
+      queue[p++] = (uint32_t)(wi * 64 + (uint64_t)(__ffsll((unsigned long long)w) - 1));
       w &= w - 1;
     }
+    pos += tile;
   }
 }
 
@@ -394,7 +421,10 @@ __global__ __launch_bounds__(256) void k_frontier_compact(const uint64_t* __rest
 // must be zero or hold the entries already queued.
 void frontier_compact(pe_ctx_s* ctx, const uint64_t* words, uint64_t nwords, uint32_t V, uint32_t* queue,
                       unsigned long long* count) {
-  k_frontier_compact<<<grid_for(nwords, 256, 4096), 256, 0, ctx->stream>>>(words, nwords, V, queue, count);
+  const uint64_t vwords = std::min<uint64_t>(nwords, (uint64_t)(V >> 6) + ((V & 63) ? 1 : 0));
+  const unsigned grid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)std::max(1, ctx->num_cus), (vwords + 255) / 256));
+  k_frontier_compact<<<grid, 256, 0, ctx->stream>>>(words, nwords, V, queue, count);
   PE_HIP(hipGetLastError());
 }
 
@@ -622,8 +652,7 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
     if (push) {
       PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
       // compact the frontier bitmap into a queue (count in heavy_count, reused as scratch)
-      k_frontier_compact<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(current->p, nwords, g->V, st.q0.p,
-                                                                     &st.cnt.p->heavy_count);
+      frontier_compact(ctx, current->p, nwords, g->V, st.q0.p, &st.cnt.p->heavy_count);
       k_bfs_push_prep<<<grid_for(std::max(nwords, next->words), 256, 4096), 256, 0, s>>>(st.vis.p, current->p, nwords,
                                                                                           next->p, next->words);
       PE_HIP(hipGetLastError());
