@@ -358,6 +358,8 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
 }
 
+static constexpr unsigned COMPACT_BLOCKS_PER_CU = 4;
+
 // Frontier bitmap -> queue.  Each block owns one contiguous range of words: it counts
 // the range's set bits, reserves that many queue slots with one atomic, then writes the
 // ids tile by tile (256 words, one per thread) at offsets from a block-wide scan.  One
@@ -423,7 +425,7 @@ void frontier_compact(pe_ctx_s* ctx, const uint64_t* words, uint64_t nwords, uin
                       unsigned long long* count) {
   const uint64_t vwords = std::min<uint64_t>(nwords, (uint64_t)(V >> 6) + ((V & 63) ? 1 : 0));
   const unsigned grid = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)std::max(1, ctx->num_cus), (vwords + 255) / 256));
+      1, std::min<uint64_t>((uint64_t)std::max(1, ctx->num_cus) * COMPACT_BLOCKS_PER_CU, (vwords + 255) / 256));
   k_frontier_compact<<<grid, 256, 0, ctx->stream>>>(words, nwords, V, queue, count);
   PE_HIP(hipGetLastError());
 }
