@@ -258,6 +258,7 @@ __global__ void k_first_in(const uint64_t* __restrict__ col_ptr, const uint32_t*
 // Bottom-up step.  The first PULL_HOT_WORDS words of the frontier bitmap (ids below
 // 64 * PULL_HOT_WORDS) are staged in LDS once per block: with degree relabeling those
 // are the hubs, which most in-edge lists start with, so most membership tests hit LDS.
+static constexpr int PULL_SCAN_UNROLL = 4;  // in-edges a pull lane loads per round trip
 #ifndef PULL_BLOCKS_PER_CU
 #define PULL_BLOCKS_PER_CU 1
 #endif
@@ -297,10 +298,19 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
         } else {
           const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
           const uint64_t lim = (e - b > PULL_CAP) ? b + PULL_CAP : e;
-          for (uint64_t i = b + 1; i < lim; ++i) {
-            const uint32_t u = row_idx[i];
-            const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
-            if ((wd >> (u & 63)) & 1ull) { claimed = true; break; }
+          // PULL_SCAN_UNROLL in-edges per round trip: their ids are loaded together, then tested
+          for (uint64_t i = b + 1; i < lim && !claimed; i += PULL_SCAN_UNROLL) {
+            uint32_t us[PULL_SCAN_UNROLL];
+#pragma unroll
+            for (int k = 0; k < PULL_SCAN_UNROLL; ++k) us[k] = i + k < lim ? row_idx[i + k] : UNVIS;
+#pragma unroll
+            for (int k = 0; k < PULL_SCAN_UNROLL; ++k) {
+              const uint32_t u = us[k];
+              if (u != UNVIS) {
+                const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
+                if ((wd >> (u & 63)) & 1ull) claimed = true;
+              }
+            }
           }
           is_heavy = !claimed && e > lim;
         }
