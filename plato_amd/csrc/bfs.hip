@@ -24,7 +24,6 @@
 
 #include "pe_internal.hpp"
 
-#include <rocprim/device/device_scan.hpp>
 
 namespace pe {
 
@@ -113,11 +112,78 @@ __device__ __forceinline__ unsigned long long wave_append(uint64_t mask,
   return base + wave_prefix(mask);
 }
 
-// Out-degree of every frontier vertex (deg[n] = 0), scanned into edge offsets.
-__global__ void k_frontier_degrees(const uint32_t* __restrict__ queue, uint64_t n, const uint64_t* __restrict__ ptr,
-                                   uint64_t* __restrict__ deg) {
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q <= n; q += (uint64_t)gridDim.x * blockDim.x)
-    deg[q] = q < n ? ptr[queue[q] + 1] - ptr[queue[q]] : 0ull;
+// Exclusive scan of the queued vertices' out-degrees into pref[0..n] (pref[n] = their
+// sum), in blocks of DEG_BLOCK entries: per-block sums, a one-block scan of those sums,
+// then each block recomputes its degrees and writes its offsets.
+static constexpr uint32_t DEG_BLOCK = 1024;  // entries per block: 256 threads x 4
+
+__device__ __forceinline__ uint64_t queued_degree(const uint32_t* __restrict__ queue, uint64_t n,
+                                                  const uint64_t* __restrict__ ptr, uint64_t q) {
+  if (q >= n) return 0ull;
+  const uint32_t v = queue[q];
+  return ptr[v + 1] - ptr[v];
+}
+
+// block-wide exclusive scan of one value per thread (256 threads); *total gets the sum
+__device__ __forceinline__ uint64_t block_excl_scan256(uint64_t x, uint64_t* total) {
+  __shared__ uint64_t s_wave[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t incl = x;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    before += k < wave ? s_wave[k] : 0ull;
+    sum += s_wave[k];
+  }
+  __syncthreads();
+  *total = sum;
+  return before + incl - x;
+}
+
+__global__ __launch_bounds__(256) void k_deg_block_sums(const uint32_t* __restrict__ queue, uint64_t n,
+                                                        const uint64_t* __restrict__ ptr, uint64_t* __restrict__ bsum) {
+  const uint64_t base = blockIdx.x * (uint64_t)DEG_BLOCK;
+  uint64_t d = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d += queued_degree(queue, n, ptr, base + (uint64_t)k * 256 + threadIdx.x);
+  uint64_t total = 0;
+  (void)block_excl_scan256(d, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// one block: bsum[0..nb) -> its exclusive scan, in place
+__global__ __launch_bounds__(256) void k_scan_block_sums(uint64_t* __restrict__ bsum, uint64_t nb) {
+  uint64_t carry = 0;
+  for (uint64_t c0 = 0; c0 < nb; c0 += 256) {
+    const uint64_t i = c0 + threadIdx.x;
+    const uint64_t x = i < nb ? bsum[i] : 0ull;
+    uint64_t total = 0;
+    const uint64_t ex = block_excl_scan256(x, &total);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_deg_scan_write(const uint32_t* __restrict__ queue, uint64_t n,
+                                                        const uint64_t* __restrict__ ptr,
+                                                        const uint64_t* __restrict__ boff, uint64_t* __restrict__ pref) {
+  const uint64_t q0 = blockIdx.x * (uint64_t)DEG_BLOCK + (uint64_t)threadIdx.x * 4;
+  uint64_t d[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d[k] = queued_degree(queue, n, ptr, q0 + k);
+  uint64_t total = 0;
+  uint64_t off = boff[blockIdx.x] + block_excl_scan256(d[0] + d[1] + d[2] + d[3], &total);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (q0 + k < n) pref[q0 + k] = off;
+    off += d[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) pref[n] = boff[blockIdx.x] + total;
 }
 
 // Edge-balanced top-down step.  The frontier's out-edges, numbered 0..T-1 through the
@@ -469,9 +535,8 @@ struct bfs_state {
   dbuf<uint64_t> vis;
   dbuf<bfs_counters> cnt;  // two sets: consecutive steps alternate
   dbuf<unsigned long long> red;
-  dbuf<uint64_t> deg, pref;  // frontier degrees and their scan (edge-balanced push)
-  dbuf<uint8_t> scan_tmp;
-  size_t scan_bytes = 0;
+  dbuf<uint64_t> pref;  // scan of the queued vertices' out-degrees (edge-balanced push)
+  dbuf<uint64_t> bsum;  // its per-block sums
 
   explicit bfs_state(pe_graph_s* graph) : g(graph) {
     pe_ctx_s* ctx = g->ctx;
@@ -493,11 +558,8 @@ struct bfs_state {
     vis.alloc(ctx, nwords);
     cnt.alloc(ctx, 2);
     red.alloc(ctx, 2);
-    deg.alloc(ctx, (uint64_t)V + 1);
     pref.alloc(ctx, (uint64_t)V + 1);
-    PE_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, deg.p, pref.p, (uint64_t)0, (size_t)V + 1,
-                                   rocprim::plus<uint64_t>(), ctx->stream));
-    scan_tmp.alloc(ctx, std::max<size_t>(1, scan_bytes));
+    bsum.alloc(ctx, ((uint64_t)V + DEG_BLOCK - 1) / DEG_BLOCK);
   }
 
   // One top-down level from the queue qc (n vertices, A active edges; A = 0 if unknown)
@@ -514,10 +576,11 @@ struct bfs_state {
       PE_HIP(hipGetLastError());
       return true;
     }
-    k_frontier_degrees<<<grid_for(n + 1, 256, 65536), 256, 0, s>>>(qc, n, g->csr_ptr(), deg.p);
-    size_t tb = scan_bytes;
-    PE_HIP(rocprim::exclusive_scan(scan_tmp.p, tb, deg.p, pref.p, (uint64_t)0, (size_t)n + 1,
-                                   rocprim::plus<uint64_t>(), s));
+    PE_REQUIRE(n > 0, PE_ERR_INTERNAL, "top-down step over an empty frontier");
+    const uint64_t nb = (n + DEG_BLOCK - 1) / DEG_BLOCK;
+    k_deg_block_sums<<<(unsigned)nb, 256, 0, s>>>(qc, n, g->csr_ptr(), bsum.p);
+    k_scan_block_sums<<<1, 256, 0, s>>>(bsum.p, nb);
+    k_deg_scan_write<<<(unsigned)nb, 256, 0, s>>>(qc, n, g->csr_ptr(), bsum.p, pref.p);
     const uint64_t waves = A ? std::max<uint64_t>(1, (A + EB_WAVE_EDGES - 1) / EB_WAVE_EDGES) : (uint64_t)push_grid * 4;
     const unsigned grid = (unsigned)std::min<uint64_t>(push_grid, (waves + 3) / 4);
     k_bfs_push_eb<<<grid, 256, 0, s>>>(qc, n, pref.p, g->csr_ptr(), g->csr_idx(), vis.p, level.p, L, bn, c,
