@@ -17,6 +17,9 @@
 // vertex, registers handled as 32-bit words of four; 64- and 128-register counters take
 // 4 or 2 in-edges per wave step (a row is 16 or 32 lanes), larger ones one edge per step
 // with each lane owning 2^P / 256 words; several rows are loaded before they are merged.
+// Vertices with more than HANF_HUB_DEG in-edges (the hubs) only copy their own row in
+// that pass; a second pass cuts their in-edges into chunks spread over all waves, each
+// merging its chunk's maximum into the hub's row, so no wave walks a hub alone.
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -24,6 +27,9 @@
 #include "pe_internal.hpp"
 
 namespace pe {
+
+static constexpr uint64_t HANF_HUB_DEG = 4096;    // in-edges above which a vertex is a hub
+static constexpr uint64_t HANF_HUB_CHUNK = 2048;  // in-edges of a hub per wave task
 
 __device__ __forceinline__ uint32_t hanf_rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
@@ -82,6 +88,10 @@ __global__ __launch_bounds__(256) void k_hanf_pull_small(const uint64_t* __restr
   for (uint64_t v = wave_id; v < V; v += nwaves) {
     uint32_t acc = slot == 0 ? cur[v * WPR + w] : 0u;
     const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    if (e - b > HANF_HUB_DEG) {  // a hub: its own row now, its in-edges in k_hanf_pull_hubs
+      if (slot == 0) nxt[v * WPR + w] = acc;
+      continue;
+    }
     for (uint64_t k0 = b; k0 < e; k0 += (uint64_t)EPS * U) {
       uint32_t val[U];
 #pragma unroll
@@ -109,6 +119,10 @@ __global__ __launch_bounds__(256) void k_hanf_pull_words(const uint64_t* __restr
   const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
   for (uint64_t v = wave_id; v < V; v += nwaves) {
     const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    if (e - b > HANF_HUB_DEG) {  // a hub: its own row now, its in-edges in k_hanf_pull_hubs
+      for (uint64_t i = lane; i < row; i += 64) nxt[v * row + i] = cur[v * row + i];
+      continue;
+    }
     for (uint64_t c0 = 0; c0 < row; c0 += 64 * W) {
       uint32_t acc[W];
 #pragma unroll
@@ -129,6 +143,118 @@ __global__ __launch_bounds__(256) void k_hanf_pull_words(const uint64_t* __restr
       }
 #pragma unroll
       for (int j = 0; j < W; ++j) nxt[v * row + c0 + (uint64_t)j * 64 + lane] = acc[j];
+    }
+  }
+}
+
+// Hubs: a bit per vertex with more than HANF_HUB_DEG in-edges (one word per wave).
+__global__ __launch_bounds__(256) void k_hanf_hub_bits(const uint64_t* __restrict__ col_ptr, uint32_t V,
+                                                       uint64_t nwords, uint64_t* __restrict__ bits) {
+  const int lane = threadIdx.x & 63;
+  for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
+       word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint64_t v = word * 64 + lane;
+    const bool hub = v < V && col_ptr[v + 1] - col_ptr[v] > HANF_HUB_DEG;
+    const uint64_t m = __ballot(hub);
+    if (lane == 0) bits[word] = m;
+  }
+}
+
+// One block: start[h] = first chunk of hub h (chunks of HANF_HUB_CHUNK in-edges),
+// start[n] = the number of chunks.
+__global__ __launch_bounds__(256) void k_hanf_hub_chunks(const uint64_t* __restrict__ col_ptr,
+                                                         const uint32_t* __restrict__ hubs,
+                                                         const unsigned long long* __restrict__ nhub,
+                                                         uint64_t* __restrict__ start) {
+  __shared__ uint64_t s_wave[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t n = *nhub;
+  uint64_t carry = 0;
+  for (uint64_t h0 = 0; h0 < n; h0 += 256) {
+    const uint64_t h = h0 + threadIdx.x;
+    uint64_t c = 0;
+    if (h < n) {
+      const uint32_t v = hubs[h];
+      c = (col_ptr[v + 1] - col_ptr[v] + HANF_HUB_CHUNK - 1) / HANF_HUB_CHUNK;
+    }
+    uint64_t incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint64_t t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint64_t before = 0, sum = 0;
+    for (int k = 0; k < 4; ++k) {
+      before += k < wave ? s_wave[k] : 0ull;
+      sum += s_wave[k];
+    }
+    __syncthreads();
+    if (h < n) start[h] = carry + before + incl - c;
+    carry += sum;
+  }
+  if (threadIdx.x == 0) start[n] = carry;
+}
+
+// Register-wise max of x into *p (a compare-and-swap loop: every retry sees a word that
+// only grew, so it ends).
+__device__ __forceinline__ void byte_max_merge(uint32_t* p, uint32_t x) {
+  uint32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {
+    const uint32_t nv = byte_max4(old, x);
+    if (nv == old) return;
+    const uint32_t prev = atomicCAS(p, old, nv);
+    if (prev == old) return;
+    old = prev;
+  }
+}
+
+// Hub in-edges, one chunk per wave at a time: the chunk's register-wise max (rows of
+// wpr words; up to 64 words: 64 / wpr in-edges per step, larger rows 64 words per step
+// with the in-edges walked once per 64-word block) merged into the hub's next row.
+__global__ __launch_bounds__(256) void k_hanf_pull_hubs(const uint64_t* __restrict__ col_ptr,
+                                                        const uint32_t* __restrict__ row_idx,
+                                                        const uint32_t* __restrict__ hubs,
+                                                        const unsigned long long* __restrict__ nhub,
+                                                        const uint64_t* __restrict__ start, uint64_t wpr,
+                                                        const uint32_t* __restrict__ cur, uint32_t* __restrict__ nxt) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  const uint64_t n = *nhub;
+  if (n == 0) return;
+  const uint64_t total = start[n];
+  for (uint64_t c = wave_id; c < total; c += nwaves) {
+    uint64_t lo = 0, hi = n - 1;  // the hub of chunk c: last h with start[h] <= c
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t v = hubs[lo];
+    const uint64_t e = col_ptr[v + 1];
+    const uint64_t k0 = col_ptr[v] + (c - start[lo]) * HANF_HUB_CHUNK;
+    const uint64_t k1 = std::min<uint64_t>(e, k0 + HANF_HUB_CHUNK);
+    if (wpr <= 64) {
+      const uint64_t eps = 64 / wpr, slot = (uint64_t)lane / wpr, w = (uint64_t)lane % wpr;
+      uint32_t acc = 0;
+      for (uint64_t k = k0 + slot; k < k1; k += eps * 4) {
+        uint32_t val[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint64_t kk = k + (uint64_t)u * eps;
+          val[u] = kk < k1 ? cur[(uint64_t)row_idx[kk] * wpr + w] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = byte_max4(acc, val[u]);
+      }
+      for (uint64_t off = wpr; off < 64; off <<= 1) acc = byte_max4(acc, (uint32_t)__shfl_xor(acc, (int)off, 64));
+      if (slot == 0) byte_max_merge(&nxt[(uint64_t)v * wpr + w], acc);
+    } else {
+      for (uint64_t cw = lane; cw < wpr; cw += 64) {
+        uint32_t acc = 0;
+        for (uint64_t k = k0; k < k1; ++k) acc = byte_max4(acc, cur[(uint64_t)row_idx[k] * wpr + cw]);
+        byte_max_merge(&nxt[(uint64_t)v * wpr + cw], acc);
+      }
     }
   }
 }
@@ -205,6 +331,11 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
     std::vector<double> hpart(est_grid);
     const unsigned pull_grid = persistent_grid(ctx, grid_for(V, 4), 8);
     const uint32_t* perm = g->relabeled ? g->perm.p : nullptr;
+    // hubs (every one holds more than HANF_HUB_DEG of the nnz in-edges)
+    const uint64_t nwords = ((uint64_t)V >> 6) + 1, max_hubs = g->nnz / (HANF_HUB_DEG + 1) + 1;
+    dbuf<uint64_t> hub_bits(ctx, nwords), hub_start(ctx, max_hubs + 1);
+    dbuf<uint32_t> hubs(ctx, max_hubs);
+    dbuf<unsigned long long> nhub(ctx, 1);
     auto anf_of = [&](const uint8_t* regs) {
       k_hanf_estimate<<<est_grid, 256, 0, s>>>(regs, V, P, part.p);
       PE_HIP(hipGetLastError());
@@ -233,9 +364,15 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
         HANF_W(14, 16, 2) HANF_W(15, 16, 2) HANF_W(16, 16, 2)
 #undef HANF_W
       }
+      k_hanf_pull_hubs<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, hubs.p, nhub.p, hub_start.p, m / 4,
+                                                 (const uint32_t*)cur, (uint32_t*)nxt);
       PE_HIP(hipGetLastError());
     };
     PE_HIP(hipEventRecord(ctx->t0, s));
+    PE_HIP(hipMemsetAsync(nhub.p, 0, sizeof(unsigned long long), s));
+    k_hanf_hub_bits<<<grid_for(nwords * 64, 256, 8192), 256, 0, s>>>(g->col_ptr.p, V, nwords, hub_bits.p);
+    frontier_compact(ctx, hub_bits.p, nwords, V, hubs.p, nhub.p);
+    k_hanf_hub_chunks<<<1, 256, 0, s>>>(g->col_ptr.p, hubs.p, nhub.p, hub_start.p);
     k_hanf_zero<<<grid_for(bytes / 4, 256, 65536), 256, 0, s>>>((uint32_t*)a.p, bytes / 4);
     k_hanf_init<<<grid_for(V, 256, 8192), 256, 0, s>>>(a.p, V, P, perm);
     PE_HIP(hipGetLastError());

@@ -44,7 +44,10 @@ def test_reference_test_case(ctx):
 
 
 @pytest.mark.parametrize("scale,bits,directed,relabel", [
-    (12, 6, True, False), (12, 7, False, True), (12, 8, True, True), (10, 12, False, False), (8, 14, True, False)])
+    (12, 6, True, False), (12, 7, False, True), (12, 8, True, True), (10, 12, False, False), (8, 14, True, False),
+    # symmetrized scale 14 has a vertex of 5561 in-edges: the hub pass (chunks of in-edges
+    # merged with compare-and-swap), for rows of 16, 32, 64 and 128 words
+    (14, 6, False, True), (14, 7, False, False), (14, 8, False, True), (14, 9, False, False)])
 def test_rmat(ctx, scale, bits, directed, relabel):
     from plato_amd import Graph
     print(f"case: hyperanf rmat s{scale} bits={bits} directed={directed} relabel={relabel}", flush=True)
