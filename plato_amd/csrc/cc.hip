@@ -20,6 +20,8 @@
 namespace pe {
 
 static constexpr uint32_t CC_PULL_CAP = 64;  // in-edges a pull lane takes; longer go one wave each
+static constexpr uint64_t CC_HUB_DEG = 4096;    // longer still: chunks of CC_HUB_CHUNK over all waves
+static constexpr uint64_t CC_HUB_CHUNK = 2048;
 
 struct cc_counters {
   unsigned long long activated;  // sum of the slots' returns
@@ -133,6 +135,7 @@ __global__ __launch_bounds__(256) void k_cc_pull_heavy(const uint64_t* __restric
   for (uint64_t h = wave_id; h < n; h += nwaves) {
     const uint32_t v = heavy[h];
     const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    if (e - b > CC_HUB_DEG) continue;  // k_cc_pull_hubs
     uint32_t cur = perm ? perm[v] : v;
     for (uint64_t k = b + lane; k < e; k += 64) cur = std::min(cur, label[row_idx[k]]);
     for (int off = 32; off > 0; off >>= 1) cur = std::min(cur, (uint32_t)__shfl_xor(cur, off, 64));
@@ -143,6 +146,37 @@ __global__ __launch_bounds__(256) void k_cc_pull_heavy(const uint64_t* __restric
     }
   }
   if (lane == 0 && act) atomicAdd(&cnt->activated, act);
+}
+
+// Pull for the hubs (more than CC_HUB_DEG in-edges), one chunk of in-edges per wave at
+// a time: the chunk's minimum label lowers the hub's with atomicMin (several chunks of one
+// hub may), and the hub counts once, when its bit in the next bitmap is first set.  A
+// hub's own id is never below its label, so the chunks leave it out.
+__global__ __launch_bounds__(256) void k_cc_pull_hubs(const uint64_t* __restrict__ col_ptr,
+                                                      const uint32_t* __restrict__ row_idx,
+                                                      const uint32_t* __restrict__ hubs,
+                                                      const unsigned long long* __restrict__ nhub,
+                                                      const uint64_t* __restrict__ start, uint32_t* __restrict__ label,
+                                                      uint64_t* __restrict__ next_bits, cc_counters* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  const uint64_t n = *nhub;
+  unsigned long long act = 0;
+  if (n) {
+    const uint64_t total = start[n];
+    for (uint64_t c = wave_id; c < total; c += nwaves) {
+      uint64_t k0, k1;
+      const uint32_t v = hub_chunk(col_ptr, hubs, start, n, CC_HUB_CHUNK, c, &k0, &k1);
+      uint32_t cur = 0xFFFFFFFFu;
+      for (uint64_t k = k0 + lane; k < k1; k += 64) cur = std::min(cur, label[row_idx[k]]);
+      for (int off = 32; off > 0; off >>= 1) cur = std::min(cur, (uint32_t)__shfl_xor(cur, off, 64));
+      if (lane == 0 && cur < label[v] && cur < atomicMin(&label[v], cur)) {
+        const unsigned long long bit = 1ull << (v & 63);
+        if (!(atomicOr((unsigned long long*)&next_bits[v >> 6], bit) & bit)) ++act;
+      }
+    }
+  }
+  cc_add(act, &cnt->activated);
 }
 
 }  // namespace pe
@@ -170,6 +204,8 @@ extern "C" int pe_connected_components(pe_graph_t* g, uint32_t* labels, uint8_t*
     PE_HIP(hipGetLastError());
     const unsigned push_grid = persistent_grid(ctx, 1u << 20, 8);
     const unsigned pull_grid = persistent_grid(ctx, grid_for(nwords * 64, 256), 8);
+    hub_table hubs;
+    hub_table_build(ctx, g->col_ptr.p, V, g->nnz, CC_HUB_DEG, CC_HUB_CHUNK, hubs);
     uint64_t A = active_edges_dev(g, cur.p, nwords);
     uint32_t epoch = 0;
     for (uint64_t act = 1; act != 0; ++epoch) {
@@ -184,6 +220,8 @@ extern "C" int pe_connected_components(pe_graph_t* g, uint32_t* labels, uint8_t*
                                             cnt.p);
         frontier_compact(ctx, hbits.p, nwords, V, heavy.p, &cnt.p->heavy);
         k_cc_pull_heavy<<<push_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, perm, label.p, nxt.p, heavy.p, cnt.p);
+        k_cc_pull_hubs<<<push_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, hubs.hubs.p, hubs.count.p, hubs.start.p,
+                                                 label.p, nxt.p, cnt.p);
       }
       PE_HIP(hipGetLastError());
       PE_HIP(hipMemcpyAsync(h, cnt.p, sizeof(cc_counters), hipMemcpyDeviceToHost, s));

@@ -147,55 +147,6 @@ __global__ __launch_bounds__(256) void k_hanf_pull_words(const uint64_t* __restr
   }
 }
 
-// Hubs: a bit per vertex with more than HANF_HUB_DEG in-edges (one word per wave).
-__global__ __launch_bounds__(256) void k_hanf_hub_bits(const uint64_t* __restrict__ col_ptr, uint32_t V,
-                                                       uint64_t nwords, uint64_t* __restrict__ bits) {
-  const int lane = threadIdx.x & 63;
-  for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
-       word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
-    const uint64_t v = word * 64 + lane;
-    const bool hub = v < V && col_ptr[v + 1] - col_ptr[v] > HANF_HUB_DEG;
-    const uint64_t m = __ballot(hub);
-    if (lane == 0) bits[word] = m;
-  }
-}
-
-// One block: start[h] = first chunk of hub h (chunks of HANF_HUB_CHUNK in-edges),
-// start[n] = the number of chunks.
-__global__ __launch_bounds__(256) void k_hanf_hub_chunks(const uint64_t* __restrict__ col_ptr,
-                                                         const uint32_t* __restrict__ hubs,
-                                                         const unsigned long long* __restrict__ nhub,
-                                                         uint64_t* __restrict__ start) {
-  __shared__ uint64_t s_wave[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t n = *nhub;
-  uint64_t carry = 0;
-  for (uint64_t h0 = 0; h0 < n; h0 += 256) {
-    const uint64_t h = h0 + threadIdx.x;
-    uint64_t c = 0;
-    if (h < n) {
-      const uint32_t v = hubs[h];
-      c = (col_ptr[v + 1] - col_ptr[v] + HANF_HUB_CHUNK - 1) / HANF_HUB_CHUNK;
-    }
-    uint64_t incl = c;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint64_t t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
-    if (lane == 63) s_wave[wave] = incl;
-    __syncthreads();
-    uint64_t before = 0, sum = 0;
-    for (int k = 0; k < 4; ++k) {
-      before += k < wave ? s_wave[k] : 0ull;
-      sum += s_wave[k];
-    }
-    __syncthreads();
-    if (h < n) start[h] = carry + before + incl - c;
-    carry += sum;
-  }
-  if (threadIdx.x == 0) start[n] = carry;
-}
-
 // Register-wise max of x into *p (a compare-and-swap loop: every retry sees a word that
 // only grew, so it ends).
 __device__ __forceinline__ void byte_max_merge(uint32_t* p, uint32_t x) {
@@ -224,16 +175,8 @@ __global__ __launch_bounds__(256) void k_hanf_pull_hubs(const uint64_t* __restri
   if (n == 0) return;
   const uint64_t total = start[n];
   for (uint64_t c = wave_id; c < total; c += nwaves) {
-    uint64_t lo = 0, hi = n - 1;  // the hub of chunk c: last h with start[h] <= c
-    while (lo < hi) {
-      const uint64_t mid = (lo + hi + 1) >> 1;
-      if (start[mid] <= c) lo = mid;
-      else hi = mid - 1;
-    }
-    const uint32_t v = hubs[lo];
-    const uint64_t e = col_ptr[v + 1];
-    const uint64_t k0 = col_ptr[v] + (c - start[lo]) * HANF_HUB_CHUNK;
-    const uint64_t k1 = std::min<uint64_t>(e, k0 + HANF_HUB_CHUNK);
+    uint64_t k0, k1;
+    const uint32_t v = hub_chunk(col_ptr, hubs, start, n, HANF_HUB_CHUNK, c, &k0, &k1);
     if (wpr <= 64) {
       const uint64_t eps = 64 / wpr, slot = (uint64_t)lane / wpr, w = (uint64_t)lane % wpr;
       uint32_t acc = 0;
@@ -331,11 +274,7 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
     std::vector<double> hpart(est_grid);
     const unsigned pull_grid = persistent_grid(ctx, grid_for(V, 4), 8);
     const uint32_t* perm = g->relabeled ? g->perm.p : nullptr;
-    // hubs (every one holds more than HANF_HUB_DEG of the nnz in-edges)
-    const uint64_t nwords = ((uint64_t)V >> 6) + 1, max_hubs = g->nnz / (HANF_HUB_DEG + 1) + 1;
-    dbuf<uint64_t> hub_bits(ctx, nwords), hub_start(ctx, max_hubs + 1);
-    dbuf<uint32_t> hubs(ctx, max_hubs);
-    dbuf<unsigned long long> nhub(ctx, 1);
+    hub_table hubs;
     auto anf_of = [&](const uint8_t* regs) {
       k_hanf_estimate<<<est_grid, 256, 0, s>>>(regs, V, P, part.p);
       PE_HIP(hipGetLastError());
@@ -364,15 +303,12 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
         HANF_W(14, 16, 2) HANF_W(15, 16, 2) HANF_W(16, 16, 2)
 #undef HANF_W
       }
-      k_hanf_pull_hubs<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, hubs.p, nhub.p, hub_start.p, m / 4,
-                                                 (const uint32_t*)cur, (uint32_t*)nxt);
+      k_hanf_pull_hubs<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, hubs.hubs.p, hubs.count.p,
+                                                 hubs.start.p, m / 4, (const uint32_t*)cur, (uint32_t*)nxt);
       PE_HIP(hipGetLastError());
     };
     PE_HIP(hipEventRecord(ctx->t0, s));
-    PE_HIP(hipMemsetAsync(nhub.p, 0, sizeof(unsigned long long), s));
-    k_hanf_hub_bits<<<grid_for(nwords * 64, 256, 8192), 256, 0, s>>>(g->col_ptr.p, V, nwords, hub_bits.p);
-    frontier_compact(ctx, hub_bits.p, nwords, V, hubs.p, nhub.p);
-    k_hanf_hub_chunks<<<1, 256, 0, s>>>(g->col_ptr.p, hubs.p, nhub.p, hub_start.p);
+    hub_table_build(ctx, g->col_ptr.p, V, g->nnz, HANF_HUB_DEG, HANF_HUB_CHUNK, hubs);
     k_hanf_zero<<<grid_for(bytes / 4, 256, 65536), 256, 0, s>>>((uint32_t*)a.p, bytes / 4);
     k_hanf_init<<<grid_for(V, 256, 8192), 256, 0, s>>>(a.p, V, P, perm);
     PE_HIP(hipGetLastError());

@@ -287,6 +287,35 @@ uint64_t active_edges_dev(pe_graph_s* g, const uint64_t* words, uint64_t nwords)
 // bitmap -> queue of its set ids below V, appended at *count (bfs.hip)
 void frontier_compact(pe_ctx_s* ctx, const uint64_t* words, uint64_t nwords, uint32_t V, uint32_t* queue,
                       unsigned long long* count);
+// Hubs of an adjacency (ptr[V+1]): the vertices with more than `deg` entries, and their
+// entries cut into chunks of `chunk` (hubs.hip).  A kernel that would give a whole
+// adjacency to one wave leaves the hubs to a pass over their chunks instead, spread over
+// every wave, so no wave walks a hub alone.  Built on the device (counts stay there).
+struct hub_table {
+  uint64_t deg = 0, chunk = 0, max_hubs = 0;
+  dbuf<uint32_t> hubs;           // hub ids (ascending within each compaction block's slice)
+  dbuf<uint64_t> start;          // [max_hubs + 1]: first chunk of hub h; start[*count] = all chunks
+  dbuf<unsigned long long> count;  // [1] number of hubs
+  dbuf<uint64_t> bits;           // scratch: hub bitmap
+};
+void hub_table_build(pe_ctx_s* ctx, const uint64_t* ptr, uint32_t V, uint64_t nnz, uint64_t deg, uint64_t chunk,
+                     hub_table& t);
+// the hub and edge range [k0, k1) of chunk c (c below start[n], n = *t.count > 0)
+__device__ __forceinline__ uint32_t hub_chunk(const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ hubs,
+                                              const uint64_t* __restrict__ start, uint64_t n, uint64_t chunk,
+                                              uint64_t c, uint64_t* k0, uint64_t* k1) {
+  uint64_t lo = 0, hi = n - 1;  // last h with start[h] <= c
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t v = hubs[lo];
+  const uint64_t e = ptr[v + 1];
+  *k0 = ptr[v] + (c - start[lo]) * chunk;
+  *k1 = *k0 + chunk < e ? *k0 + chunk : e;
+  return v;
+}
 // vectors
 struct vec_view {
   int dtype;
