@@ -18,7 +18,8 @@
 // histogram (values at or above KC_HIST share the last bin; when the answer reaches
 // it, a binary search over [KC_HIST, est] with wave-wide counts finishes the job), and
 // adjacencies above KC_HUGE to a third, one 1024-thread block per vertex with a
-// KC_HHIST-bin histogram.
+// KC_HHIST-bin histogram; adjacencies above KC_GIANT are cut into chunks whose block
+// histograms add up in global memory, so no block walks a giant alone.
 // Modified vertices set their neighbours' bits in the next bitmap (test, then atomic
 // OR, as the reference's get_bit / set_bit).  Then estimate = coreness for the queue.
 #include <algorithm>
@@ -37,6 +38,8 @@ static constexpr uint32_t KC_HUGE = 4096;    // longer adjacencies take one bloc
 static constexpr int KC_HTHREADS = 1024;
 static constexpr uint32_t KC_HHIST = 16384;  // histogram bins 0..KC_HHIST of the block pass
 static constexpr uint32_t KC_HPER = 17;      // bins per thread in its scan (1024 * 17 > KC_HHIST)
+static constexpr uint64_t KC_GIANT = 65536;  // longer adjacencies are cut into chunks over many blocks
+static constexpr uint64_t KC_GCHUNK = 16384;
 
 struct kc_counters {
   unsigned long long queued;    // active vertices this epoch
@@ -196,22 +199,95 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_big(
   if (lane == 0 && nmod) atomicAdd(&cnt->modified, (unsigned long long)nmod);
 }
 
+// Block-wide h-index from a histogram in LDS (hist[j] = neighbours whose estimate, capped
+// at top, is j, for j <= top = min(cap, KC_HHIST)): the largest j in [1, top] whose suffix
+// sum reaches j; when that is KC_HHIST and cap is larger, a binary search on
+// [KC_HHIST, cap] with block-wide counts over the adjacency [b, e) finishes it.  0 when
+// no j qualifies.  Every thread of the block calls it and gets the answer.
+__device__ uint32_t kc_block_hindex(const uint32_t* hist, uint32_t top, uint32_t cap, uint64_t b, uint64_t e,
+                                    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ est) {
+  __shared__ uint32_t s_wave[KC_HTHREADS / 64];
+  __shared__ uint32_t s_best;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int WAVES = KC_HTHREADS / 64;
+  // thread t owns bins [17 t, 17 t + 17); suffix sums over the threads above it
+  const uint32_t j0 = (uint32_t)tid * KC_HPER;
+  uint32_t own = 0;
+  for (uint32_t j = j0; j < j0 + KC_HPER && j <= top; ++j) own += hist[j];
+  uint32_t incl = own;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_down(incl, off, 64);
+    if (lane + off < 64) incl += t;
+  }
+  if (lane == 0) s_wave[wave] = incl;  // the wave's total
+  if (tid == 0) s_best = 0;
+  __syncthreads();
+  uint32_t sum = incl - own;
+  for (int w = wave + 1; w < WAVES; ++w) sum += s_wave[w];
+  uint32_t best = 0;
+  for (int j = (int)std::min(j0 + KC_HPER - 1, top); j >= (int)j0 && j >= 1; --j) {
+    sum += hist[j];
+    if (sum >= (uint32_t)j) {
+      best = (uint32_t)j;
+      break;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) best = std::max(best, (uint32_t)__shfl_xor(best, off, 64));
+  if (lane == 0 && best) atomicMax(&s_best, best);
+  __syncthreads();
+  best = s_best;
+  if (best == KC_HHIST && cap > KC_HHIST) {
+    // binary search on [KC_HHIST, cap] with block-wide counts
+    uint32_t lo = KC_HHIST, hi = cap;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      uint32_t c = 0;
+      for (uint64_t k = b + tid; k < e; k += KC_HTHREADS) c += est[idx[k]] >= mid ? 1u : 0u;
+      for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+      __syncthreads();
+      if (tid == 0) s_best = 0;
+      __syncthreads();
+      if (lane == 0 && c) atomicAdd(&s_best, c);
+      __syncthreads();
+      if (s_best >= mid) lo = mid;
+      else hi = mid - 1;
+    }
+    best = lo;
+  }
+  __syncthreads();  // s_best and s_wave are free again
+  return best;
+}
+
+// A block's vertex v with h-index `best`: a drop below its coreness is recorded and its
+// neighbours are activated.  Returns 1 for a modified vertex.
+__device__ uint32_t kc_block_finish(uint32_t v, uint32_t best, uint64_t b, uint64_t e,
+                                    const uint32_t* __restrict__ idx, uint32_t* __restrict__ core,
+                                    uint64_t* __restrict__ next_bits) {
+  const bool mod = best && best < core[v];
+  __syncthreads();  // every thread has read core[v]
+  if (mod) {
+    if (threadIdx.x == 0) core[v] = best;
+    for (uint64_t k = b + threadIdx.x; k < e; k += KC_HTHREADS) kc_activate(next_bits, idx[k]);
+  }
+  __syncthreads();
+  return mod ? 1u : 0u;
+}
+
 // One block per queued vertex with more than KC_HUGE neighbours (the hubs, whose
-// adjacency would otherwise keep one wave busy for the whole epoch).
+// adjacency would otherwise keep one wave busy for the whole epoch), up to KC_GIANT;
+// longer ones are left to the chunked passes below.
 __global__ __launch_bounds__(KC_HTHREADS) void k_kc_huge(
     const uint32_t* __restrict__ huge, const kc_counters* __restrict__ cnt_in, const uint64_t* __restrict__ ptr,
     const uint32_t* __restrict__ idx, const uint32_t* __restrict__ est, uint32_t* __restrict__ core,
     uint64_t* __restrict__ next_bits, kc_counters* __restrict__ cnt) {
   __shared__ uint32_t hist[KC_HTHREADS * KC_HPER];
-  __shared__ uint32_t s_wave[KC_HTHREADS / 64];
-  __shared__ uint32_t s_best;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int WAVES = KC_HTHREADS / 64;
+  const int tid = threadIdx.x;
   const uint64_t nhuge = cnt_in->nhuge;
   uint32_t nmod = 0;
   for (uint64_t q = blockIdx.x; q < nhuge; q += gridDim.x) {
     const uint32_t v = huge[q];
     const uint64_t b = ptr[v], e = ptr[v + 1];
+    if (e - b > KC_GIANT) continue;  // uniform: k_kc_giant_*
     const uint32_t cap = (uint32_t)std::min<uint64_t>(e - b, (uint64_t)est[v]);
     const uint32_t top = std::min(cap, KC_HHIST);
     for (uint32_t j = tid; j <= top; j += KC_HTHREADS) hist[j] = 0;
@@ -221,58 +297,71 @@ __global__ __launch_bounds__(KC_HTHREADS) void k_kc_huge(
       atomicAdd(&hist[eu < top ? eu : top], 1u);
     }
     __syncthreads();
-    // thread t owns bins [17 t, 17 t + 17); suffix sums over the threads above it
-    const uint32_t j0 = (uint32_t)tid * KC_HPER;
-    uint32_t own = 0;
-    for (uint32_t j = j0; j < j0 + KC_HPER && j <= top; ++j) own += hist[j];
-    uint32_t incl = own;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t t = __shfl_down(incl, off, 64);
-      if (lane + off < 64) incl += t;
-    }
-    if (lane == 0) s_wave[wave] = incl;  // the wave's total
-    if (tid == 0) s_best = 0;
+    const uint32_t best = kc_block_hindex(hist, top, cap, b, e, idx, est);
+    nmod += kc_block_finish(v, best, b, e, idx, core, next_bits);
+  }
+  if (tid == 0 && nmod) atomicAdd(&cnt->modified, (unsigned long long)nmod);
+}
+
+// Giants (more than KC_GIANT neighbours): their adjacencies in chunks of KC_GCHUNK, one
+// block per chunk, each adding its LDS histogram of the neighbours' capped estimates
+// into the giant's histogram in global memory (ghist, KC_HHIST + 1 bins per giant);
+// then one block per giant finds the h-index from it.  Inactive giants are skipped.
+__global__ void k_kc_giant_zero(uint32_t* __restrict__ ghist, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    ghist[i] = 0u;
+}
+
+__global__ __launch_bounds__(KC_HTHREADS) void k_kc_giant_hist(
+    const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ giants,
+    const unsigned long long* __restrict__ ngiant, const uint64_t* __restrict__ start,
+    const uint64_t* __restrict__ active, const uint32_t* __restrict__ est, uint32_t* __restrict__ ghist) {
+  __shared__ uint32_t hist[KC_HHIST + 1];
+  const int tid = threadIdx.x;
+  const uint64_t n = *ngiant;
+  if (n == 0) return;
+  const uint64_t total = start[n];
+  for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+    uint64_t k0, k1, gi;
+    const uint32_t v = hub_chunk(ptr, giants, start, n, KC_GCHUNK, c, &k0, &k1, &gi);
+    if (!((active[v >> 6] >> (v & 63)) & 1ull)) continue;  // uniform
+    const uint64_t len = ptr[v + 1] - ptr[v];
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(len, (uint64_t)est[v]);
+    const uint32_t top = std::min(cap, KC_HHIST);
+    for (uint32_t j = tid; j <= top; j += KC_HTHREADS) hist[j] = 0;
     __syncthreads();
-    uint32_t sum = incl - own;
-    for (int w = wave + 1; w < WAVES; ++w) sum += s_wave[w];
-    uint32_t best = 0;
-    for (int j = (int)std::min(j0 + KC_HPER - 1, top); j >= (int)j0 && j >= 1; --j) {
-      sum += hist[j];
-      if (sum >= (uint32_t)j) {
-        best = (uint32_t)j;
-        break;
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) best = std::max(best, (uint32_t)__shfl_xor(best, off, 64));
-    if (lane == 0 && best) atomicMax(&s_best, best);
-    __syncthreads();
-    best = s_best;
-    if (best == KC_HHIST && cap > KC_HHIST) {
-      // binary search on [KC_HHIST, cap] with block-wide counts
-      uint32_t lo = KC_HHIST, hi = cap;
-      while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo + 1) / 2;
-        uint32_t c = 0;
-        for (uint64_t k = b + tid; k < e; k += KC_HTHREADS) c += est[idx[k]] >= mid ? 1u : 0u;
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-        __syncthreads();
-        if (tid == 0) s_best = 0;
-        __syncthreads();
-        if (lane == 0 && c) atomicAdd(&s_best, c);
-        __syncthreads();
-        if (s_best >= mid) lo = mid;
-        else hi = mid - 1;
-      }
-      best = lo;
-    }
-    const bool mod = best && best < core[v];
-    __syncthreads();  // every thread has read core[v]
-    if (mod) {
-      if (tid == 0) core[v] = best;
-      for (uint64_t k = b + tid; k < e; k += KC_HTHREADS) kc_activate(next_bits, idx[k]);
-      ++nmod;
+    for (uint64_t k = k0 + tid; k < k1; k += KC_HTHREADS) {
+      const uint32_t eu = est[idx[k]];
+      atomicAdd(&hist[eu < top ? eu : top], 1u);
     }
     __syncthreads();
+    uint32_t* gh = ghist + gi * (uint64_t)(KC_HHIST + 1);
+    for (uint32_t j = tid; j <= top; j += KC_HTHREADS)
+      if (hist[j]) atomicAdd(&gh[j], hist[j]);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(KC_HTHREADS) void k_kc_giant_final(
+    const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ giants,
+    const unsigned long long* __restrict__ ngiant, const uint64_t* __restrict__ active,
+    const uint32_t* __restrict__ est, const uint32_t* __restrict__ ghist, uint32_t* __restrict__ core,
+    uint64_t* __restrict__ next_bits, kc_counters* __restrict__ cnt) {
+  __shared__ uint32_t hist[KC_HTHREADS * KC_HPER];
+  const int tid = threadIdx.x;
+  const uint64_t n = *ngiant;
+  uint32_t nmod = 0;
+  for (uint64_t gi = blockIdx.x; gi < n; gi += gridDim.x) {
+    const uint32_t v = giants[gi];
+    if (!((active[v >> 6] >> (v & 63)) & 1ull)) continue;  // uniform
+    const uint64_t b = ptr[v], e = ptr[v + 1];
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(e - b, (uint64_t)est[v]);
+    const uint32_t top = std::min(cap, KC_HHIST);
+    const uint32_t* gh = ghist + gi * (uint64_t)(KC_HHIST + 1);
+    for (uint32_t j = tid; j <= top; j += KC_HTHREADS) hist[j] = gh[j];
+    __syncthreads();
+    const uint32_t best = kc_block_hindex(hist, top, cap, b, e, idx, est);
+    nmod += kc_block_finish(v, best, b, e, idx, core, next_bits);
   }
   if (tid == 0 && nmod) atomicAdd(&cnt->modified, (unsigned long long)nmod);
 }
@@ -313,6 +402,14 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
     const unsigned big_grid = persistent_grid(ctx, 1u << 20, 4);
     const unsigned small_grid = persistent_grid(ctx, 1u << 20, 8);
     const unsigned huge_grid = persistent_grid(ctx, 1u << 20, 1);
+    // giants and their histograms (one per giant, KC_HHIST + 1 bins)
+    hub_table giants;
+    hub_table_build(ctx, ptr, V, g->nnz, KC_GIANT, KC_GCHUNK, giants);
+    unsigned long long ngiant = 0;
+    PE_HIP(hipMemcpyAsync(&ngiant, giants.count.p, sizeof(ngiant), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    const uint64_t ghist_words = ngiant * (uint64_t)(KC_HHIST + 1);
+    dbuf<uint32_t> ghist(ctx, std::max<uint64_t>(1, ghist_words));
     uint32_t epoch = 0;
     uint64_t total_mod = 0;
     for (;;) {
@@ -335,6 +432,13 @@ extern "C" int pe_kcore(pe_graph_t* g, int out_edges, uint32_t* coreness, uint64
         frontier_compact(ctx, huge_bits.p, nwords, V, huge.p, &cnt.p->nhuge);
         k_kc_big<<<big_grid, KC_THREADS, 0, s>>>(big.p, cnt.p, ptr, idx, est.p, core.p, nxt.p, cnt.p);
         k_kc_huge<<<huge_grid, KC_HTHREADS, 0, s>>>(huge.p, cnt.p, ptr, idx, est.p, core.p, nxt.p, cnt.p);
+        if (ngiant) {
+          k_kc_giant_zero<<<grid_for(ghist_words, 256, 4096), 256, 0, s>>>(ghist.p, ghist_words);
+          k_kc_giant_hist<<<huge_grid, KC_HTHREADS, 0, s>>>(ptr, idx, giants.hubs.p, giants.count.p, giants.start.p,
+                                                            cur.p, est.p, ghist.p);
+          k_kc_giant_final<<<(unsigned)std::min<uint64_t>(ngiant, huge_grid), KC_HTHREADS, 0, s>>>(
+              ptr, idx, giants.hubs.p, giants.count.p, cur.p, est.p, ghist.p, core.p, nxt.p, cnt.p);
+        }
         k_kc_sync<<<grid_for(n, 256, 8192), 256, 0, s>>>(queue.p, n, core.p, est.p);
       }
       PE_HIP(hipGetLastError());

@@ -300,10 +300,11 @@ struct hub_table {
 };
 void hub_table_build(pe_ctx_s* ctx, const uint64_t* ptr, uint32_t V, uint64_t nnz, uint64_t deg, uint64_t chunk,
                      hub_table& t);
-// the hub and edge range [k0, k1) of chunk c (c below start[n], n = *t.count > 0)
+// the hub (its id, and its index h in the table) and edge range [k0, k1) of chunk c
+// (c below start[n], n = *t.count > 0)
 __device__ __forceinline__ uint32_t hub_chunk(const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ hubs,
                                               const uint64_t* __restrict__ start, uint64_t n, uint64_t chunk,
-                                              uint64_t c, uint64_t* k0, uint64_t* k1) {
+                                              uint64_t c, uint64_t* k0, uint64_t* k1, uint64_t* h = nullptr) {
   uint64_t lo = 0, hi = n - 1;  // last h with start[h] <= c
   while (lo < hi) {
     const uint64_t mid = (lo + hi + 1) >> 1;
@@ -312,6 +313,7 @@ __device__ __forceinline__ uint32_t hub_chunk(const uint64_t* __restrict__ ptr, 
   }
   const uint32_t v = hubs[lo];
   const uint64_t e = ptr[v + 1];
+  if (h) *h = lo;
   *k0 = ptr[v] + (c - start[lo]) * chunk;
   *k1 = *k0 + chunk < e ? *k0 + chunk : e;
   return v;

@@ -83,3 +83,27 @@ def test_big_vertex_search_above_histogram(ctx):
     g = Graph.from_edges(ctx, s, d, V, directed=False)
     res = _check(g, oracle.CSC(V, s, d, symmetrize=True))
     assert res.coreness[:n].tolist() == [n - 1] * n and res.coreness[n:].tolist() == [1, 1, 1]
+
+
+def test_giant_hub(ctx):
+    """A vertex of 70000 neighbours (above the 65536 at which its adjacency is cut into
+    chunks whose histograms add up in global memory), on a wheel whose rim carries random
+    chords, so the hub's h-index moves over several epochs; directed and relabeled too."""
+    from plato_amd import Graph
+    rim = 70000
+    rng = np.random.default_rng(8)
+    r = np.arange(1, rim + 1, dtype=np.uint32)
+    chord_a = rng.integers(1, rim + 1, 3 * rim).astype(np.uint32)
+    chord_b = rng.integers(1, rim + 1, 3 * rim).astype(np.uint32)
+    s = np.concatenate([np.zeros(rim, np.uint32), r, chord_a])
+    d = np.concatenate([r, r % rim + 1, chord_b])
+    V = rim + 1
+    print(f"case: kcore giant hub V={V} E={len(s)}", flush=True)
+    og = oracle.CSC(V, s, d, symmetrize=True)
+    assert int(np.diff(og.col_ptr)[0]) > 65536
+    res = _check(Graph.from_edges(ctx, s, d, V, directed=False), og)
+    assert res.coreness[0] >= 3
+    _check(Graph.from_edges(ctx, s, d, V, directed=False, relabel=True), og)
+    # directed: the hub's 70000 in-edges come from the rim's reversed edges
+    s2, d2 = np.concatenate([s, r]), np.concatenate([d, np.zeros(rim, np.uint32)])
+    _check(Graph.from_edges(ctx, s2, d2, V, build_csr=True), oracle.CSC(V, s2, d2))
