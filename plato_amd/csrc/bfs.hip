@@ -95,12 +95,6 @@ __global__ void k_bfs_reset(bfs_counters* __restrict__ cnt) {
   cnt->heavy_count = 0;
 }
 
-__global__ void k_zero_words(uint64_t* __restrict__ w, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    w[i] = 0ull;
-}
-
 // wave-aggregated append of the lanes in `mask`; returns this lane's slot
 __device__ __forceinline__ unsigned long long wave_append(uint64_t mask,
                                                           unsigned long long* counter) {

@@ -67,25 +67,6 @@ __device__ __forceinline__ float rmat_weight(uint64_t sk, uint64_t e) {
   return (float)((r >> 40) + 1u) * (1.0f / 16777216.0f);
 }
 
-// keys for the CSC sort: (dst << b) | src; for the CSR sort (src << b) | dst.
-// Entries [E, 2E) are the reversals when symmetrizing.
-__global__ void k_rmat_keys(rmat_params p, uint64_t E, uint64_t e0, int b, int csr, int sym,
-                            uint64_t* __restrict__ keys, float* __restrict__ w) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t s, d;
-    rmat_edge(p, e0 + i, s, d);
-    const uint64_t maj = csr ? s : d, mnr = csr ? d : s;
-    keys[i] = (maj << b) | mnr;
-    if (sym) keys[E + i] = (mnr << b) | maj;
-    if (w) {
-      const float wt = rmat_weight(p.sk, e0 + i);
-      w[i] = wt;
-      if (sym) w[E + i] = wt;
-    }
-  }
-}
-
 __global__ void k_rmat_edges(rmat_params p, uint64_t e0, uint64_t n, uint32_t* __restrict__ src,
                              uint32_t* __restrict__ dst, float* __restrict__ w) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
