@@ -202,7 +202,77 @@ __global__ __launch_bounds__(256) void k_hanf_pull_hubs(const uint64_t* __restri
   }
 }
 
-// Per-vertex estimate (hyperloglog.hpp:77-116), summed per block in a fixed order.
+// 2^-r exactly, from the exponent bits (r <= 33)
+__device__ __forceinline__ double hanf_pow2_neg(uint32_t r) {
+  return __longlong_as_double((long long)(1023u - r) << 52);
+}
+
+// The estimate of one counter from its register sum s (exact: at most 2^16 terms of 2^-r)
+// and its zero registers (hyperloglog.hpp:77-116, with its small- and large-range
+// corrections); alpha is alpha_m * m^2.
+__device__ __forceinline__ double hanf_estimate(double alpha, double m, double s, uint32_t zeros) {
+  double est = alpha / s;
+  if (est <= 5.0 / 2.0 * m) {
+    if (zeros) est = m * log(m / zeros);
+  } else if (est > (1.0 / 30.0) * 4294967296.0) {
+    est = -4294967296.0 * log(1.0 - (est / 4294967296.0));
+  }
+  return est;
+}
+
+__device__ __forceinline__ double hanf_alpha(int P) {
+  const double m = (double)(1u << P);
+  double alpha;
+  switch (P) {
+    case 4: alpha = 0.673; break;
+    case 5: alpha = 0.697; break;
+    case 6: alpha = 0.709; break;
+    default: alpha = 0.7213 / (1.0 + 1.079 / m); break;
+  }
+  return alpha * m * m;
+}
+
+// Estimates of counters of at most 1024 registers, summed per block in a fixed order:
+// m / 16 lanes per counter (16 registers each, one 16-byte load), 1024 / m counters per
+// wave step, so a wave keeps many counters' loads in flight instead of one.
+__global__ __launch_bounds__(256) void k_hanf_estimate_rows(const uint8_t* __restrict__ regs, uint32_t V, int P,
+                                                            double* __restrict__ part) {
+  __shared__ double s_w[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t m = 1u << P, L = m / 16, sub = (uint32_t)lane % L, slot = (uint32_t)lane / L, per = 64 / L;
+  const double alpha = hanf_alpha(P);
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  double total = 0.0;  // lane's running sum over its counters, in order
+  for (uint64_t v0 = wave_id * per; v0 < V; v0 += nwaves * per) {
+    const uint64_t v = v0 + slot;
+    double s = 0.0;
+    uint32_t zeros = 0;
+    if (v < V) {
+      const uint4 q = *reinterpret_cast<const uint4*>(regs + v * m + (uint64_t)sub * 16);
+      const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int b = 0; b < 32; b += 8) {
+          const uint32_t r = (w4[i] >> b) & 0xFFu;
+          s += hanf_pow2_neg(r);
+          zeros += r == 0 ? 1u : 0u;
+        }
+    }
+    for (uint32_t off = 1; off < L; off <<= 1) {
+      s += __shfl_xor(s, (int)off, 64);
+      zeros += (uint32_t)__shfl_xor((int)zeros, (int)off, 64);
+    }
+    if (v < V && sub == 0) total += hanf_estimate(alpha, (double)m, s, zeros);
+  }
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_down(total, off, 64);
+  if (lane == 0) s_w[wave] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Per-vertex estimate (hyperloglog.hpp:77-116) of larger counters, one wave per counter,
+// summed per block in a fixed order.
 // The register sum of 2^-reg is exact in FP64 (at most 2^16 terms of 2^-k, k <= 27).
 __global__ __launch_bounds__(256) void k_hanf_estimate(const uint8_t* __restrict__ regs, uint32_t V, int P,
                                                        double* __restrict__ part) {
@@ -210,34 +280,21 @@ __global__ __launch_bounds__(256) void k_hanf_estimate(const uint8_t* __restrict
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t m = 1ull << P;
   const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
-  double alpha;
-  switch (P) {
-    case 4: alpha = 0.673; break;
-    case 5: alpha = 0.697; break;
-    case 6: alpha = 0.709; break;
-    default: alpha = 0.7213 / (1.0 + 1.079 / (double)m); break;
-  }
-  alpha *= (double)m * (double)m;
+  const double alpha = hanf_alpha(P);
   double total = 0.0;  // lane 0's running sum over this wave's vertices, in order
   for (uint64_t v = wave_id; v < V; v += nwaves) {
     double s = 0.0;
     uint32_t zeros = 0;
     for (uint64_t i = lane; i < m; i += 64) {
       const uint32_t r = regs[v * m + i];
-      s += 1.0 / (double)(1u << r);
+      s += hanf_pow2_neg(r);
       zeros += r == 0 ? 1u : 0u;
     }
     for (int off = 32; off > 0; off >>= 1) {
       s += __shfl_xor(s, off, 64);
       zeros += __shfl_xor(zeros, off, 64);
     }
-    double est = alpha / s;
-    if (est <= 5.0 / 2.0 * (double)m) {
-      if (zeros) est = (double)m * log((double)m / zeros);
-    } else if (est > (1.0 / 30.0) * 4294967296.0) {
-      est = -4294967296.0 * log(1.0 - (est / 4294967296.0));
-    }
-    total += est;
+    total += hanf_estimate(alpha, (double)m, s, zeros);
   }
   if (lane == 0) s_w[wave] = total;
   __syncthreads();
@@ -276,7 +333,10 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
     const uint32_t* perm = g->relabeled ? g->perm.p : nullptr;
     hub_table hubs;
     auto anf_of = [&](const uint8_t* regs) {
-      k_hanf_estimate<<<est_grid, 256, 0, s>>>(regs, V, P, part.p);
+      if (P <= 10)
+        k_hanf_estimate_rows<<<est_grid, 256, 0, s>>>(regs, V, P, part.p);
+      else
+        k_hanf_estimate<<<est_grid, 256, 0, s>>>(regs, V, P, part.p);
       PE_HIP(hipGetLastError());
       PE_HIP(hipMemcpyAsync(hpart.data(), part.p, est_grid * sizeof(double), hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
