@@ -30,6 +30,8 @@ namespace pe {
 
 static constexpr uint64_t HANF_HUB_DEG = 4096;    // in-edges above which a vertex is a hub
 static constexpr uint64_t HANF_HUB_CHUNK = 2048;  // in-edges of a hub per wave task
+static constexpr uint64_t HANF_LOW_DEG = 32;    // counters of up to 128 registers: fewer in-edges
+                                                 // take a lane group per vertex
 
 __device__ __forceinline__ uint32_t hanf_rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
@@ -81,13 +83,15 @@ __device__ __forceinline__ uint32_t byte_max4(uint32_t a, uint32_t b) {
 template <int EPS, int U>
 __global__ __launch_bounds__(256) void k_hanf_pull_small(const uint64_t* __restrict__ col_ptr,
                                                          const uint32_t* __restrict__ row_idx, uint32_t V,
-                                                         const uint32_t* __restrict__ cur, uint32_t* __restrict__ nxt) {
+                                                         uint64_t min_deg, const uint32_t* __restrict__ cur,
+                                                         uint32_t* __restrict__ nxt) {
   constexpr int WPR = 64 / EPS;  // words per counter
   const int lane = threadIdx.x & 63, slot = lane / WPR, w = lane % WPR;
   const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
   for (uint64_t v = wave_id; v < V; v += nwaves) {
-    uint32_t acc = slot == 0 ? cur[v * WPR + w] : 0u;
     const uint64_t b = col_ptr[v], e = col_ptr[v + 1];
+    if (e - b < min_deg) continue;  // k_hanf_pull_lowdeg
+    uint32_t acc = slot == 0 ? cur[v * WPR + w] : 0u;
     if (e - b > HANF_HUB_DEG) {  // a hub: its own row now, its in-edges in k_hanf_pull_hubs
       if (slot == 0) nxt[v * WPR + w] = acc;
       continue;
@@ -104,6 +108,45 @@ __global__ __launch_bounds__(256) void k_hanf_pull_small(const uint64_t* __restr
     }
     for (int off = WPR; off < 64; off <<= 1) acc = byte_max4(acc, (uint32_t)__shfl_xor(acc, off, 64));
     if (slot == 0) nxt[v * WPR + w] = acc;
+  }
+}
+
+// Pull for the vertices with fewer than max_deg in-edges and counters of WPR <= 32
+// words: one group of WPR lanes per vertex, 64 / WPR vertices per wave step, U in-edges
+// of each in flight, so the vertices' short chains of dependent loads overlap.
+template <int WPR, int U>
+__global__ __launch_bounds__(256) void k_hanf_pull_lowdeg(const uint64_t* __restrict__ col_ptr,
+                                                          const uint32_t* __restrict__ row_idx, uint32_t V,
+                                                          uint64_t max_deg, const uint32_t* __restrict__ cur,
+                                                          uint32_t* __restrict__ nxt) {
+  constexpr int G = 64 / WPR;  // vertices per wave step
+  const int lane = threadIdx.x & 63, grp = lane / WPR, w = lane % WPR;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  for (uint64_t v0 = wave_id * G; v0 < V; v0 += nwaves * G) {
+    const uint64_t v = v0 + grp;
+    uint64_t b = 0, e = 0;
+    bool mine = false;
+    if (v < V) {
+      b = col_ptr[v];
+      e = col_ptr[v + 1];
+      mine = e - b < max_deg;
+    }
+    if (!mine) e = b;
+    uint32_t acc = mine ? cur[v * WPR + w] : 0u;
+    // the wave steps until its longest list is done
+    uint64_t len = e - b;
+    for (int off = WPR; off < 64; off <<= 1) len = std::max<uint64_t>(len, (uint64_t)__shfl_xor((long long)len, off, 64));
+    for (uint64_t k0 = 0; k0 < len; k0 += U) {
+      uint32_t val[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t k = b + k0 + u;
+        val[u] = k < e ? cur[(uint64_t)row_idx[k] * WPR + w] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = byte_max4(acc, val[u]);
+    }
+    if (mine) nxt[v * WPR + w] = acc;
   }
 }
 
@@ -347,12 +390,16 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
     auto pull = [&](const uint8_t* cur, uint8_t* nxt) {
       switch (P) {
         case 6:
-          k_hanf_pull_small<4, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, (const uint32_t*)cur,
-                                                            (uint32_t*)nxt);
+          k_hanf_pull_lowdeg<16, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
+                                                              (const uint32_t*)cur, (uint32_t*)nxt);
+          k_hanf_pull_small<4, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
+                                                            (const uint32_t*)cur, (uint32_t*)nxt);
           break;
         case 7:
-          k_hanf_pull_small<2, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, (const uint32_t*)cur,
-                                                            (uint32_t*)nxt);
+          k_hanf_pull_lowdeg<32, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
+                                                              (const uint32_t*)cur, (uint32_t*)nxt);
+          k_hanf_pull_small<2, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
+                                                            (const uint32_t*)cur, (uint32_t*)nxt);
           break;
 #define HANF_W(PP, W, U)                                                                                      \
   case PP:                                                                                                    \
