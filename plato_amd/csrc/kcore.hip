@@ -14,12 +14,13 @@
 //
 // Per epoch: the active bitmap is compacted into a queue; one lane per vertex of at
 // most KC_SMALL neighbours counts them in a byte histogram of its own in LDS; longer
-// adjacencies go to a second queue handled one wave per vertex with a KC_HIST-bin LDS
-// histogram (values at or above KC_HIST share the last bin; when the answer reaches
-// it, a binary search over [KC_HIST, est] with wave-wide counts finishes the job), and
-// adjacencies above KC_HUGE to a third, one 1024-thread block per vertex with a
-// KC_HHIST-bin histogram; adjacencies above KC_GIANT are cut into chunks whose block
-// histograms add up in global memory, so no block walks a giant alone.
+// adjacencies whose answer cannot exceed KC_HIST (min(length, estimate) <= KC_HIST) go to
+// a second queue handled one wave per vertex with a KC_HIST-bin LDS histogram, and the
+// others (adjacencies above KC_HUGE, or estimates above KC_HIST) to a third, one
+// 1024-thread block per vertex with a KC_HHIST-bin histogram (when the answer reaches
+// its last bin, a binary search over [KC_HHIST, est] with block-wide counts finishes
+// the job); adjacencies above KC_GIANT are cut into chunks whose block histograms add
+// up in global memory, so no block walks a giant alone.
 // Modified vertices set their neighbours' bits in the next bitmap (test, then atomic
 // OR, as the reference's get_bit / set_bit).  Then estimate = coreness for the queue.
 #include <algorithm>
@@ -43,8 +44,8 @@ static constexpr uint64_t KC_GCHUNK = 16384;
 
 struct kc_counters {
   unsigned long long queued;    // active vertices this epoch
-  unsigned long long nbig;      // of them, with KC_SMALL < neighbours <= KC_HUGE
-  unsigned long long nhuge;     // of them, with more than KC_HUGE neighbours
+  unsigned long long nbig;      // of them, with KC_SMALL < neighbours <= KC_HUGE and min(neighbours, estimate) <= KC_HIST
+  unsigned long long nhuge;     // of them, the other ones above KC_SMALL neighbours
   unsigned long long modified;  // vertices whose coreness dropped
 };
 
@@ -106,7 +107,10 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_small(
     if (len == 0) {
       core[v] = 0;  // kcore.hpp:214-218
     } else if (len > KC_SMALL) {
-      atomicOr((unsigned long long*)&(len > KC_HUGE ? huge_bits : big_bits)[v >> 6], 1ull << (v & 63));
+      // the block pass also takes the vertices whose answer may exceed the wave pass's
+      // KC_HIST bins (which would need its binary search over the adjacency)
+      const bool block = len > KC_HUGE || std::min<uint64_t>(len, est[v]) > KC_HIST;
+      atomicOr((unsigned long long*)&(block ? huge_bits : big_bits)[v >> 6], 1ull << (v & 63));
     } else {
       const uint32_t cap = std::min<uint32_t>((uint32_t)len, est[v]);
       for (uint32_t j = 0; j <= cap; ++j) h[j] = 0;
@@ -176,19 +180,6 @@ __global__ __launch_bounds__(KC_THREADS) void k_kc_big(
       }
     }
     for (int off = 32; off > 0; off >>= 1) best = std::max(best, (uint32_t)__shfl_xor(best, off, 64));
-    if (best == KC_HIST && cap > KC_HIST) {
-      // the answer may lie above the histogram: binary search on [KC_HIST, cap]
-      uint32_t lo = KC_HIST, hi = cap;
-      while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo + 1) / 2;
-        uint32_t c = 0;
-        for (uint64_t k = b + lane; k < e; k += 64) c += est[idx[k]] >= mid ? 1u : 0u;
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-        if (c >= mid) lo = mid;
-        else hi = mid - 1;
-      }
-      best = lo;
-    }
     if (best && best < core[v]) {
       if (lane == 0) core[v] = best;
       for (uint64_t k = b + lane; k < e; k += 64) kc_activate(next_bits, idx[k]);

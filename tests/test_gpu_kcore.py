@@ -73,7 +73,8 @@ def test_rmat_bit_exact(ctx, scale, directed):
 
 
 def test_big_vertex_search_above_histogram(ctx):
-    """A clique of 1100 vertices (coreness 1099 > 1024 histogram bins) plus a tail."""
+    """A clique of 1100 vertices (coreness 1099, above the wave pass's 1024 histogram
+    bins, so the block pass takes them) plus a tail."""
     from plato_amd import Graph
     n = 1100
     a, b = np.triu_indices(n, 1)
