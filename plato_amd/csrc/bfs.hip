@@ -320,11 +320,11 @@ __global__ void k_first_in(const uint64_t* __restrict__ col_ptr, const uint32_t*
 // are the hubs, which most in-edge lists start with, so most membership tests hit LDS.
 static constexpr int PULL_SCAN_UNROLL = 4;  // in-edges a pull lane loads per round trip
 #ifndef PULL_BLOCKS_PER_CU
-#define PULL_BLOCKS_PER_CU 1
+#define PULL_BLOCKS_PER_CU 2  // two 1024-thread blocks, the most a CU holds
 #endif
 static constexpr uint32_t PULL_THREADS = 1024;
 #ifndef PULL_HOT_WORDS
-#define PULL_HOT_WORDS 16384  // 128 KiB: ids below 2^20
+#define PULL_HOT_WORDS 8192  // 64 KiB: ids below 2^19 (two blocks per CU fit in LDS)
 #endif
 
 __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
