@@ -9,6 +9,9 @@
 
 namespace pe {
 
+static constexpr uint64_t PPR_HUB_DEG = 4096;    // out-edges above which the push takes a vertex in chunks
+static constexpr uint64_t PPR_HUB_CHUNK = 2048;
+
 // x0(v) per the algorithm's init (pagerank.hpp:90-98, weighted_pagerank.hpp:104-109,
 // personalized_pagerank.hpp:71-81, trustrank.hpp:197-206)
 template <typename T>
@@ -73,12 +76,41 @@ __global__ __launch_bounds__(256) void k_ppr_push(const uint64_t* __restrict__ a
       w &= w - 1;
       const uint64_t v = wi * 64 + b;
       if (v >= V) break;
+      if (ptr[v + 1] - ptr[v] > PPR_HUB_DEG) continue;  // k_ppr_push_hubs
       const T xv = x[v];
       for (uint64_t e = ptr[v] + lane; e < ptr[v + 1]; e += 64) {
         const uint32_t dst = idx[e];
         atomicOr((unsigned long long*)&newly[dst >> 6], 1ull << (dst & 63));
         atomicAdd(&y[dst], xv);
       }
+    }
+  }
+}
+
+// Push for the active hubs (more than PPR_HUB_DEG out-edges), one chunk of out-edges per
+// wave at a time, so no wave walks a hub alone: the same scatter as k_ppr_push.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ppr_push_hubs(const uint64_t* __restrict__ active,
+                                                       const uint64_t* __restrict__ ptr,
+                                                       const uint32_t* __restrict__ idx,
+                                                       const uint32_t* __restrict__ hubs,
+                                                       const unsigned long long* __restrict__ nhub,
+                                                       const uint64_t* __restrict__ start, const T* __restrict__ x,
+                                                       T* __restrict__ y, uint64_t* __restrict__ newly) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  const uint64_t n = *nhub;
+  if (n == 0) return;
+  const uint64_t total = start[n];
+  for (uint64_t c = wave_id; c < total; c += nwaves) {
+    uint64_t k0, k1;
+    const uint32_t v = hub_chunk(ptr, hubs, start, n, PPR_HUB_CHUNK, c, &k0, &k1);
+    if (!((active[v >> 6] >> (v & 63)) & 1ull)) continue;  // uniform
+    const T xv = x[v];
+    for (uint64_t e = k0 + lane; e < k1; e += 64) {
+      const uint32_t dst = idx[e];
+      atomicOr((unsigned long long*)&newly[dst >> 6], 1ull << (dst & 63));
+      atomicAdd(&y[dst], xv);
     }
   }
 }
@@ -409,6 +441,8 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
     PE_HIP(hipMemcpyAsync(active.p, ss.words.p, nwords * 8, hipMemcpyDeviceToDevice, s));
     rank_work wk;
     wk.init(g, prec);
+    hub_table hubs;  // out-edge hubs, pushed in chunks
+    hub_table_build(ctx, g->csr_ptr(), V, g->nnz, PPR_HUB_DEG, PPR_HUB_CHUNK, hubs);
     if (prec == 64)
       k_init_rank<double><<<grid_for(V, 256, 8192), 256, 0, s>>>(V, g->od.p, ss.words.p, 1.0, 1, (double*)b0.p);
     else
@@ -435,12 +469,20 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
         else k_zero_t<float><<<zg, 256, 0, s>>>((float*)y, V);
         k_zero_t<uint64_t><<<grid_for(nwords, 256, 4096), 256, 0, s>>>(newly.p, nwords);
         const unsigned pg = grid_for(nwords * 64, 256, 4096);
-        if (prec == 64)
+        const unsigned hg = persistent_grid(ctx, 1u << 20, 4);
+        if (prec == 64) {
           k_ppr_push<double><<<pg, 256, 0, s>>>(active.p, nwords, V, g->csr_ptr(), g->csr_idx(),
                                                (const double*)x, (double*)y, newly.p);
-        else
+          k_ppr_push_hubs<double><<<hg, 256, 0, s>>>(active.p, g->csr_ptr(), g->csr_idx(), hubs.hubs.p,
+                                                     hubs.count.p, hubs.start.p, (const double*)x, (double*)y,
+                                                     newly.p);
+        } else {
           k_ppr_push<float><<<pg, 256, 0, s>>>(active.p, nwords, V, g->csr_ptr(), g->csr_idx(),
                                               (const float*)x, (float*)y, newly.p);
+          k_ppr_push_hubs<float><<<hg, 256, 0, s>>>(active.p, g->csr_ptr(), g->csr_idx(), hubs.hubs.p,
+                                                    hubs.count.p, hubs.start.p, (const float*)x, (float*)y,
+                                                    newly.p);
+        }
         k_or_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(active.p, newly.p, nwords);
         PE_HIP(hipGetLastError());
         rank_apply_only(g, prec, x, y, a, wk);

@@ -209,3 +209,20 @@ def test_relabeled_graph_gives_same_ranks(ctx, s16):
     refp, _, modes = oracle.ppr_pushpull(og, src, 30, D, 0.0)
     rp = personalized_pagerank_pushpull(g, personalized_pagerank_opts_t(30, D, src, 0.0), precision=64)
     assert np.array_equal(rp.modes, modes) and float(np.abs(rp.rank - refp).sum()) <= 1e-13
+
+
+def test_ppr_pushpull_hub_seed(s16):
+    """Seeded at the vertex of most out-edges (more than 4096: its push is cut into chunks
+    over all waves), in both precisions."""
+    from plato_amd.algo import personalized_pagerank_opts_t, personalized_pagerank_pushpull
+    g, og = s16
+    src = int(np.argmax(og.od))
+    assert og.od[src] > 4096
+    ref, rdel, rmodes = oracle.ppr_pushpull(og, src, 30, D, 0.0)
+    assert rmodes[0] == 1
+    res = personalized_pagerank_pushpull(g, personalized_pagerank_opts_t(30, D, src, 0.0), precision=64)
+    assert np.array_equal(res.modes, rmodes)
+    assert float(np.abs(res.rank - ref).sum()) <= 1e-13
+    res32 = personalized_pagerank_pushpull(g, personalized_pagerank_opts_t(30, D, src, 0.0))
+    assert np.array_equal(res32.modes, rmodes)
+    assert float(np.abs(res32.rank - ref).sum()) <= 1e-5
