@@ -30,6 +30,7 @@ namespace pe {
 
 static constexpr uint64_t HANF_HUB_DEG = 4096;    // in-edges above which a vertex is a hub
 static constexpr uint64_t HANF_HUB_CHUNK = 2048;  // in-edges of a hub per wave task
+static constexpr int HANF_HUB_U = 8;             // rows each lane of the hub pass keeps in flight
 static constexpr uint64_t HANF_LOW_DEG = 32;    // counters of up to 128 registers: fewer in-edges
                                                  // take a lane group per vertex
 
@@ -223,15 +224,15 @@ __global__ __launch_bounds__(256) void k_hanf_pull_hubs(const uint64_t* __restri
     if (wpr <= 64) {
       const uint64_t eps = 64 / wpr, slot = (uint64_t)lane / wpr, w = (uint64_t)lane % wpr;
       uint32_t acc = 0;
-      for (uint64_t k = k0 + slot; k < k1; k += eps * 4) {
-        uint32_t val[4];
+      for (uint64_t k = k0 + slot; k < k1; k += eps * HANF_HUB_U) {
+        uint32_t val[HANF_HUB_U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < HANF_HUB_U; ++u) {
           const uint64_t kk = k + (uint64_t)u * eps;
           val[u] = kk < k1 ? cur[(uint64_t)row_idx[kk] * wpr + w] : 0u;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = byte_max4(acc, val[u]);
+        for (int u = 0; u < HANF_HUB_U; ++u) acc = byte_max4(acc, val[u]);
       }
       for (uint64_t off = wpr; off < 64; off <<= 1) acc = byte_max4(acc, (uint32_t)__shfl_xor(acc, (int)off, 64));
       if (slot == 0) byte_max_merge(&nxt[(uint64_t)v * wpr + w], acc);
