@@ -393,7 +393,7 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
         case 6:
           k_hanf_pull_lowdeg<16, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
                                                               (const uint32_t*)cur, (uint32_t*)nxt);
-          k_hanf_pull_small<4, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
+          k_hanf_pull_small<4, 8><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
                                                             (const uint32_t*)cur, (uint32_t*)nxt);
           break;
         case 7:
