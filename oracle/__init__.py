@@ -84,6 +84,18 @@ def lib():
     L.orc_format_csv.argtypes = [C.c_uint64, _u32p, _u32p, C.c_void_p, C.c_uint64]
     L.orc_format_csv.restype = C.c_int64
     L.orc_topk.argtypes = [C.c_uint32, _f64p, C.c_uint32, _u32p, _f64p]
+    L.orc_rmat_weights.argtypes = [C.c_uint64, C.c_uint64, _u64p, _f32p]
+    L.orc_rmat_edges_at.argtypes = [C.c_uint32, C.c_uint64, _u32p, C.c_uint64, _u64p, _u32p, _u32p]
+    L.orc_rmat_scan.argtypes = [C.c_uint32, C.c_uint64, _u32p, C.c_uint64, C.c_uint32, _u32p, _u32p,
+                                _u32p, C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, _u64p]
+    L.orc_rmat_scan.restype = C.c_int64
+    L.orc_edges_digest.argtypes = [C.c_uint64, _u32p, _u32p]
+    L.orc_edges_digest.restype = C.c_uint64
+    L.orc_csc_digest.argtypes = [C.c_uint32, _u64p, _u32p, C.c_void_p]
+    L.orc_csc_digest.restype = C.c_uint64
+    L.orc_weighted_pagerank_step.argtypes = [C.c_uint32, _u64p, _u32p, _f32p, _f64p, C.c_double,
+                                             _f64p, _f64p, C.c_uint32, C.c_uint32]
+    L.orc_weighted_pagerank_step.restype = C.c_double
     _lib = L
     return L
 
@@ -111,9 +123,69 @@ def rmat_edges(scale: int, edge_factor: int, seed: int, a=0.57, b=0.19, c=0.19,
     return src, dst
 
 
-def rmat_weights(seed: int, E: int) -> np.ndarray:
-    L = lib()
-    return np.array([L.orc_rmat_weight(seed, e) for e in range(E)], np.float32)
+def rmat_edges_at(scale: int, seed: int, eids, a=0.57, b=0.19, c=0.19):
+    """The RMAT edges with the given edge ids."""
+    eids = np.ascontiguousarray(eids, np.uint64)
+    src = np.zeros(eids.size, np.uint32)
+    dst = np.zeros(eids.size, np.uint32)
+    lib().orc_rmat_edges_at(scale, seed, rmat_thresholds(a, b, c), eids.size, eids, src, dst)
+    return src, dst
+
+
+def rmat_weights(seed: int, E: int = 0, eids=None) -> np.ndarray:
+    """Weights of edges [0, E), or of the edge ids in `eids`."""
+    if eids is None:
+        eids = np.arange(E, dtype=np.uint64)
+    eids = np.ascontiguousarray(eids, np.uint64)
+    out = np.zeros(eids.size, np.float32)
+    lib().orc_rmat_weights(seed, eids.size, eids, out)
+    return out
+
+
+def edges_digest(src, dst) -> int:
+    """Order-independent digest of an edge multiset (orc_edges_digest)."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    return int(lib().orc_edges_digest(src.size, src, dst))
+
+
+def csc_digest(V, col_ptr, row_idx, perm=None) -> int:
+    """The same digest over a CSC's entries, both ends mapped through perm (internal ->
+    input ids) when given."""
+    cp = np.ascontiguousarray(col_ptr, np.uint64)
+    ri = np.ascontiguousarray(row_idx, np.uint32)
+    pm = None if perm is None else np.ascontiguousarray(perm, np.uint32)
+    return int(lib().orc_csc_digest(int(V), cp, ri, None if pm is None else pm.ctypes.data))
+
+
+class RmatScan:
+    """One pass over an RMAT graph too large to hold twice on the host: out-degrees,
+    per-source weight sums, the edge-multiset digest, and the in-edges of the columns in
+    `ranges` (a list of [lo, hi) pairs) as a CSC over all V columns (empty elsewhere)."""
+
+    def __init__(self, scale, edge_factor, seed, ranges, weighted=False, a=0.57, b=0.19, c=0.19):
+        V, E = 1 << scale, edge_factor << scale
+        lo = np.array([r[0] for r in ranges], np.uint32)
+        hi = np.array([r[1] for r in ranges], np.uint32)
+        self.V, self.E, self.ranges = V, E, [(int(x), int(y)) for x, y in ranges]
+        self.od = np.zeros(V, np.uint32)
+        self.W = np.zeros(V, np.float64) if weighted else None
+        dg = C.c_uint64()
+        thr = rmat_thresholds(a, b, c)
+        cap = max(1 << 20, E // 4)
+        while True:
+            sel = np.zeros(cap, np.uint64)
+            n = lib().orc_rmat_scan(scale, seed, thr, E, lo.size, lo, hi, self.od,
+                                    None if self.W is None else self.W.ctypes.data, C.byref(dg), cap, sel)
+            if n <= cap:
+                break
+            cap = int(n)
+        self.digest = int(dg.value)
+        eids = np.sort(sel[:n])  # edge order, as the structure builders keep it
+        self.eids = eids
+        ss, dd = rmat_edges_at(scale, seed, eids, a, b, c)
+        w = rmat_weights(seed, eids=eids) if weighted else None
+        self.slice = CSC(V, ss, dd, w=w, csr=False)
 
 
 # ---- structures --------------------------------------------------------------
@@ -194,6 +266,19 @@ def weighted_pagerank(g: CSC, iters=20, damping=0.85, eps=0.0):
     if n < 0:
         raise RuntimeError("orc_weighted_pagerank failed")
     return out, deltas[:n]
+
+
+def weighted_pagerank_step(g: CSC, W, x, damping, v_lo=0, v_hi=None, y=None):
+    """One weighted PageRank iteration over columns [v_lo, v_hi) (x: undivided ranks,
+    W: FP64 out-weight sums)."""
+    if v_hi is None:
+        v_hi = g.V
+    if y is None:
+        y = np.zeros(g.V, np.float64)
+    lib().orc_weighted_pagerank_step(g.V, g.col_ptr, np.ascontiguousarray(g.row_idx),
+                                     np.ascontiguousarray(g.w), np.ascontiguousarray(W, np.float64),
+                                     damping, np.ascontiguousarray(x, np.float64), y, v_lo, v_hi)
+    return y
 
 
 def ppr(g: CSC, seeds, iters=100, damping=0.85, eps=0.0):

@@ -39,31 +39,142 @@ void orc_rmat_thresholds(double a, double b, double c, uint32_t out[3]) {
 /* draw(e, j) = mix64(mix64(seed) ^ (e*32 + j)); level l uses the low (l even)
  * or high (l odd) half of draw(e, l/2); level l decides bit (scale-1-l).
  * Quadrant: u < t0 -> (0,0); u < t1 -> dst bit; u < t2 -> src bit; else both. */
+static inline void rmat_one(uint32_t scale, uint64_t sk, const uint32_t thr[3], uint64_t e,
+                            uint32_t* src, uint32_t* dst) {
+  uint32_t s = 0, d = 0;
+  uint64_t word = 0;
+  for (uint32_t l = 0; l < scale; ++l) {
+    if ((l & 1u) == 0) word = orc_mix64(sk ^ (e * 32u + (l >> 1)));
+    const uint32_t u = (l & 1u) ? (uint32_t)(word >> 32) : (uint32_t)word;
+    const uint32_t bit = 1u << (scale - 1u - l);
+    if (u >= thr[0]) {
+      if (u < thr[1]) d |= bit;
+      else if (u < thr[2]) s |= bit;
+      else { s |= bit; d |= bit; }
+    }
+  }
+  *src = s;
+  *dst = d;
+}
+
 void orc_rmat_edges(uint32_t scale, uint64_t seed, const uint32_t thr[3],
                     uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst) {
   const uint64_t sk = orc_mix64(seed);
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint64_t e = e0 + i;
-    uint32_t s = 0, d = 0;
-    uint64_t word = 0;
-    for (uint32_t l = 0; l < scale; ++l) {
-      if ((l & 1u) == 0) word = orc_mix64(sk ^ (e * 32u + (l >> 1)));
-      const uint32_t u = (l & 1u) ? (uint32_t)(word >> 32) : (uint32_t)word;
-      const uint32_t bit = 1u << (scale - 1u - l);
-      if (u >= thr[0]) {
-        if (u < thr[1]) d |= bit;
-        else if (u < thr[2]) s |= bit;
-        else { s |= bit; d |= bit; }
-      }
-    }
-    src[i] = s;
-    dst[i] = d;
-  }
+  /* every edge is a pure function of (seed, edge id): the threads' split does not
+   * change the output */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 65536)
+#endif
+  for (uint64_t i = 0; i < n; ++i) rmat_one(scale, sk, thr, e0 + i, src + i, dst + i);
+}
+
+void orc_rmat_edges_at(uint32_t scale, uint64_t seed, const uint32_t thr[3], uint64_t n,
+                       const uint64_t* eids, uint32_t* src, uint32_t* dst) {
+  const uint64_t sk = orc_mix64(seed);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 4096)
+#endif
+  for (uint64_t i = 0; i < n; ++i) rmat_one(scale, sk, thr, eids[i], src + i, dst + i);
 }
 
 float orc_rmat_weight(uint64_t seed, uint64_t e) {
   const uint64_t r = orc_mix64(orc_mix64(seed) ^ (e * 32u + 31u));
   return (float)((r >> 40) + 1u) * (1.0f / 16777216.0f);
+}
+
+void orc_rmat_weights(uint64_t seed, uint64_t n, const uint64_t* eids, float* out) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 65536)
+#endif
+  for (uint64_t i = 0; i < n; ++i) out[i] = orc_rmat_weight(seed, eids[i]);
+}
+
+/* One pass over edges [0, E) of the generator: out-degrees, FP64 out-weight sums,
+ * the order-independent digest of the (src, dst) multiset, and the ids of the edges
+ * whose destination lies in one of the ranges [lo[r], hi[r]). */
+int64_t orc_rmat_scan(uint32_t scale, uint64_t seed, const uint32_t thr[3], uint64_t E,
+                      uint32_t nranges, const uint32_t* lo, const uint32_t* hi,
+                      uint32_t* od, double* W, uint64_t* digest,
+                      uint64_t cap, uint64_t* sel_eid) {
+  const uint32_t V = 1u << scale;
+  memset(od, 0, sizeof(uint32_t) * V);
+  if (W) memset(W, 0, sizeof(double) * V);
+  uint64_t dg = 0, nsel = 0;
+  const uint64_t B = 1u << 16;
+  const uint64_t nblocks = (E + B - 1) / B;
+#ifdef _OPENMP
+#pragma omp parallel reduction(+ : dg)
+#endif
+  {
+    uint32_t* s = (uint32_t*)malloc(sizeof(uint32_t) * B);
+    uint32_t* d = (uint32_t*)malloc(sizeof(uint32_t) * B);
+    uint64_t* pick = (uint64_t*)malloc(sizeof(uint64_t) * B);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (uint64_t b = 0; b < nblocks; ++b) {
+      const uint64_t e0 = b * B, n = (e0 + B <= E) ? B : E - e0;
+      const uint64_t sk = orc_mix64(seed);
+      uint64_t np = 0;
+      for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t e = e0 + i;
+        uint32_t ss, dd;
+        rmat_one(scale, sk, thr, e, &ss, &dd);
+        s[i] = ss;
+        d[i] = dd;
+        dg += orc_mix64(((uint64_t)ss << 32) | dd);
+        for (uint32_t r = 0; r < nranges; ++r)
+          if (dd >= lo[r] && dd < hi[r]) { pick[np++] = e; break; }
+      }
+      for (uint64_t i = 0; i < n; ++i) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        od[s[i]]++;
+        if (W) {
+          const double w = (double)orc_rmat_weight(seed, e0 + i);
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+          W[s[i]] += w;
+        }
+      }
+      uint64_t at;
+#ifdef _OPENMP
+#pragma omp atomic capture
+#endif
+      { at = nsel; nsel += np; }
+      for (uint64_t i = 0; i < np && at + i < cap; ++i) sel_eid[at + i] = pick[i];
+    }
+    free(s); free(d); free(pick);
+  }
+  *digest = dg;
+  return (int64_t)nsel;
+}
+
+uint64_t orc_edges_digest(uint64_t n, const uint32_t* src, const uint32_t* dst) {
+  uint64_t dg = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : dg) schedule(static, 65536)
+#endif
+  for (uint64_t i = 0; i < n; ++i) dg += orc_mix64(((uint64_t)src[i] << 32) | dst[i]);
+  return dg;
+}
+
+uint64_t orc_csc_digest(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                        const uint32_t* map) {
+  uint64_t dg = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : dg) schedule(dynamic, 4096)
+#endif
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint64_t dv = map ? map[v] : v;
+    for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
+      const uint64_t u = map ? map[row_idx[e]] : row_idx[e];
+      dg += orc_mix64((u << 32) | dv);
+    }
+  }
+  return dg;
 }
 
 /* ---- structures ----------------------------------------------------------- */
@@ -75,68 +186,133 @@ static inline void entry_at(uint64_t E, const uint32_t* src, const uint32_t* dst
   else { *s = dst[i - E]; *d = src[i - E]; }
 }
 
-/* Stable LSD counting sort of entries by (major, minor): first by minor, then
- * by major.  major_is_dst selects CSC (group by dst) or CSR (group by src). */
+/* Entries grouped by major key, sorted by minor key inside a group, equal keys in
+ * entry order: a stable LSD radix sort of the packed key (major << b | minor),
+ * 11 bits per pass.  Each pass is parallel over contiguous slices of the entries
+ * with per-slice digit counts, so it is stable and its output does not depend on
+ * the number of threads.  major_is_dst selects CSC (group by dst) or CSR. */
+#define ORC_RADIX 11
 static int build_grouped(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                          const float* w, int symmetrize, int major_is_dst,
                          uint64_t* ptr, uint32_t* idx, float* w_out) {
   const uint64_t nnz = E * (symmetrize ? 2u : 1u);
-  uint64_t* cnt = (uint64_t*)calloc((size_t)V + 1, sizeof(uint64_t));
-  uint64_t* perm = (uint64_t*)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
-  uint64_t* perm2 = (uint64_t*)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
-  if (!cnt || !perm || !perm2) { free(cnt); free(perm); free(perm2); return -1; }
-  uint32_t s, d;
-  /* pass 1: by minor key */
+  uint32_t b = 1;
+  while (b < 32 && ((uint64_t)1 << b) < (uint64_t)V) ++b;
+  const uint64_t mask = ((uint64_t)1 << b) - 1;
+  const int carry_w = (w && w_out);
+  uint64_t* k0 = (uint64_t*)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+  uint64_t* k1 = (uint64_t*)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+  float* w0 = carry_w ? (float*)malloc(sizeof(float) * (nnz ? nnz : 1)) : NULL;
+  float* w1 = carry_w ? (float*)malloc(sizeof(float) * (nnz ? nnz : 1)) : NULL;
+  if (!k0 || !k1 || (carry_w && (!w0 || !w1))) { free(k0); free(k1); free(w0); free(w1); return -1; }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 65536)
+#endif
   for (uint64_t i = 0; i < nnz; ++i) {
+    uint32_t s, d;
     entry_at(E, src, dst, i, &s, &d);
-    cnt[(major_is_dst ? s : d) + 1]++;
+    const uint64_t major = major_is_dst ? d : s, minor = major_is_dst ? s : d;
+    k0[i] = (major << b) | minor;
+    if (carry_w) w0[i] = w[i < E ? i : i - E];
   }
-  for (uint32_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
-  for (uint64_t i = 0; i < nnz; ++i) {
-    entry_at(E, src, dst, i, &s, &d);
-    perm[cnt[major_is_dst ? s : d]++] = i;
+  const uint32_t bits = 2 * b, nb = 1u << ORC_RADIX;
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  uint64_t* cnt = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nth * nb);
+  if (!cnt) { free(k0); free(k1); free(w0); free(w1); return -1; }
+  for (uint32_t shift = 0; shift < bits; shift += ORC_RADIX) {
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth)
+#endif
+    {
+      int t = 0, T = 1;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+      T = omp_get_num_threads();
+#endif
+      const uint64_t lo = nnz * (uint64_t)t / (uint64_t)T, hi = nnz * (uint64_t)(t + 1) / (uint64_t)T;
+      uint64_t* c = cnt + (size_t)t * nb;
+      memset(c, 0, sizeof(uint64_t) * nb);
+      for (uint64_t i = lo; i < hi; ++i) c[(k0[i] >> shift) & (nb - 1)]++;
+#ifdef _OPENMP
+#pragma omp barrier
+#pragma omp single
+#endif
+      {
+        uint64_t run = 0;
+        for (uint32_t dgt = 0; dgt < nb; ++dgt)
+          for (int q = 0; q < T; ++q) {
+            const uint64_t x = cnt[(size_t)q * nb + dgt];
+            cnt[(size_t)q * nb + dgt] = run;
+            run += x;
+          }
+      }
+      for (uint64_t i = lo; i < hi; ++i) {
+        const uint64_t at = c[(k0[i] >> shift) & (nb - 1)]++;
+        k1[at] = k0[i];
+        if (carry_w) w1[at] = w0[i];
+      }
+    }
+    uint64_t* tk = k0; k0 = k1; k1 = tk;
+    float* tw = w0; w0 = w1; w1 = tw;
   }
-  /* pass 2: by major key, stable */
-  memset(cnt, 0, sizeof(uint64_t) * ((size_t)V + 1));
-  for (uint64_t i = 0; i < nnz; ++i) {
-    entry_at(E, src, dst, i, &s, &d);
-    cnt[(major_is_dst ? d : s) + 1]++;
+  free(cnt);
+  /* ptr[v] = first entry whose major is >= v */
+  if (nnz == 0) {
+    for (uint64_t v = 0; v <= V; ++v) ptr[v] = 0;
+  } else {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 65536)
+#endif
+    for (uint64_t k = 0; k < nnz; ++k) {
+      const uint64_t m = k0[k] >> b;
+      const uint64_t prev = k ? (k0[k - 1] >> b) + 1 : 0;
+      if (k == 0 || m != (k0[k - 1] >> b))
+        for (uint64_t v = prev; v <= m; ++v) ptr[v] = k;
+      idx[k] = (uint32_t)(k0[k] & mask);
+      if (carry_w) w_out[k] = w0[k];
+    }
+    for (uint64_t v = (k0[nnz - 1] >> b) + 1; v <= V; ++v) ptr[v] = nnz;
   }
-  for (uint32_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
-  memcpy(ptr, cnt, sizeof(uint64_t) * ((size_t)V + 1));
-  for (uint64_t k = 0; k < nnz; ++k) {
-    const uint64_t i = perm[k];
-    entry_at(E, src, dst, i, &s, &d);
-    perm2[cnt[major_is_dst ? d : s]++] = i;
-  }
-  for (uint64_t k = 0; k < nnz; ++k) {
-    const uint64_t i = perm2[k];
-    entry_at(E, src, dst, i, &s, &d);
-    idx[k] = major_is_dst ? s : d;
-    if (w && w_out) w_out[k] = w[i < E ? i : i - E];
-  }
-  free(cnt); free(perm); free(perm2);
+  free(k0); free(k1); free(w0); free(w1);
   return 0;
+}
+
+static int ids_in_range(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst) {
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(| : bad) schedule(static, 65536)
+#endif
+  for (uint64_t i = 0; i < E; ++i) bad |= (src[i] >= V || dst[i] >= V);
+  return !bad;
 }
 
 int orc_build_csc(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                   const float* w, int symmetrize,
                   uint64_t* col_ptr, uint32_t* row_idx, float* w_out) {
-  for (uint64_t i = 0; i < E; ++i)
-    if (src[i] >= V || dst[i] >= V) return -2;
+  if (!ids_in_range(V, E, src, dst)) return -2;
   return build_grouped(V, E, src, dst, w, symmetrize, 1, col_ptr, row_idx, w_out);
 }
 
 int orc_build_csr(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                   int symmetrize, uint64_t* row_ptr, uint32_t* col_idx) {
-  for (uint64_t i = 0; i < E; ++i)
-    if (src[i] >= V || dst[i] >= V) return -2;
+  if (!ids_in_range(V, E, src, dst)) return -2;
   return build_grouped(V, E, src, dst, NULL, symmetrize, 0, row_ptr, col_idx, NULL);
 }
 
 void orc_out_degrees(uint32_t V, uint64_t nnz, const uint32_t* row_idx, uint32_t* od) {
   memset(od, 0, sizeof(uint32_t) * V);
-  for (uint64_t e = 0; e < nnz; ++e) od[row_idx[e]]++;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 65536)
+#endif
+  for (uint64_t e = 0; e < nnz; ++e) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+    od[row_idx[e]]++;
+  }
 }
 
 /* ---- rank algorithms ------------------------------------------------------ */
@@ -226,28 +402,59 @@ int orc_weighted_pagerank(uint32_t V, const uint64_t* col_ptr, const uint32_t* r
     for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) W[row_idx[e]] += (double)w[e];
   for (uint32_t v = 0; v < V; ++v) x[v] = 1.0;  /* (104-109) */
   int executed = 0;
+  const uint32_t nchunks = (V + ORC_CHUNK - 1) / ORC_CHUNK;
+  double* part = (double*)calloc(nchunks ? nchunks : 1, sizeof(double));
   for (uint32_t t = 0; t < iters; ++t) {
     const int final_iter = (iters - 1 == t);
-    double delta = 0.0;
-    for (uint32_t v = 0; v < V; ++v) {
-      double sum = 0.0;
-      for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
-        const uint32_t u = row_idx[e];
-        const double beta = ((double)w[e] * 1.0) / W[u];  /* (119-120) */
-        sum += x[u] * beta;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (uint32_t c = 0; c < nchunks; ++c) {
+      const uint32_t lo = c * ORC_CHUNK, hi = (lo + ORC_CHUNK < V) ? lo + ORC_CHUNK : V;
+      double dl = 0.0;
+      for (uint32_t v = lo; v < hi; ++v) {
+        double sum = 0.0;
+        for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
+          const uint32_t u = row_idx[e];
+          const double beta = ((double)w[e] * 1.0) / W[u];  /* (119-120) */
+          sum += x[u] * beta;
+        }
+        const double r = 1.0 - damping + damping * sum;
+        y[v] = r;
+        dl += fabs(r - x[v]);  /* (138-143) */
       }
-      const double r = 1.0 - damping + damping * sum;
-      y[v] = r;
-      if (!final_iter) delta += fabs(r - x[v]);  /* (138-143) */
+      part[c] = dl;
     }
-    if (deltas) deltas[executed] = final_iter ? 0.0 : delta;
+    const double delta = final_iter ? 0.0 : chunked_sum(part, nchunks);
+    if (deltas) deltas[executed] = delta;
     ++executed;
     if (!final_iter && eps > 0.0 && delta < eps) t = iters - 2;
     double* tmp = x; x = y; y = tmp;
   }
   memcpy(out, x, sizeof(double) * V);
-  free(W); free(x); free(y);
+  free(part); free(W); free(x); free(y);
   return executed;
+}
+
+double orc_weighted_pagerank_step(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                                  const float* w, const double* W, double damping,
+                                  const double* x, double* y, uint32_t v_lo, uint32_t v_hi) {
+  (void)V;
+  double delta = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : delta)
+#endif
+  for (uint32_t v = v_lo; v < v_hi; ++v) {
+    double sum = 0.0;
+    for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) {
+      const uint32_t u = row_idx[e];
+      sum += x[u] * (((double)w[e] * 1.0) / W[u]);
+    }
+    const double r = 1.0 - damping + damping * sum;
+    y[v] = r;
+    delta += fabs(r - x[v]);
+  }
+  return delta;
 }
 
 /* seed membership as a byte map (a seed bitmap: duplicates count once) */
@@ -299,6 +506,10 @@ int orc_ppr(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
   int executed = 0;
   for (uint32_t t = 0; t < iters; ++t) {
     const int final_iter = (iters - 1 == t);
+    /* each column is summed by one thread in column order: thread-count independent */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096)
+#endif
     for (uint32_t v = 0; v < V; ++v) {
       double sum = 0.0;
       for (uint64_t e = col_ptr[v]; e < col_ptr[v + 1]; ++e) sum += x[row_idx[e]];

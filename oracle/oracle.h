@@ -36,22 +36,54 @@ void orc_rmat_thresholds(double a, double b, double c, uint32_t out[3]);
 void orc_rmat_edges(uint32_t scale, uint64_t seed, const uint32_t thr[3],
                     uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst);
 
+/* The edges with ids eids[0..n) (orc_rmat_edges of each id; OpenMP). */
+void orc_rmat_edges_at(uint32_t scale, uint64_t seed, const uint32_t thr[3], uint64_t n,
+                       const uint64_t* eids, uint32_t* src, uint32_t* dst);
+
 /* Edge weight in (0, 1] of edge e (24-bit uniform grid). */
 float orc_rmat_weight(uint64_t seed, uint64_t e);
+
+/* orc_rmat_weight of each edge id in eids[n] (OpenMP). */
+void orc_rmat_weights(uint64_t seed, uint64_t n, const uint64_t* eids, float* out);
+
+/* One pass over the generator's edges [0, E) at a scale too large to hold twice:
+ * od[2^scale] = out-degree of every source, W (optional) = FP64 sum of each source's
+ * edge weights (atomic adds: the last bits may depend on the thread schedule), *digest =
+ * the digest of the edge multiset (orc_edges_digest), and the ids of the edges whose
+ * destination lies in one of the ranges [lo[r], hi[r]) written to sel_eid (at most cap,
+ * in no particular order).  Returns the number of such edges.
+ * Pinned: equals orc_rmat_edges + bincount + orc_edges_digest (tests/test_oracle.py). */
+int64_t orc_rmat_scan(uint32_t scale, uint64_t seed, const uint32_t thr[3], uint64_t E,
+                      uint32_t nranges, const uint32_t* lo, const uint32_t* hi,
+                      uint32_t* od, double* W, uint64_t* digest,
+                      uint64_t cap, uint64_t* sel_eid);
+
+/* Order-independent digest of an edge multiset: the sum mod 2^64 of
+ * mix64(src << 32 | dst) over the edges.  A dropped, added or altered edge, or a
+ * swapped direction (src != dst), changes it except with probability about 2^-64. */
+uint64_t orc_edges_digest(uint64_t n, const uint32_t* src, const uint32_t* dst);
+
+/* The same digest over the entries of a CSC (entry e of column v is the edge
+ * row_idx[e] -> v), with both ends mapped through map[] when it is given (a
+ * relabeled graph's internal -> input ids). */
+uint64_t orc_csc_digest(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                        const uint32_t* map);
 
 /* ---- structures ----------------------------------------------------------- */
 
 /* In-edge CSC of an edge list: entries are the E input edges, followed (when
  * symmetrize != 0) by their E reversals, as Plato adds reverse entries for
  * undirected input (dcsc.hpp:379-384, bcsr.hpp:411-426).  Columns are sorted by
- * source id, stable in entry order.  col_ptr has V+1 entries; row_idx and
+ * source id, stable in entry order (a parallel LSD radix sort; the output does not
+ * depend on the thread count).  col_ptr has V+1 entries; row_idx and
  * w_out (optional) have nnz = E * (symmetrize ? 2 : 1) entries.
  * Pinned: reference DCSC fixtures (dcsc.cc:81-191) and message_passing.cc:48-88. */
 int orc_build_csc(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                   const float* w, int symmetrize,
                   uint64_t* col_ptr, uint32_t* row_idx, float* w_out);
 
-/* Out-edge CSR: the same entries grouped by source, sorted by destination. */
+/* Out-edge CSR: the same entries grouped by source, sorted by destination.
+ * Pinned with orc_build_csc: numpy's stable lexsort of the same entries. */
 int orc_build_csr(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                   int symmetrize, uint64_t* row_ptr, uint32_t* col_idx);
 
@@ -83,6 +115,14 @@ double orc_pagerank_step(uint32_t V, const uint64_t* col_ptr, const uint32_t* ro
 int orc_weighted_pagerank(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
                           const float* w, uint32_t iters, double damping, double eps,
                           double* out, double* deltas);
+
+/* One weighted PageRank iteration over columns [v_lo, v_hi) (weighted_pagerank.hpp:
+ * 110-143): y(v) = 1 - d + d * sum over in-edges u of x(u) * w / W(u), x the
+ * undivided rank state, W(u) the sum of u's out-weights.  Returns the range's delta.
+ * Pinned: equals one iteration of orc_weighted_pagerank (tests/test_oracle.py). */
+double orc_weighted_pagerank_step(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx,
+                                  const float* w, const double* W, double damping,
+                                  const double* x, double* y, uint32_t v_lo, uint32_t v_hi);
 
 /* Personalized PageRank with restart mass spread evenly over a seed set:
  * nseeds == 1 is plato::algo::personalized_pagerank, pull form

@@ -116,6 +116,49 @@ struct ktime_scope {
   ~ktime_scope();
 };
 
+// A captured stream segment and its executable.  Capture always ends: if anything
+// throws between begin() and end(), the destructor ends the capture and drops the
+// partial graph, so the stream is not left in capture mode.  An executable that has
+// been launched is destroyed only after its stream has drained, so no launch of it is
+// still in flight (or still being traced) when its memory goes.
+struct stream_graph {
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool capturing = false, launched = false;
+
+  void begin(hipStream_t st) {
+    reset();
+    s = st;
+    PE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+  }
+  void end() {
+    capturing = false;
+    PE_HIP(hipStreamEndCapture(s, &graph));
+    PE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  void launch() {
+    launched = true;
+    PE_HIP(hipGraphLaunch(exec, s));
+  }
+  void reset() {
+    if (capturing) {
+      hipGraph_t partial = nullptr;
+      if (hipStreamEndCapture(s, &partial) == hipSuccess && partial) (void)hipGraphDestroy(partial);
+      (void)hipGetLastError();
+      capturing = false;
+    }
+    if (launched) (void)hipStreamSynchronize(s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    launched = false;
+  }
+  ~stream_graph() { reset(); }
+};
+
 inline unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap = 1u << 30) {
   uint64_t g = (work + per_block - 1) / per_block;
   if (g == 0) g = 1;

@@ -370,14 +370,8 @@ struct pe_ppr_batch_s {
   unsigned task_grid = 0, fix_grid = 0;
   std::vector<std::vector<uint32_t>> slot_seeds;  // internal ids per slot
   std::vector<uint32_t> host_status;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
+  stream_graph sg;  // the captured iterations, replayed by step()
   uint32_t graph_iters = 0;
-
-  ~pe_ppr_batch_s() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-  }
 
   void build_tasks(uint32_t C) {
     pe_ctx_s* ctx = g->ctx;
@@ -463,18 +457,14 @@ struct pe_ppr_batch_s {
       for (uint32_t i = 0; i < n; ++i) launch_iteration();
       return;
     }
-    if (!exec || graph_iters != n) {
-      if (exec) (void)hipGraphExecDestroy(exec);
-      if (graph) (void)hipGraphDestroy(graph);
-      exec = nullptr;
-      graph = nullptr;
-      PE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    if (!sg.exec || graph_iters != n) {
+      graph_iters = 0;
+      sg.begin(s);  // waits for launches of the previous executable first
       for (uint32_t i = 0; i < n; ++i) launch_iteration();
-      PE_HIP(hipStreamEndCapture(s, &graph));
-      PE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      sg.end();
       graph_iters = n;
     }
-    PE_HIP(hipGraphLaunch(exec, s));
+    sg.launch();
   }
 };
 

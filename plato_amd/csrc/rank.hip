@@ -230,16 +230,14 @@ struct rank_run {
       } else {
         // capture a chunk of iterations once, replay it until the controller is done
         const uint32_t chunk = early ? std::min<uint32_t>(iters, 8) : std::min<uint32_t>(iters, 16);
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        PE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        stream_graph sg;  // destroyed after the stream drains, also on an exception
+        sg.begin(s);
         for (uint32_t i = 0; i < chunk; ++i)
           rank_iteration(g, prec, b0.p, b1.p, weighted, args, ctl.p, wk, deltas.p, rsums.p);
-        PE_HIP(hipStreamEndCapture(s, &graph));
-        PE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        sg.end();
         uint32_t launched = 0;
         while (launched < iters) {
-          PE_HIP(hipGraphLaunch(exec, s));
+          sg.launch();
           launched += chunk;
           if (early) {
             PE_HIP(hipMemcpyAsync(&host_ctl, ctl.p, sizeof(iter_ctl), hipMemcpyDeviceToHost, s));
@@ -247,8 +245,7 @@ struct rank_run {
             if (host_ctl.done) break;
           }
         }
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
+        PE_HIP(hipStreamSynchronize(s));
       }
     }
     PE_HIP(hipEventRecord(ctx->t1, s));

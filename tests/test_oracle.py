@@ -475,3 +475,149 @@ def test_hyperanf_reference_test_average_distance():
         real += sum(i * (friends[i] - friends[i - 1]) for i in range(1, mx + 1)) / (friends[-1] - friends[0])
     real /= V
     assert abs(real - avg) <= 0.05, (real, avg)
+
+
+# ---- parallel structure builder, edge digests, the one-pass scan ---------------------------
+
+def _lexsort_csc(V, s, d, w=None, symmetrize=False):
+    """Independent restatement of the CSC: numpy's stable lexsort of the entries by
+    (destination, source), reversals appended for undirected input."""
+    if symmetrize:
+        s, d = np.concatenate([s, d]), np.concatenate([d, s])
+        w = None if w is None else np.concatenate([w, w])
+    order = np.lexsort((s, d))
+    ptr = np.zeros(V + 1, np.uint64)
+    ptr[1:] = np.cumsum(np.bincount(d, minlength=V))
+    return ptr, s[order], (None if w is None else w[order])
+
+
+@pytest.mark.parametrize("symmetrize", [False, True])
+def test_csc_and_csr_equal_stable_lexsort(symmetrize):
+    s, d = oracle.rmat_edges(12, 8, 29)
+    w = oracle.rmat_weights(29, s.size)
+    g = oracle.CSC(1 << 12, s, d, w=w, symmetrize=symmetrize)
+    ptr, idx, ww = _lexsort_csc(1 << 12, s, d, w, symmetrize)
+    assert np.array_equal(g.col_ptr, ptr) and np.array_equal(g.row_idx, idx)
+    assert np.array_equal(g.w, ww)  # equal (dst, src) keep input order: weights pin stability
+    rp, ci, _ = _lexsort_csc(1 << 12, d, s, None, symmetrize)
+    assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, ci)
+    # V not a power of two, empty leading and trailing columns
+    g2 = oracle.CSC(5000, s + 3, d + 3)
+    p2, i2, _ = _lexsort_csc(5000, s + 3, d + 3)
+    assert np.array_equal(g2.col_ptr, p2) and np.array_equal(g2.row_idx, i2)
+
+
+def test_csc_independent_of_thread_count():
+    s, d = oracle.rmat_edges(13, 8, 31)
+    ref = oracle.CSC(1 << 13, s, d, symmetrize=True)
+    g = oracle.CSC(1 << 13, s, d)
+    x = np.ones(g.V)
+    for t in (1, 3):
+        oracle.pagerank_step(g, x, D, False, 0, 1, nthreads=t)  # sets the OpenMP team size
+        h = oracle.CSC(1 << 13, s, d, symmetrize=True)
+        assert np.array_equal(h.col_ptr, ref.col_ptr) and np.array_equal(h.row_idx, ref.row_idx)
+    oracle.pagerank_step(g, x, D, False, 0, 1, nthreads=0)
+
+
+def test_edges_digest_matches_numpy_twin_and_csc():
+    s, d = oracle.rmat_edges(11, 16, 37)
+    twin = int(np.sum(R.mix64((s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)),
+                      dtype=np.uint64))
+    assert oracle.edges_digest(s, d) == twin
+    g = oracle.CSC(1 << 11, s, d)
+    assert oracle.csc_digest(g.V, g.col_ptr, g.row_idx) == twin
+    # through a relabeling map: internal ids v' = rank_of[v], map = perm (internal -> input)
+    perm = np.random.default_rng(1).permutation(g.V).astype(np.uint32)
+    rank_of = np.argsort(perm).astype(np.uint32)
+    h = oracle.CSC(g.V, rank_of[s], rank_of[d])
+    assert oracle.csc_digest(h.V, h.col_ptr, h.row_idx, perm) == twin
+    # a dropped edge, a moved endpoint and a reversed edge each change it
+    assert oracle.edges_digest(s[1:], d[1:]) != twin
+    d2 = d.copy()
+    d2[5] ^= 1
+    assert oracle.edges_digest(s, d2) != twin
+    k = int(np.flatnonzero(s != d)[0])
+    s3, d3 = s.copy(), d.copy()
+    s3[k], d3[k] = d[k], s[k]
+    assert oracle.edges_digest(s3, d3) != twin
+
+
+def test_rmat_scan_matches_full_generation():
+    scale, ef, seed = 12, 16, 41
+    ranges = [(0, 40), (1000, 1100), (4000, 4096)]
+    sc = oracle.RmatScan(scale, ef, seed, ranges, weighted=True)
+    s, d = oracle.rmat_edges(scale, ef, seed)
+    w = oracle.rmat_weights(seed, s.size)
+    assert np.array_equal(sc.od, np.bincount(s, minlength=1 << scale).astype(np.uint32))
+    np.testing.assert_allclose(sc.W, np.bincount(s, weights=w.astype(np.float64), minlength=1 << scale),
+                               rtol=1e-13)
+    assert sc.digest == oracle.edges_digest(s, d)
+    keep = np.zeros(s.size, bool)
+    for lo, hi in ranges:
+        keep |= (d >= lo) & (d < hi)
+    assert np.array_equal(sc.eids, np.flatnonzero(keep).astype(np.uint64))
+    ptr, idx, ww = _lexsort_csc(1 << scale, s[keep], d[keep], w[keep])
+    assert np.array_equal(sc.slice.col_ptr, ptr) and np.array_equal(sc.slice.row_idx, idx)
+    assert np.array_equal(sc.slice.w, ww)
+
+
+def test_weighted_pagerank_step_equals_one_iteration():
+    s, d = oracle.rmat_edges(10, 16, 43)
+    w = oracle.rmat_weights(43, s.size)
+    g = oracle.CSC(1 << 10, s, d, w=w)
+    r1, _ = oracle.weighted_pagerank(g, 1, D, 0.0)
+    W = np.bincount(g.row_idx, weights=g.w.astype(np.float64), minlength=g.V)
+    y = oracle.weighted_pagerank_step(g, W, np.ones(g.V), D)
+    np.testing.assert_allclose(y, r1, rtol=1e-14)
+    # two iterations: the step applied to the first iteration's state
+    r2, _ = oracle.weighted_pagerank(g, 2, D, 0.0)
+    np.testing.assert_allclose(oracle.weighted_pagerank_step(g, W, r1, D), r2, rtol=1e-14)
+
+
+# ---- the push/pull PPR switch, pinned by hop balls ---------------------------------------------
+
+def _expected_ppr_modes(g, src, iters):
+    """personalized_pagerank.hpp:188-205: push while the accumulated active set (every
+    vertex within t out-hops of src at epoch t) has active edges <= E'/20 (integer
+    division), then pull for good.  Hop distances from scipy, independent of the oracle."""
+    V = g.V
+    cols = np.repeat(np.arange(V), np.diff(g.col_ptr).astype(np.int64))
+    A = sp.csr_matrix((np.ones(g.nnz), (g.row_idx, cols)), shape=(V, V))  # u -> v
+    dist = csg.shortest_path(A, directed=True, unweighted=True, indices=src)
+    modes, pull = [], False
+    for t in range(iters):
+        if not pull:
+            active_edges = int(g.od[dist <= t].sum())
+            pull = active_edges > g.edges_prime // 20
+        modes.append(0 if pull else 1)
+    return np.array(modes, np.uint8)
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_ppr_pushpull_switch_follows_hop_balls(directed):
+    s, d = oracle.rmat_edges(11, 16, 53)
+    g = oracle.CSC(1 << 11, s, d, symmetrize=not directed)
+    cand = np.flatnonzero(g.od > 0)
+    order = cand[np.argsort(g.od[cand], kind="stable")]
+    srcs = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    seen = set()
+    for src in srcs:
+        _, _, modes = oracle.ppr_pushpull(g, src, 12, D, 0.0)
+        exp = _expected_ppr_modes(g, src, 12)
+        assert np.array_equal(modes, exp), (src, modes, exp)
+        seen.add(int(np.argmin(exp)) if (exp == 0).any() else 99)
+    assert len(seen) >= 2  # the sources switch at different epochs
+
+
+def test_ppr_pushpull_switch_boundary():
+    """A_t == E'/20 stays in push; the first A_t above it pulls (strict '>')."""
+    # 40 edges: a source 0 with two out-edges to 1 and 2, vertex 1 with 3 out-edges,
+    # the rest a chain among vertices 10..45 that the source never reaches
+    s = [0, 0, 1, 1, 1] + list(range(10, 45))
+    d = [1, 2, 3, 4, 5] + list(range(11, 46))
+    g = oracle.CSC(46, np.array(s, np.uint32), np.array(d, np.uint32))
+    assert g.edges_prime // 20 == 2
+    _, _, modes = oracle.ppr_pushpull(g, 0, 4, D, 0.0)
+    # epoch 0: A = od(0) = 2, not above 2 -> push; epoch 1: A = 2 + 3 + 0 = 5 -> pull
+    assert modes.tolist() == [1, 0, 0, 0]
+    assert np.array_equal(modes, _expected_ppr_modes(g, 0, 4))
