@@ -376,6 +376,7 @@ def run_batched_ppr(ctx, scale, tol=1e-7, max_iters=200, seed=24, single_runs=Tr
     return {"scale": scale, "k": 64, "tol": tol, "batched_qps": 64 / t_batch,
             "batched_seconds": t_batch, "passes": passes,
             "single_seed_qps": 64 / best, "speedup_vs_single": best / t_batch,
+            "single_seed_engine": min(single, key=single.get),
             "single_seed_qps_by_layout": {k: 64 / v for k, v in single.items()},
             "speedup_vs_single_by_layout": {k: v / t_batch for k, v in single.items()},
             "pushpull_single_qps": 8 / t_pp,

@@ -24,7 +24,9 @@ int main(int argc, char** argv) {
   auto graph = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, input);
   using dcsc_spec_t = plato::dcsc_t<plato::empty_t>;
   using bcsr_spec_t = plato::bcsr_t<plato::empty_t>;
-  plato::dualmode_engine_t<dcsc_spec_t, bcsr_spec_t> engine(graph.second, graph.first, graph_info);
+  plato::dualmode_engine_t<dcsc_spec_t, bcsr_spec_t> engine(
+      std::shared_ptr<dcsc_spec_t>(&graph.second, [](dcsc_spec_t*) {}),
+      std::shared_ptr<bcsr_spec_t>(&graph.first, [](bcsr_spec_t*) {}), graph_info);
   plato::algo::connected_component_t<dcsc_spec_t, bcsr_spec_t> cc(&engine, graph_info);
   auto t1 = std::chrono::steady_clock::now();
   cc.compute();

@@ -25,8 +25,8 @@ int main(int argc, char** argv) {
   opts.bits = (uint32_t)std::atoi(flag(argc, argv, "bits", "6"));
   auto t0 = std::chrono::steady_clock::now();
   auto graph = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, input);
-  auto& in = *graph.second;
-  auto& out = *graph.first;
+  auto& in = graph.second;
+  auto& out = graph.first;
   auto t1 = std::chrono::steady_clock::now();
   double avg = 0.0;
   switch (opts.bits) {

@@ -21,8 +21,8 @@ int main(int argc, char** argv) {
   opts.precision_ = std::atoi(flag(argc, argv, "precision", "32"));
   const std::string type = flag(argc, argv, "type", "pull");
   auto ranks = type == "pushpull"
-                   ? plato::algo::personalized_pagerank(*graph.second, *graph.first, graph_info, opts)
-                   : plato::algo::personalized_pagerank(*graph.second, graph_info, opts);
+                   ? plato::algo::personalized_pagerank(graph.second, graph.first, graph_info, opts)
+                   : plato::algo::personalized_pagerank(graph.second, graph_info, opts);
   const std::string output = flag(argc, argv, "output", "");
   if (!output.empty()) {
     const std::string path = output + "/0000_0.csv";

@@ -48,7 +48,7 @@ int main(int argc, char** argv) {
   opts.select_method_ = method;
   opts.random_seed_ = std::strtoull(flag(argc, argv, "random_seed", "0"), nullptr, 10);
   auto t1 = std::chrono::steady_clock::now();
-  auto ranks = plato::algo::trustrank(*graph.second, *graph.first, graph_info, good_vertices, opts);
+  auto ranks = plato::algo::trustrank(graph.second, graph.first, graph_info, good_vertices, opts);
   auto t2 = std::chrono::steady_clock::now();
   std::printf("vertices %u edges %llu good %zu load %.3fs trustrank %.3fs\n", graph_info.vertices_,
               (unsigned long long)graph_info.edges_, good_vertices.count(),

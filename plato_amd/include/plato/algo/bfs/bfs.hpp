@@ -19,15 +19,13 @@ struct bfs_opts_t {
   vid_t root_ = 0;
 };
 
-// Levels of the last search (root 0, unreached -1), input id order.
-inline std::vector<int32_t>& last_bfs_levels() {
-  static std::vector<int32_t> levels;
-  return levels;
-}
-
+// Returns |visited| as the reference does.  When `levels` is given it receives this
+// search's levels (root 0, unreached -1) in input id order; every call owns its own
+// device state, so searches on different contexts or threads do not share anything.
 template <typename INCOMING, typename OUTGOING>
 vid_t breadth_first_search(INCOMING& in_edges, OUTGOING& out_edges, const graph_info_t& graph_info,
-                           const bfs_opts_t& opts, bool log_epochs = false) {
+                           const bfs_opts_t& opts, std::vector<int32_t>* levels = nullptr,
+                           bool log_epochs = false) {
   dualmode_engine_t<INCOMING, OUTGOING> engine(std::shared_ptr<INCOMING>(&in_edges, [](INCOMING*) {}),
                                                std::shared_ptr<OUTGOING>(&out_edges, [](OUTGOING*) {}),
                                                graph_info);
@@ -49,9 +47,10 @@ vid_t breadth_first_search(INCOMING& in_edges, OUTGOING& out_edges, const graph_
                   engine.last_was_push() ? "push" : "pull", (unsigned long long)engine.last_active_edges());
     std::swap(active_current, active_next);
   }
-  auto& levels = last_bfs_levels();
-  levels.assign(in_edges.info().V, -1);
-  check_status(pe_bfs_get_levels(state.h, levels.data()), "pe_bfs_get_levels");
+  if (levels) {
+    levels->assign(in_edges.info().V, -1);
+    check_status(pe_bfs_get_levels(state.h, levels->data()), "pe_bfs_get_levels");
+  }
   pe_bfs_destroy(state.h);
   return visited;
 }

@@ -23,7 +23,7 @@ struct pagerank_opts_t {
 
 namespace detail {
 template <typename GRAPH>
-host_state_t<double, typename GRAPH::partition_t> run_pagerank(GRAPH& graph, const pagerank_opts_t& opts,
+dense_state_t<double, typename GRAPH::partition_t> run_pagerank(GRAPH& graph, const pagerank_opts_t& opts,
                                                                bool weighted) {
   pe_rank_opts_t o{};
   o.iterations = opts.iteration_;
@@ -34,18 +34,18 @@ host_state_t<double, typename GRAPH::partition_t> run_pagerank(GRAPH& graph, con
   std::vector<double> out(graph.info().V);
   pe_rank_stats_t st{};
   check_status(pe_pagerank(graph.handle(), &o, out.data(), nullptr, &st), "pe_pagerank");
-  return host_state_t<double, typename GRAPH::partition_t>(std::move(out));
+  return dense_state_t<double, typename GRAPH::partition_t>::from_host(std::move(out));
 }
 }  // namespace detail
 
 template <typename GRAPH>
-host_state_t<double, typename GRAPH::partition_t> pagerank(GRAPH& graph, const graph_info_t& /*graph_info*/,
+dense_state_t<double, typename GRAPH::partition_t> pagerank(GRAPH& graph, const graph_info_t& /*graph_info*/,
                                                            const pagerank_opts_t& opts = pagerank_opts_t()) {
   return detail::run_pagerank(graph, opts, false);
 }
 
 template <typename GRAPH>
-host_state_t<double, typename GRAPH::partition_t> weighted_pagerank(GRAPH& graph, const graph_info_t& /*graph_info*/,
+dense_state_t<double, typename GRAPH::partition_t> weighted_pagerank(GRAPH& graph, const graph_info_t& /*graph_info*/,
                                                                     const pagerank_opts_t& opts = pagerank_opts_t()) {
   return detail::run_pagerank(graph, opts, true);
 }

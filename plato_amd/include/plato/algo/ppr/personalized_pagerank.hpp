@@ -19,7 +19,7 @@ struct personalized_pagerank_opts_t {
 };
 
 template <typename GRAPH>
-host_state_t<double, typename GRAPH::partition_t> personalized_pagerank(
+dense_state_t<double, typename GRAPH::partition_t> personalized_pagerank(
     GRAPH& graph, const graph_info_t& /*graph_info*/,
     const personalized_pagerank_opts_t& opts = personalized_pagerank_opts_t()) {
   pe_rank_opts_t o{};
@@ -30,11 +30,11 @@ host_state_t<double, typename GRAPH::partition_t> personalized_pagerank(
   std::vector<double> out(graph.info().V);
   const uint32_t src = opts.src_;
   check_status(pe_ppr(graph.handle(), &src, 1, &o, out.data(), nullptr, nullptr), "pe_ppr");
-  return host_state_t<double, typename GRAPH::partition_t>(std::move(out));
+  return dense_state_t<double, typename GRAPH::partition_t>::from_host(std::move(out));
 }
 
 template <typename INCOMING, typename OUTGOING>
-host_state_t<double, typename INCOMING::partition_t> personalized_pagerank(
+dense_state_t<double, typename INCOMING::partition_t> personalized_pagerank(
     INCOMING& in_edges, OUTGOING& /*out_edges*/, const graph_info_t& /*graph_info*/,
     const personalized_pagerank_opts_t& opts = personalized_pagerank_opts_t()) {
   pe_rank_opts_t o{};
@@ -45,7 +45,7 @@ host_state_t<double, typename INCOMING::partition_t> personalized_pagerank(
   std::vector<double> out(in_edges.info().V);
   check_status(pe_ppr_pushpull(in_edges.handle(), opts.src_, &o, out.data(), nullptr, nullptr, nullptr),
                "pe_ppr_pushpull");
-  return host_state_t<double, typename INCOMING::partition_t>(std::move(out));
+  return dense_state_t<double, typename INCOMING::partition_t>::from_host(std::move(out));
 }
 
 }  // namespace algo

@@ -91,7 +91,7 @@ bitmap_t<> select_good_seeds(INCOMING& in_edges, OUTGOING& /*out_edges*/, const 
 
 // trustrank (trustrank.hpp:163-259): each vertex's rank, dense
 template <typename INCOMING, typename OUTGOING>
-host_state_t<double, typename INCOMING::partition_t> trustrank(INCOMING& in_edges, OUTGOING& /*out_edges*/,
+dense_state_t<double, typename INCOMING::partition_t> trustrank(INCOMING& in_edges, OUTGOING& /*out_edges*/,
                                                              const graph_info_t& graph_info,
                                                              const bitmap_t<>& good_vertices,
                                                              const trustrank_opts_t& opts = trustrank_opts_t()) {
@@ -102,7 +102,7 @@ host_state_t<double, typename INCOMING::partition_t> trustrank(INCOMING& in_edge
   std::vector<double> out(V);
   check_status(pe_trustrank(in_edges.handle(), rev.h, good.data(), &o, out.data(), nullptr, nullptr),
                "pe_trustrank");
-  return host_state_t<double, typename INCOMING::partition_t>(std::move(out));
+  return dense_state_t<double, typename INCOMING::partition_t>::from_host(std::move(out));
 }
 
 }  // namespace algo

@@ -5,3 +5,5 @@
 #include "plato/graph/state/dense_state.hpp"
 #include "plato/graph/structure.hpp"
 #include "plato/util/bitmap.hpp"
+#include "plato/util/foutput.h"
+#include "plato/util/perf.hpp"

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "plato/graph/base.hpp"
+#include "plato/util/bitmap.hpp"
 
 namespace plato {
 
@@ -21,6 +22,7 @@ template <typename T, typename PART_IMPL>
 class dense_state_t {
  public:
   using value_t = T;
+  using partition_t = PART_IMPL;
 
   dense_state_t() = default;
   // (max_v_i, partitioner) as the reference's constructor (dense_state.hpp:133-137)
@@ -49,10 +51,35 @@ class dense_state_t {
     check_status(pe_vec_download(h_, host_.data(), n_), "pe_vec_download");
     return host_;
   }
+  // a new state holding the given host values (one per vertex)
+  static dense_state_t from_host(std::vector<T> v) {
+    dense_state_t st((uint64_t)v.size());
+    st.upload(v);
+    st.host_ = std::move(v);
+    return st;
+  }
   void upload(const std::vector<T>& v) { check_status(pe_vec_upload(h_, v.data(), v.size()), "pe_vec_upload"); }
   T operator[](vid_t i) {
     if (host_.size() != n_) to_host();
     return host_[i];
+  }
+  // foreach (dense_state.hpp:119,422): process(v, T* value) for every vertex in
+  // [0, size) -- or only those set in *pactives -- on the host; the sum of the returns
+  // comes back, and values written through the pointer are uploaded to the device.
+  template <typename R, typename PROCESS>
+  R foreach(PROCESS&& process, bitmap_t<>* pactives = nullptr, size_t /*chunk_size*/ = 0) {
+    to_host();
+    std::vector<uint64_t> words;
+    if (pactives) {
+      words.resize((pactives->size_ >> 6) + 1);
+      check_status(pe_bitmap_download(pactives->handle(), words.data(), words.size()), "pe_bitmap_download");
+    }
+    R r = R();
+    for (uint64_t v = 0; v < n_; ++v)
+      if (!pactives || (v < pactives->size_ && ((words[v >> 6] >> (v & 63)) & 1u)))
+        r += process((vid_t)v, &host_[v]);
+    upload(host_);
+    return r;
   }
   double sum() const {
     double s = 0;

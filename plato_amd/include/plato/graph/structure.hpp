@@ -125,12 +125,25 @@ inline std::shared_ptr<device_graph> build(graph_info_t* info, const std::string
 }
 }  // namespace detail
 
-template <typename EDATA, typename DECODER = void*>
+// The reference's loaders stage edges in host caches (edge_block_cache_t,
+// edge_file_cache_t, graph/structure/edge_cache.hpp); here the device parses the file,
+// so these types only keep the template arguments of the reference's call sites.
+template <typename EDATA, typename VID_T = vid_t>
+struct edge_block_cache_t {};
+template <typename EDATA, typename VID_T = vid_t>
+struct edge_file_cache_t {};
+
+// create_dcsc_seqs_from_path (structure.hpp:544-555): same template and argument lists;
+// alpha and use_in_degree shape the reference's MPI partitions and have no meaning on
+// one device; a vid encoder is not supported (ids are used as given).
+template <typename EDATA, typename VID_T = vid_t, template <typename, typename> class CACHE = edge_block_cache_t,
+          typename DECODER = void*, typename ENCODER = std::nullptr_t>
 std::shared_ptr<dcsc_t<EDATA>> create_dcsc_seqs_from_path(graph_info_t* info, const std::string& path,
                                                           edge_format_t = edge_format_t::CSV,
-                                                          DECODER = nullptr, bool relabel = false) {
+                                                          DECODER = nullptr, int /*alpha*/ = -1,
+                                                          bool /*use_in_degree*/ = false, ENCODER = nullptr) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
-  auto g = detail::build(info, path, weighted, false, relabel);
+  auto g = detail::build(info, path, weighted, false, false);
   return std::make_shared<dcsc_t<EDATA>>(g);
 }
 
@@ -147,7 +160,8 @@ std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqd_from_path(graph_info_t* info, co
 
 // create_bcsr_seqs_from_path (structure.hpp:727-739): is_outgoing = false keeps the
 // incoming edges in the structure (as example/kcore_simple.cc asks for).
-template <typename EDATA, typename DECODER = void*, typename ENCODER = std::nullptr_t>
+template <typename EDATA, typename VID_T = vid_t, template <typename, typename> class CACHE = edge_block_cache_t,
+          typename DECODER = void*, typename ENCODER = std::nullptr_t>
 std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqs_from_path(graph_info_t* info, const std::string& path,
                                                           edge_format_t = edge_format_t::CSV,
                                                           DECODER = nullptr, int /*alpha*/ = -1,
@@ -160,15 +174,17 @@ std::shared_ptr<bcsr_t<EDATA>> create_bcsr_seqs_from_path(graph_info_t* info, co
   return b;
 }
 
-template <typename EDATA, typename DECODER = void*>
-std::pair<std::shared_ptr<bcsr_t<EDATA>>, std::shared_ptr<dcsc_t<EDATA>>> create_dualmode_seq_from_path(
+// create_dualmode_seq_from_path (structure.hpp:978-1086): same template and argument lists.
+template <typename EDATA, typename VID_T = vid_t, template <typename, typename> class CACHE = edge_block_cache_t,
+          typename DECODER = void*, typename ENCODER = std::nullptr_t>
+std::pair<bcsr_t<EDATA>, dcsc_t<EDATA>> create_dualmode_seq_from_path(
     graph_info_t* info, const std::string& path, edge_format_t = edge_format_t::CSV, DECODER = nullptr,
-    bool relabel = false) {
+    int /*alpha*/ = -1, bool /*use_in_degree*/ = false, ENCODER = nullptr) {
   const bool weighted = !std::is_same<EDATA, empty_t>::value;
-  auto g = detail::build(info, path, weighted, true, relabel);
-  auto out = std::make_shared<bcsr_t<EDATA>>(g);
-  out->set_outgoing(true);
-  return {out, std::make_shared<dcsc_t<EDATA>>(g)};
+  auto g = detail::build(info, path, weighted, true, false);
+  bcsr_t<EDATA> out(g);
+  out.set_outgoing(true);
+  return {out, dcsc_t<EDATA>(g)};
 }
 
 // out-degrees counted from the in-edges (structure.hpp:292-346)

@@ -11,6 +11,9 @@
 #include "plato/algo/ppr/personalized_pagerank.hpp"
 #include "plato/algo/trustrank/trustrank.hpp"
 #include "plato/algo/cgm/connected_component.hpp"
+#include "plato/util/foutput.h"
+
+#include <zlib.h>
 
 static int failures = 0;
 #define EXPECT(cond)                                                   \
@@ -103,8 +106,8 @@ static void PPR_ChainClosedForm() {
   opts.eps_ = 0.0;
   opts.src_ = 0;
   opts.precision_ = 64;
-  auto pull = plato::algo::personalized_pagerank(*g.second, graph_info, opts);
-  auto pp = plato::algo::personalized_pagerank(*g.second, *g.first, graph_info, opts);
+  auto pull = plato::algo::personalized_pagerank(g.second, graph_info, opts);
+  auto pp = plato::algo::personalized_pagerank(g.second, g.first, graph_info, opts);
   double expect = 0.15;
   for (plato::vid_t v = 0; v < 10; ++v) {
     EXPECT(std::fabs(pull[v] - expect) <= 1e-14 * expect);
@@ -119,16 +122,16 @@ static void BFS_BinaryTree() {
   auto g = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/complete_binary_tree_10_9.csv");
   plato::algo::bfs_opts_t opts;
   opts.root_ = 0;
-  const plato::vid_t visited = plato::algo::breadth_first_search(*g.second, *g.first, graph_info, opts);
+  std::vector<int32_t> lv;
+  const plato::vid_t visited = plato::algo::breadth_first_search(g.second, g.first, graph_info, opts, &lv);
   EXPECT(visited == 10);
-  const auto& lv = plato::algo::last_bfs_levels();
   for (int v = 0; v < 10; ++v) EXPECT(lv[v] == (int)std::floor(std::log2(v + 1)));
   plato::graph_info_t undirected(false);
   auto u = plato::create_dualmode_seq_from_path<plato::empty_t>(&undirected, dir + "/v100_e2150_ua_c3.csv");
   // the graph has 3 connected components; vertex 0's holds 20 of the 100 vertices
   // (scipy.sparse.csgraph.connected_components on the symmetrized edges)
   opts.root_ = 0;
-  EXPECT(plato::algo::breadth_first_search(*u.second, *u.first, undirected, opts) == 20);
+  EXPECT(plato::algo::breadth_first_search(u.second, u.first, undirected, opts) == 20);
 }
 
 // TrustRank on the chain 0->1->...->9 (trustrank.hpp:80-259).  PageRank grows along
@@ -143,11 +146,11 @@ static void TrustRank_Chain() {
   plato::algo::trustrank_opts_t opts;
   opts.seed_num_ = 1;
   opts.select_method_ = 1;
-  auto s1 = plato::algo::select_good_seeds(*g.second, *g.first, graph_info, all, opts);
+  auto s1 = plato::algo::select_good_seeds(g.second, g.first, graph_info, all, opts);
   auto v1 = s1.to_vector();
   EXPECT(v1.size() == 1 && v1[0] == 9);
   opts.select_method_ = 0;
-  auto s0 = plato::algo::select_good_seeds(*g.second, *g.first, graph_info, all, opts);
+  auto s0 = plato::algo::select_good_seeds(g.second, g.first, graph_info, all, opts);
   auto v0 = s0.to_vector();
   EXPECT(v0.size() == 1 && v0[0] == 0);
   plato::bitmap_t<> good(graph_info.max_v_i_ + 1);
@@ -155,13 +158,13 @@ static void TrustRank_Chain() {
   opts.seed_num_ = 100;
   opts.iteration_ = 20;
   opts.eps_ = 0.0;
-  auto r = plato::algo::trustrank(*g.second, *g.first, graph_info, good, opts);
+  auto r = plato::algo::trustrank(g.second, g.first, graph_info, good, opts);
   double expect = 0.15;
   for (plato::vid_t v = 0; v < 10; ++v) {
     EXPECT(std::fabs(r[v] - expect) <= 1e-14 * expect);
     expect *= 0.85;
   }
-  auto pairs = plato::algo::generate_rank_pairs(*g.second, graph_info);
+  auto pairs = plato::algo::generate_rank_pairs(g.second, graph_info);
   EXPECT(pairs.size() == 10 && pairs[9].v_i == 9 && pairs[9].rank > pairs[0].rank);
 }
 
@@ -173,7 +176,9 @@ static void CGM_CalcComponent() {
   auto graph = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/v100_e2150_ua_c3.csv");
   using dcsc_spec_t = plato::dcsc_t<plato::empty_t>;
   using bcsr_spec_t = plato::bcsr_t<plato::empty_t>;
-  plato::dualmode_engine_t<dcsc_spec_t, bcsr_spec_t> engine(graph.second, graph.first, graph_info);
+  plato::dualmode_engine_t<dcsc_spec_t, bcsr_spec_t> engine(
+      std::shared_ptr<dcsc_spec_t>(&graph.second, [](dcsc_spec_t*) {}),
+      std::shared_ptr<bcsr_spec_t>(&graph.first, [](bcsr_spec_t*) {}), graph_info);
   plato::algo::connected_component_t<dcsc_spec_t, bcsr_spec_t> cc(&engine, graph_info);
   cc.compute();
   // the test's reference labels, by BFS over the file's edges
@@ -208,6 +213,76 @@ static void CGM_CalcComponent() {
   EXPECT(cc.get_major_vertices() == cc.get_components_info()->at(cc.get_major_label()).vertices_);
 }
 
+// The plato:: calls of the reference driver example/pagerank_simple.cc:67-106 in the
+// form that file writes them: the three-argument factory template with six arguments,
+// pagerank<graph_spec_t> returning a dense state, its mutable foreach, and
+// thread_local_fs_output with a gzip stream (the driver's gflags, glog and
+// boost::format are its own dependencies and are replaced by constants here).
+static void Reference_PagerankSimpleCallSites() {
+  auto& cluster_info = plato::cluster_info_t::get_instance();
+  plato::graph_info_t graph_info(true);
+  auto pdcsc = plato::create_dcsc_seqs_from_path<plato::empty_t, plato::vid_t, plato::edge_file_cache_t>(
+    &graph_info, dir + "/graph_10_9.csv", plato::edge_format_t::CSV,
+    plato::dummy_decoder<plato::empty_t>, -1, false
+  );
+  using graph_spec_t = std::remove_reference<decltype(*pdcsc)>::type;
+  plato::algo::pagerank_opts_t opts;
+  opts.iteration_ = 20;
+  opts.damping_   = 0.85;
+  opts.eps_       = 0.0;
+  auto ranks = plato::algo::pagerank<graph_spec_t>(*pdcsc, graph_info, opts);
+  char tmpl[] = "/tmp/plato_api_XXXXXX";
+  const std::string out = mkdtemp(tmpl);
+  {
+    plato::thread_local_fs_output os(out, "0000_", true);
+    ranks.foreach<int> (
+      [&](plato::vid_t v_i, double* pval) {
+        auto& fs_output = os.local();
+        fs_output << v_i << "," << *pval << "\n";
+        return 0;
+      }
+    );
+  }
+  // the file holds the chain's closed form r_v = 0.15 * sum_{i<=v} 0.85^i
+  gzFile f = gzopen((out + "/0000_0.csv.gz").c_str(), "rb");
+  EXPECT(f != nullptr);
+  if (f) {
+    char line[128];
+    int n = 0;
+    while (gzgets(f, line, sizeof(line))) {
+      unsigned v = 0;
+      double r = 0;
+      EXPECT(std::sscanf(line, "%u,%lf", &v, &r) == 2);
+      double want = 0, p = 1;
+      for (unsigned i = 0; i <= v; ++i, p *= 0.85) want += 0.15 * p;
+      EXPECT(v == (unsigned)n && std::fabs(r - want) <= 1e-5 * want);
+      ++n;
+    }
+    gzclose(f);
+    EXPECT(n == 10);
+  }
+  std::remove((out + "/0000_0.csv.gz").c_str());
+  std::remove(out.c_str());
+  // writes through the pointer reach the device state
+  const double before = ranks.sum();
+  const int touched = ranks.foreach<int>([](plato::vid_t, double* pval) { *pval *= 2.0; return 1; });
+  EXPECT(touched == 10);
+  EXPECT(std::fabs(ranks.sum() - 2.0 * before) <= 1e-12 * before);
+  // the dual-mode factory and engine as example/nstepdegrees.cc:146-163 write them
+  auto graph = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/graph_10_9.csv",
+      plato::edge_format_t::CSV, plato::dummy_decoder<plato::empty_t>, -1, true);
+  using bcsr_spec_t = plato::bcsr_t<plato::empty_t>;
+  using dcsc_spec_t = plato::dcsc_t<plato::empty_t>;
+  plato::dualmode_engine_t<dcsc_spec_t, bcsr_spec_t> engine (
+    std::shared_ptr<dcsc_spec_t>(&graph.second,  [](dcsc_spec_t*) { }),
+    std::shared_ptr<bcsr_spec_t>(&graph.first, [](bcsr_spec_t*) { }),
+    graph_info);
+  auto actives = engine.alloc_v_subset();
+  actives.fill();
+  EXPECT(actives.count() == 10);
+  (void)cluster_info;
+}
+
 int main(int argc, char** argv) {
   dir = argc > 1 ? argv[1] : "tests/golden";
   struct { const char* name; void (*fn)(); } cases[] = {
@@ -219,6 +294,7 @@ int main(int argc, char** argv) {
       {"BFS.BinaryTree", BFS_BinaryTree},
       {"TrustRank.Chain", TrustRank_Chain},
       {"CGM.CalcComponent", CGM_CalcComponent},
+      {"Reference.PagerankSimpleCallSites", Reference_PagerankSimpleCallSites},
   };
   for (auto& c : cases) {
     const int before = failures;
