@@ -223,6 +223,20 @@ int pe_bitmap_to_vector(pe_bitmap_t* b, uint32_t* ids, uint64_t cap, uint64_t* c
 /* ===========================================================================
  * Engine operators (plato/engine/dualmode.hpp, plato/graph/message_passing.hpp)
  * ========================================================================= */
+/* spread_message (plato/graph/message_passing.hpp:152-249): every vertex v set in
+ * `actives` sends one message to target t = key[v] (key: optional PE_U32 vector
+ * covering the bitmap; without it t = v), delivered to a sink from a closed set:
+ *   PE_SPREAD_COUNT  sink[t] += 1 (PE_U32 sink, no value vector)
+ *   PE_SPREAD_ADD    sink[t] += value[v] (PE_F32 or PE_F64 sink, value of the same
+ *                    dtype; without a value vector each message adds 1)
+ * Every delivered message returns 1, so *result is the number of messages (the
+ * reference's sum of sink returns).  A target outside the sink fails with
+ * PE_ERR_RANGE after the other messages are delivered.  FP atomics make the order of
+ * additions into one target unspecified. */
+#define PE_SPREAD_COUNT 0
+#define PE_SPREAD_ADD 1
+int pe_spread_message(pe_bitmap_t* actives, int op, pe_vec_t* key, pe_vec_t* value, pe_vec_t* sink,
+                      uint64_t* result);
 /* out-degrees counted from the in-edges: generate_dense_out_degrees_fg
  * (plato/graph/structure.hpp:292-346) */
 int pe_graph_out_degrees(pe_graph_t* g, uint32_t* od_host);

@@ -6,6 +6,7 @@ This Python package is a thin ctypes binding used by the tests and the bench;
 it has no CPU compute path.
 """
 from . import _lib  # noqa: F401
-from .engine import Bitmap, Context, DenseState, Graph, dualmode_engine_t, graph_info_t  # noqa: F401
+from .engine import (Bitmap, Context, DenseState, Graph, dualmode_engine_t, graph_info_t,  # noqa: F401
+                     spread_message)
 
-__all__ = ["Context", "Graph", "DenseState", "Bitmap", "dualmode_engine_t", "graph_info_t"]
+__all__ = ["Context", "Graph", "DenseState", "Bitmap", "dualmode_engine_t", "graph_info_t", "spread_message"]

@@ -20,6 +20,7 @@ PE_KT_PULL_BIN_SCATTER, PE_KT_PULL_BIN_GATHER = 5, 6
 PE_PULL_MERGE_PATH, PE_PULL_BINNED = 0, 1
 PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED, PE_GRAPH_RELABEL_DEGREE = 1, 2, 4, 8
 PE_GRAPH_ENCODE_IDS = 16
+PE_SPREAD_COUNT, PE_SPREAD_ADD = 0, 1
 
 
 class PlatoHipError(RuntimeError):
@@ -162,6 +163,7 @@ SIGNATURES = {
     "pe_bitmap_upload": [_vp, _vp, _u64],
     "pe_bitmap_download": [_vp, _vp, _u64],
     "pe_bitmap_to_vector": [_vp, _vp, _u64, _P(_u64)],
+    "pe_spread_message": [_vp, _i32, _vp, _vp, _vp, _P(_u64)],
     "pe_graph_out_degrees": [_vp, _vp],
     "pe_graph_in_degrees": [_vp, _vp],
     "pe_active_edges": [_vp, _vp, _P(_u64)],

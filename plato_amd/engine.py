@@ -398,6 +398,20 @@ class Bitmap:
         return ids[: n.value]
 
 
+def spread_message(actives: Bitmap, sink: DenseState, key: DenseState | None = None,
+                   value: DenseState | None = None) -> int:
+    """spread_message (plato/graph/message_passing.hpp:152-249) with a device sink: each
+    vertex set in `actives` sends one message to key[v] (or v); a uint32 sink counts the
+    messages per target, a float sink adds value[v] (or 1).  Returns the number of
+    messages (the sum of the sink's returns)."""
+    op = L.PE_SPREAD_COUNT if sink.dtype == np.uint32 else L.PE_SPREAD_ADD
+    n = C.c_uint64()
+    check(actives.lib.pe_spread_message(actives.h, op, key.h if key is not None else None,
+                                        value.h if value is not None else None, sink.h, C.byref(n)),
+          actives.ctx.h)
+    return n.value
+
+
 class dualmode_engine_t:
     """The engine of plato/engine/dualmode.hpp:75-166 over one device graph.
 

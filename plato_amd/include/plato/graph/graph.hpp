@@ -2,6 +2,7 @@
 #pragma once
 
 #include "plato/graph/base.hpp"
+#include "plato/graph/message_passing.hpp"
 #include "plato/graph/state/dense_state.hpp"
 #include "plato/graph/structure.hpp"
 #include "plato/util/bitmap.hpp"

@@ -12,6 +12,7 @@
 #include "plato/algo/trustrank/trustrank.hpp"
 #include "plato/algo/cgm/connected_component.hpp"
 #include "plato/util/foutput.h"
+#include "plato/graph/message_passing.hpp"
 
 #include <zlib.h>
 
@@ -213,6 +214,41 @@ static void CGM_CalcComponent() {
   EXPECT(cc.get_major_vertices() == cc.get_components_info()->at(cc.get_major_label()).vertices_);
 }
 
+// plato/graph/test/message_passing.cc:90-169: spread_message over a full and a sparse
+// bitmap of graph_10_9.csv's 10 vertices, each active vertex sending itself to a sink
+// that counts per vertex and returns 1.
+static void MessagePassing_SpreadMessage() {
+  plato::graph_info_t graph_info(true);
+  auto pbcsr = plato::create_bcsr_seqd_from_path<float>(&graph_info, dir + "/graph_10_9.csv",
+                                                        plato::edge_format_t::CSV, plato::float_decoder);
+  using part_t = plato::sequence_balanced_by_destination_t;
+  {
+    plato::bitmap_t<> bitmap(graph_info.vertices_);
+    bitmap.fill();
+    plato::dense_state_t<uint32_t, part_t> values((uint64_t)10);
+    values.fill(0);
+    const int sum = plato::spread_message<plato::vid_t, int>(bitmap, plato::ops::send_self{},
+                                                            plato::ops::count_into(values));
+    EXPECT(sum == 10);
+    for (plato::vid_t i = 0; i < 10; ++i) EXPECT(values[i] == 1u);
+  }
+  {
+    plato::bitmap_t<> bitmap(graph_info.vertices_);
+    bitmap.set_bit(0);
+    bitmap.set_bit(5);
+    bitmap.set_bit(9);
+    plato::dense_state_t<uint32_t, part_t> values((uint64_t)10);
+    values.fill(0);
+    const int sum = plato::spread_message<plato::vid_t, int>(bitmap, plato::ops::send_self{},
+                                                            plato::ops::count_into(values));
+    EXPECT(sum == 3);
+    EXPECT(values[0] == 1u && values[5] == 1u && values[9] == 1u);
+    uint32_t total = 0;
+    for (plato::vid_t i = 0; i < 10; ++i) total += values[i];
+    EXPECT(total == 3u);
+  }
+}
+
 // The plato:: calls of the reference driver example/pagerank_simple.cc:67-106 in the
 // form that file writes them: the three-argument factory template with six arguments,
 // pagerank<graph_spec_t> returning a dense state, its mutable foreach, and
@@ -294,6 +330,7 @@ int main(int argc, char** argv) {
       {"BFS.BinaryTree", BFS_BinaryTree},
       {"TrustRank.Chain", TrustRank_Chain},
       {"CGM.CalcComponent", CGM_CalcComponent},
+      {"MessagePassing.SpreadMessage", MessagePassing_SpreadMessage},
       {"Reference.PagerankSimpleCallSites", Reference_PagerankSimpleCallSites},
   };
   for (auto& c : cases) {

@@ -120,11 +120,72 @@ def test_bitmap_known_answers(ctx):
     c = Bitmap(ctx, 10)
     c.copy_from(f)
     assert c.count() == 10
-    # message_passing.cc:125-169: spread over a sparse active set {0, 5, 9} reaches 3 vertices
     sp = Bitmap(ctx, 10)
     for v in (0, 5, 9):
         sp.set_bit(v)
     assert sp.count() == 3 and sp.to_vector().tolist() == [0, 5, 9]
+
+
+def test_spread_message_known_answers(ctx):
+    """message_passing.cc:90-169: every active vertex sends itself, the sink counts per
+    vertex and returns 1: 10 over the full set of graph_10_9.csv's vertices, 3 over
+    {0, 5, 9}, one count at each of those."""
+    from plato_amd import Bitmap, DenseState, spread_message
+    full = Bitmap(ctx, 10)
+    full.fill()
+    counts = DenseState(ctx, 10, np.uint32)
+    counts.fill(0)
+    assert spread_message(full, counts) == 10
+    assert counts.numpy().tolist() == [1] * 10
+    sp = Bitmap(ctx, 10)
+    for v in (0, 5, 9):
+        sp.set_bit(v)
+    counts.fill(0)
+    assert spread_message(sp, counts) == 3
+    c = counts.numpy()
+    assert c[[0, 5, 9]].tolist() == [1, 1, 1] and int(c.sum()) == 3
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.float64, np.float32])
+def test_spread_message_keys_and_values(ctx, dtype):
+    """A random active set over 2^20 + 37 ids (a partial last word), messages sent to
+    key[v] < 5000: counts against numpy's bincount, sums of value[v] against numpy."""
+    from plato_amd import Bitmap, DenseState, spread_message
+    n, m = (1 << 20) + 37, 5000
+    rng = np.random.default_rng(4)
+    act = rng.random(n) < 0.3
+    words = np.zeros((n >> 6) + 1, np.uint64)
+    ids = np.flatnonzero(act)
+    np.bitwise_or.at(words, ids >> 6, np.left_shift(np.uint64(1), (ids & 63).astype(np.uint64)))
+    b = Bitmap(ctx, n)
+    b.upload_words(words)
+    keys = rng.integers(0, m, n).astype(np.uint32)
+    kd = DenseState(ctx, n, np.uint32)
+    kd.upload(keys)
+    sink = DenseState(ctx, m, dtype)
+    sink.fill(0)
+    if dtype == np.uint32:
+        assert spread_message(b, sink, key=kd) == ids.size
+        assert np.array_equal(sink.numpy(), np.bincount(keys[ids], minlength=m).astype(np.uint32))
+        return
+    vals = rng.random(n).astype(dtype)
+    vd = DenseState(ctx, n, dtype)
+    vd.upload(vals)
+    assert spread_message(b, sink, key=kd, value=vd) == ids.size
+    want = np.bincount(keys[ids], weights=vals[ids].astype(np.float64), minlength=m)
+    np.testing.assert_allclose(sink.numpy().astype(np.float64), want,
+                               rtol=1e-12 if dtype == np.float64 else 2e-5)
+
+
+def test_spread_message_target_out_of_range(ctx):
+    from plato_amd import Bitmap, DenseState, spread_message
+    from plato_amd._lib import PlatoHipError
+    b = Bitmap(ctx, 100)
+    b.fill()
+    sink = DenseState(ctx, 50, np.uint32)
+    sink.fill(0)
+    with pytest.raises(PlatoHipError, match="beyond the sink"):
+        spread_message(b, sink)
 
 
 def test_dense_state_ops(ctx):
