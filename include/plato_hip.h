@@ -254,6 +254,18 @@ int pe_choose_mode(pe_graph_t* g, pe_bitmap_t* actives, double threshold, int* i
  * weighted_pagerank.hpp:115-128).  x, y: PE_F32 or PE_F64, length V. */
 int pe_pull_sum(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted);
 
+/* pe_pull_sum with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The scratch pe_pull_sum allocates
+ * per call (compensated partial sums per merge-path tile, per-block partials) comes
+ * from a static device array of PE_PULL_SUM_NOALLOC_MAX_BYTES, one per GPU, held until
+ * the process ends; a graph needing more (about 2 x 16 B per 2048 (edge + vertex)
+ * items in FP64, half that in FP32, so up to ~4 G items in FP64) fails with
+ * PE_ERR_RANGE.  Kernel timing is skipped.  Every call on one GPU shares its array:
+ * calls that overlap on one GPU (other streams or contexts) race. */
+#define PE_PULL_SUM_NOALLOC_MAX_BYTES (64u << 20)
+int pe_pull_sum_noalloc(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted);
+
 /* Restart-apply of the rank algorithms (the dense_state_t::foreach lambdas of
  * pagerank.hpp:118-140, weighted_pagerank.hpp:130-148,
  * personalized_pagerank.hpp:101-125, trustrank.hpp:226-250). */

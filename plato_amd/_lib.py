@@ -169,6 +169,7 @@ SIGNATURES = {
     "pe_active_edges": [_vp, _vp, _P(_u64)],
     "pe_choose_mode": [_vp, _vp, _dbl, _P(_i32), _P(_u64)],
     "pe_pull_sum": [_vp, _vp, _vp, _i32],
+    "pe_pull_sum_noalloc": [_vp, _vp, _vp, _i32],
     "pe_rank_apply": [_vp, _vp, _vp, _P(ApplyParams), _P(_dbl)],
     "pe_rank_iterate": [_vp, _vp, _vp, _i32, _P(ApplyParams), _P(_dbl)],
     "pe_pagerank": [_vp, _P(RankOpts), _vp, _vp, _P(RankStats)],

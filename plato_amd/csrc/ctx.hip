@@ -1,5 +1,6 @@
 // Context, allocation ledger, timers, engine state vectors and bitmaps.
 #include "pe_internal.hpp"
+#include "pull.hpp"
 
 namespace pe {
 
@@ -24,7 +25,7 @@ void dfree(pe_ctx_s* ctx, void* p, size_t bytes) {
 }
 
 ktime_scope::ktime_scope(pe_ctx_s* c, int w) : ctx(c), which(w) {
-  if (!ctx->timing) return;
+  if (!ctx || !ctx->timing) return;
   hipEvent_t e0;
   PE_HIP(hipEventCreate(&e0));
   PE_HIP(hipEventCreate(&e1));
@@ -249,6 +250,7 @@ int pe_ctx_create(int device, pe_ctx_t** out) {
     PE_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
     PE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_mail), 4096, hipHostMallocCoherent));
     ctx->host_mail[0] = 0ull;
+    ctx->noalloc_scratch = pull_scratch_setup(device);  // with the device current
   });
   if (rc != PE_OK) {
     std::fprintf(stderr, "pe_ctx_create: %s\n", ctx->last_error.c_str());

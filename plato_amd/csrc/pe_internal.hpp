@@ -63,6 +63,8 @@ struct pe_ctx_s {
   // the host polls: a step's counters reach the host without a copy or a stream wait
   unsigned long long* host_mail = nullptr;  // 4 KiB; [0] = sequence number, [1..] = words
   unsigned long long mail_seq = 0;
+  // this device's static scratch array of pe_pull_sum_noalloc (its device address)
+  void* noalloc_scratch = nullptr;
 };
 
 namespace pe {

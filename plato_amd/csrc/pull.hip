@@ -21,6 +21,8 @@
 // Per-column order is fixed by the tiling, so results are bit-reproducible.
 #include <algorithm>
 
+#include <mutex>
+
 #include "pe_internal.hpp"
 #include "pull.hpp"
 
@@ -447,25 +449,25 @@ static void launch_tiles_t(pe_graph_s* g, T* b0, T* b1, const rank_args& a, cons
                            rank_work& wk) {
   hipStream_t s = g->ctx->stream;
   const unsigned G = wk.tile_grid;
-  auto* tc = reinterpret_cast<cpair<T>*>(wk.tile_carry.p);
-  auto* th = reinterpret_cast<cpair<T>*>(wk.tile_head.p);
+  auto* tc = reinterpret_cast<cpair<T>*>(wk.tile_carry);
+  auto* th = reinterpret_cast<cpair<T>*>(wk.tile_head);
   const float* wn = W ? g->wn.p : nullptr;
-  ktime_scope kt(g->ctx, PE_KT_PULL_TILE);
+  ktime_scope kt(wk.timed ? g->ctx : nullptr, PE_KT_PULL_TILE);
   switch (g->tile_items) {
     case 1024:
       k_pull_tiles<T, W, 256, 4><<<G, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, wn, g->tile_row.p,
                                                    g->tile_edge.p, g->num_tiles, b0, b1, g->od.p, a,
-                                                   ctl, tc, th, wk.part_delta.p, wk.part_rsum.p);
+                                                   ctl, tc, th, wk.part_delta, wk.part_rsum);
       break;
     case 4096:
       k_pull_tiles<T, W, 256, 16><<<G, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, wn, g->tile_row.p,
                                                     g->tile_edge.p, g->num_tiles, b0, b1, g->od.p, a,
-                                                    ctl, tc, th, wk.part_delta.p, wk.part_rsum.p);
+                                                    ctl, tc, th, wk.part_delta, wk.part_rsum);
       break;
     default:
       k_pull_tiles<T, W, 256, 8><<<G, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, wn, g->tile_row.p,
                                                    g->tile_edge.p, g->num_tiles, b0, b1, g->od.p, a,
-                                                   ctl, tc, th, wk.part_delta.p, wk.part_rsum.p);
+                                                   ctl, tc, th, wk.part_delta, wk.part_rsum);
   }
   PE_HIP(hipGetLastError());
 }
@@ -477,20 +479,20 @@ static void launch_iteration_t(pe_graph_s* g, T* b0, T* b1, bool weighted, const
   if (weighted) launch_tiles_t<T, true>(g, b0, b1, a, ctl, wk);
   else launch_tiles_t<T, false>(g, b0, b1, a, ctl, wk);
   const uint64_t nshort = g->num_fix - g->num_fix_long, nlong = g->num_fix_long;
-  auto* tc = reinterpret_cast<cpair<T>*>(wk.tile_carry.p);
-  auto* th = reinterpret_cast<cpair<T>*>(wk.tile_head.p);
+  auto* tc = reinterpret_cast<cpair<T>*>(wk.tile_carry);
+  auto* th = reinterpret_cast<cpair<T>*>(wk.tile_head);
   {
-    ktime_scope kt(g->ctx, PE_KT_PULL_FIXUP);
+    ktime_scope kt(wk.timed ? g->ctx : nullptr, PE_KT_PULL_FIXUP);
     k_pull_fixup_short<T><<<wk.fix_grid, 256, 0, s>>>(
         g->fix_row.p, g->fix_t0.p, g->fix_t1.p, nshort, tc, th, b0, b1, g->od.p, a, ctl,
-        wk.part_delta.p + wk.tile_grid, wk.part_rsum.p + wk.tile_grid);
+        wk.part_delta + wk.tile_grid, wk.part_rsum + wk.tile_grid);
     k_pull_fixup_long<T><<<wk.fix_grid, 256, 0, s>>>(
         g->fix_row.p + nshort, g->fix_t0.p + nshort, g->fix_t1.p + nshort, nlong, tc, th, b0, b1,
-        g->od.p, a, ctl, wk.part_delta.p + wk.tile_grid + wk.fix_grid,
-        wk.part_rsum.p + wk.tile_grid + wk.fix_grid);
+        g->od.p, a, ctl, wk.part_delta + wk.tile_grid + wk.fix_grid,
+        wk.part_rsum + wk.tile_grid + wk.fix_grid);
   }
   PE_HIP(hipGetLastError());
-  k_iter_finish<<<1, 256, 0, s>>>(wk.part_delta.p, wk.part_rsum.p, wk.tile_grid + 2 * wk.fix_grid, ctl,
+  k_iter_finish<<<1, 256, 0, s>>>(wk.part_delta, wk.part_rsum, wk.tile_grid + 2 * wk.fix_grid, ctl,
                                   deltas, rsums);
   PE_HIP(hipGetLastError());
 }
@@ -500,15 +502,74 @@ void rank_work::init(pe_graph_s* g, int precision) {
   const size_t pair_bytes = precision == 64 ? 16 : 8;
   tile_grid = persistent_grid(ctx, g->num_tiles, 8);
   fix_grid = persistent_grid(ctx, std::max<uint64_t>(1, (g->num_fix + 255) / 256), 2);
-  tile_carry.alloc(ctx, g->num_tiles * pair_bytes);
-  tile_head.alloc(ctx, g->num_tiles * pair_bytes);
+  own_tiles.alloc(ctx, 2 * g->num_tiles * pair_bytes);
+  tile_carry = own_tiles.p;
+  tile_head = own_tiles.p + g->num_tiles * pair_bytes;
   pb_parts = 0;
   if (precision != 64 && g->pull_layout == PE_PULL_BINNED && g->pb)
     pb_parts = binned_grid_gather(g) + binned_grid_split(g);
   const size_t parts = std::max<size_t>(tile_grid + 2 * fix_grid, pb_parts);
-  part_delta.alloc(ctx, parts);
-  part_rsum.alloc(ctx, parts);
-  scalars.alloc(ctx, 2);
+  own_parts.alloc(ctx, 2 * parts + 2);
+  part_delta = own_parts.p;
+  part_rsum = own_parts.p + parts;
+  scalars = own_parts.p + 2 * parts;
+  timed = true;
+}
+
+static uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
+
+uint64_t rank_work::arena_bytes(pe_graph_s* g, int precision) {
+  const uint64_t pair_bytes = precision == 64 ? 16 : 8;
+  const uint64_t tg = persistent_grid(g->ctx, g->num_tiles, 8);
+  const uint64_t fg = persistent_grid(g->ctx, std::max<uint64_t>(1, (g->num_fix + 255) / 256), 2);
+  const uint64_t parts = tg + 2 * fg;
+  return 2 * align256(g->num_tiles * pair_bytes) + 2 * align256(parts * 8) + align256(16);
+}
+
+void rank_work::init_in(pe_graph_s* g, int precision, uint8_t* arena, uint64_t cap) {
+  const uint64_t need = arena_bytes(g, precision);
+  if (need > cap) {
+    char msg[160];
+    std::snprintf(msg, sizeof(msg), "graph needs %llu bytes of scratch, above the %llu-byte static array",
+                  (unsigned long long)need, (unsigned long long)cap);
+    fail(PE_ERR_RANGE, msg);
+  }
+  const uint64_t pair_bytes = precision == 64 ? 16 : 8;
+  tile_grid = persistent_grid(g->ctx, g->num_tiles, 8);
+  fix_grid = persistent_grid(g->ctx, std::max<uint64_t>(1, (g->num_fix + 255) / 256), 2);
+  const uint64_t parts = tile_grid + 2 * fix_grid;
+  uint8_t* p = arena;
+  tile_carry = p;
+  p += align256(g->num_tiles * pair_bytes);
+  tile_head = p;
+  p += align256(g->num_tiles * pair_bytes);
+  part_delta = reinterpret_cast<double*>(p);
+  p += align256(parts * 8);
+  part_rsum = reinterpret_cast<double*>(p);
+  p += align256(parts * 8);
+  scalars = reinterpret_cast<double*>(p);
+  pb_parts = 0;  // raw sums and FP64 iterations never take the binned layout
+  timed = false;
+}
+
+// The static scratch array of pe_pull_sum_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes its name to a kernel: it is a host-side
+// stand-in, not a device address.  Every user goes through the address that
+// hipGetSymbolAddress returns for the device (pull_scratch_setup).
+static __device__ __align__(256) uint8_t scratch[PE_PULL_SUM_NOALLOC_MAX_BYTES];
+
+void* pull_scratch_setup(int device) {
+  static std::mutex mu;
+  static void* addr[256] = {};
+  PE_REQUIRE(device >= 0 && device < 256, PE_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* ptr = nullptr;
+    PE_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    PE_REQUIRE(ptr != nullptr, PE_ERR_HIP, "no device address for the scratch array");
+    addr[device] = ptr;
+  }
+  return addr[device];
 }
 
 void rank_iteration(pe_graph_s* g, int precision, void* b0, void* b1, bool weighted,
@@ -516,8 +577,8 @@ void rank_iteration(pe_graph_s* g, int precision, void* b0, void* b1, bool weigh
                     double* rsums) {
   if (precision != 64 && a.mode == MODE_APPLY && a.mass > 0.0 && g->pull_layout == PE_PULL_BINNED && g->pb) {
     PE_REQUIRE(wk.pb_parts > 0, PE_ERR_STATE, "pull layout changed after the iteration was set up");
-    binned_iteration(g, (float*)b0, (float*)b1, weighted, a, ctl, wk.part_delta.p, wk.part_rsum.p);
-    k_iter_finish<<<1, 256, 0, g->ctx->stream>>>(wk.part_delta.p, wk.part_rsum.p, wk.pb_parts, ctl, deltas,
+    binned_iteration(g, (float*)b0, (float*)b1, weighted, a, ctl, wk.part_delta, wk.part_rsum);
+    k_iter_finish<<<1, 256, 0, g->ctx->stream>>>(wk.part_delta, wk.part_rsum, wk.pb_parts, ctl, deltas,
                                                  rsums);
     PE_HIP(hipGetLastError());
     return;
@@ -534,13 +595,13 @@ void rank_apply_only(pe_graph_s* g, int precision, const void* x, void* y, const
   const unsigned G = wk.tile_grid;
   if (precision == 64)
     k_rank_apply<double><<<G, 256, 0, s>>>(g->V, (const double*)x, (double*)y, g->od.p, a,
-                                           wk.part_delta.p, wk.part_rsum.p);
+                                           wk.part_delta, wk.part_rsum);
   else
     k_rank_apply<float><<<G, 256, 0, s>>>(g->V, (const float*)x, (float*)y, g->od.p, a,
-                                          wk.part_delta.p, wk.part_rsum.p);
+                                          wk.part_delta, wk.part_rsum);
   PE_HIP(hipGetLastError());
-  k_iter_finish<<<1, 256, 0, s>>>(wk.part_delta.p, wk.part_rsum.p, G, nullptr, wk.scalars.p,
-                                  wk.scalars.p + 1);
+  k_iter_finish<<<1, 256, 0, s>>>(wk.part_delta, wk.part_rsum, G, nullptr, wk.scalars,
+                                  wk.scalars + 1);
   PE_HIP(hipGetLastError());
 }
 

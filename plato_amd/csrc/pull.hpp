@@ -99,12 +99,27 @@ __device__ __forceinline__ void block_reduce2(double& a, double& b, double* sm) 
 
 struct rank_work {
   unsigned tile_grid = 0, fix_grid = 0;
-  unsigned pb_parts = 0;                // partials of a binned iteration (0: not binned)
-  dbuf<uint8_t> tile_carry, tile_head;  // compensated pair per tile
-  dbuf<double> part_delta, part_rsum;   // per-block partials
-  dbuf<double> scalars;                 // delta, rank sum (engine-level calls)
+  unsigned pb_parts = 0;                     // partials of a binned iteration (0: not binned)
+  uint8_t* tile_carry = nullptr;             // compensated pair per tile
+  uint8_t* tile_head = nullptr;
+  double* part_delta = nullptr;              // per-block partials
+  double* part_rsum = nullptr;
+  double* scalars = nullptr;                 // delta, rank sum (engine-level calls)
+  bool timed = true;                         // per-kernel event timing allowed
+  dbuf<uint8_t> own_tiles;                   // storage when init() allocates it
+  dbuf<double> own_parts;
   void init(pe_graph_s* g, int precision);
+  // The same buffers carved from caller-provided device memory (cap bytes, 256-byte
+  // aligned) with no allocation and no kernel timing; fails with PE_ERR_RANGE when the
+  // graph needs more than cap.
+  void init_in(pe_graph_s* g, int precision, uint8_t* arena, uint64_t cap);
+  // bytes init_in needs for this graph and precision
+  static uint64_t arena_bytes(pe_graph_s* g, int precision);
 };
+
+// Address of this GPU's static scratch array of pe_pull_sum_noalloc (looked up once per
+// device, with the device current, by pe_ctx_create).
+void* pull_scratch_setup(int device);
 
 // One pull iteration (tiles + fix-up + finish) on the context stream.
 // With ctl: buffers picked by ctl->cur, deltas/rsums are the per-iteration logs.

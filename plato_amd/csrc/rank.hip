@@ -331,8 +331,28 @@ int pe_pull_sum(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted) {
     wk.init(g, vec_prec(x));
     rank_args a = make_args(1.0, 0.0, 0.0, nullptr, 0, nullptr);
     a.mode = MODE_RAW_SUM;
-    rank_iteration(g, vec_prec(x), x->p, y->p, weighted != 0, a, nullptr, wk, wk.scalars.p,
-                   wk.scalars.p + 1);
+    rank_iteration(g, vec_prec(x), x->p, y->p, weighted != 0, a, nullptr, wk, wk.scalars,
+                   wk.scalars + 1);
+    PE_HIP(hipStreamSynchronize(g->ctx->stream));
+  });
+}
+
+int pe_pull_sum_noalloc(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted) {
+  if (!g || !x || !y) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(x->dtype == y->dtype && (x->dtype == PE_F32 || x->dtype == PE_F64), PE_ERR_INVALID,
+               "x and y must both be PE_F32 or both PE_F64");
+    PE_REQUIRE(x->n >= g->V && y->n >= g->V, PE_ERR_INVALID, "vectors shorter than V");
+    PE_REQUIRE(!weighted || g->has_weights, PE_ERR_STATE, "graph has no weights");
+    PE_REQUIRE(g->ctx->noalloc_scratch != nullptr, PE_ERR_STATE, "context has no scratch array");
+    // the same kernels as pe_pull_sum, their tile partials and per-block sums in this
+    // GPU's static scratch array instead of fresh allocations (nothing there needs to
+    // start at zero: every word is written before it is read)
+    rank_work wk;
+    wk.init_in(g, vec_prec(x), static_cast<uint8_t*>(g->ctx->noalloc_scratch), PE_PULL_SUM_NOALLOC_MAX_BYTES);
+    rank_args a = make_args(1.0, 0.0, 0.0, nullptr, 0, nullptr);
+    a.mode = MODE_RAW_SUM;
+    rank_iteration(g, vec_prec(x), x->p, y->p, weighted != 0, a, nullptr, wk, wk.scalars, wk.scalars + 1);
     PE_HIP(hipStreamSynchronize(g->ctx->stream));
   });
 }
@@ -349,7 +369,7 @@ int pe_rank_apply(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, const pe_apply_params
                                   p->divide_by_od, p);
     rank_apply_only(g, vec_prec(x), x->p, y->p, a, wk);
     double h = 0.0;
-    PE_HIP(hipMemcpyAsync(&h, wk.scalars.p, 8, hipMemcpyDeviceToHost, g->ctx->stream));
+    PE_HIP(hipMemcpyAsync(&h, wk.scalars, 8, hipMemcpyDeviceToHost, g->ctx->stream));
     PE_HIP(hipStreamSynchronize(g->ctx->stream));
     if (delta) *delta = h;
   });
@@ -369,10 +389,10 @@ int pe_rank_iterate(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted, const
                             p->divide_by_od, p);
     if (vec_prec(x) == 32 && g->pull_layout == PE_PULL_BINNED && g->pb)
       a.mass = binned_mass_bound(g, (const float*)x->p, weighted != 0);
-    rank_iteration(g, vec_prec(x), x->p, y->p, weighted != 0, a, nullptr, wk, wk.scalars.p,
-                   wk.scalars.p + 1);
+    rank_iteration(g, vec_prec(x), x->p, y->p, weighted != 0, a, nullptr, wk, wk.scalars,
+                   wk.scalars + 1);
     double h = 0.0;
-    PE_HIP(hipMemcpyAsync(&h, wk.scalars.p, 8, hipMemcpyDeviceToHost, g->ctx->stream));
+    PE_HIP(hipMemcpyAsync(&h, wk.scalars, 8, hipMemcpyDeviceToHost, g->ctx->stream));
     PE_HIP(hipStreamSynchronize(g->ctx->stream));
     if (delta) *delta = h;
   });
@@ -459,7 +479,7 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
       a.final_iter = final_iter ? 1 : 0;
       if (!do_pull) do_pull = active_edges_dev(g, active.p, nwords) > Ep / 20;  // (188-205)
       if (do_pull) {
-        rank_iteration(g, prec, x, y, false, a, nullptr, wk, wk.scalars.p, wk.scalars.p + 1);
+        rank_iteration(g, prec, x, y, false, a, nullptr, wk, wk.scalars, wk.scalars + 1);
       } else {
         const unsigned zg = grid_for(V, 256, 8192);
         if (prec == 64) k_zero_t<double><<<zg, 256, 0, s>>>((double*)y, V);
@@ -485,7 +505,7 @@ int pe_ppr_pushpull(pe_graph_t* g, uint32_t src, const pe_rank_opts_t* opts, dou
         rank_apply_only(g, prec, x, y, a, wk);
       }
       double delta = 0.0;
-      PE_HIP(hipMemcpyAsync(&delta, wk.scalars.p, 8, hipMemcpyDeviceToHost, s));
+      PE_HIP(hipMemcpyAsync(&delta, wk.scalars, 8, hipMemcpyDeviceToHost, s));
       PE_HIP(hipStreamSynchronize(s));
       if (modes) modes[executed] = do_pull ? 0 : 1;
       if (deltas) deltas[executed] = final_iter ? 0.0 : delta;
