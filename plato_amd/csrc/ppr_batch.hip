@@ -86,6 +86,10 @@ __device__ __forceinline__ float batch_apply(const lane_cfg& L, double damping, 
 #define PE_BATCH_UNROLL 4
 #endif
 static constexpr int BU = PE_BATCH_UNROLL;  // row gathers in flight per wave
+#ifndef PE_BATCH_GROUP
+#define PE_BATCH_GROUP 32  // columns per task at most (their pointers fit one load per lane)
+#endif
+static_assert(PE_BATCH_GROUP >= 1 && PE_BATCH_GROUP <= 63, "a task's column pointers take one lane each");
 // Cache hints: the new block Y and the edge list stream through once per pass, so
 // they are stored / loaded non-temporally and leave the caches to the hot rows of X
 // (with degree relabeling the first 2^20 rows cover ~88% of the gathers at scale 24).
@@ -121,6 +125,39 @@ __device__ __forceinline__ float ld_row(const float* X, uint32_t u, int lane) {
   return *p;
 }
 
+// Sum over the in-edges [eb, ee) of one column in ascending source order, BU rows in
+// flight per wave: every group of BU edges (the last one masked) costs one round trip
+// for its ids (scalar loads: the range is wave-uniform) and one for its rows.  Lanes
+// whose column needs no sum issue no row loads.
+__device__ __forceinline__ void column_sum(const float* X, const uint32_t* __restrict__ row_idx, uint64_t eb,
+                                           uint64_t ee, int lane, bool needs, float& s, float& c) {
+  for (uint64_t e = eb; e < ee; e += BU) {
+    const uint32_t n = (uint32_t)std::min<uint64_t>(BU, ee - e);  // wave-uniform
+    uint32_t u[BU];
+#pragma unroll
+    for (int i = 0; i < BU; ++i) u[i] = ld_edge(row_idx + e + ((uint32_t)i < n ? i : 0));
+    float vals[BU];
+#pragma unroll
+    for (int i = 0; i < BU; ++i) vals[i] = (needs && (uint32_t)i < n) ? ld_row(X, u[i], lane) : 0.0f;
+#pragma unroll
+    for (int i = 0; i < BU; ++i)
+      if ((uint32_t)i < n) kahan_add(s, c, vals[i]);
+  }
+}
+
+// the value of lane + 1 (64-bit, as two 32-bit shuffles)
+__device__ __forceinline__ uint64_t lane_pair_down(uint64_t v) {
+  const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)v, 1, 64);
+  const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(v >> 32), 1, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int i) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, i);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), i);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __global__ __launch_bounds__(256) void k_batch_tasks(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
     const uint32_t* __restrict__ od, const uint64_t* __restrict__ seedmask,
@@ -137,40 +174,45 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
   L.status = status[lane];
   L.init_dist = init_dist[lane];
   L.restart = (1.0 - damping) * L.init_dist;
+  // only ACTIVE and FINAL columns use the gathered sum; when none does, no row is read
+  const bool needs = L.status == COL_ACTIVE || L.status == COL_FINAL;
+  const bool any_needs = __ballot(needs) != 0ull;
   double delta = 0.0;
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
   for (uint64_t ti = wave_id; ti < ntasks; ti += nwaves) {
     const bt_task t = tasks[ti];
     if (t.chunk == 0xFFFFFFFFu) {
-      for (uint32_t v = t.v0; v < t.v1; ++v) {
-        const uint64_t eb = col_ptr[v], ee = col_ptr[v + 1];
+      // the task's columns (at most 63) with their pointers, seed masks and out-degrees
+      // in one load per lane, read back per column with readlane
+      const uint32_t n = t.v1 - t.v0;
+      const uint64_t cp = (uint32_t)lane <= n ? col_ptr[t.v0 + lane] : 0ull;
+      const uint64_t smv = (uint32_t)lane < n ? seedmask[t.v0 + lane] : 0ull;
+      const uint32_t odv = (uint32_t)lane < n ? od[t.v0 + lane] : 0u;
+      // A vertex without in-edges that seeds no column holds 0 in every column and both
+      // buffers, in every column state (its sum is 0 and it gets no restart), so it is
+      // skipped: no read, no write.  Release zeroes a seed's entries when it stops being one.
+      const uint64_t cp_next = lane_pair_down(cp);
+      const bool live = (uint32_t)lane < n && (cp != cp_next || smv != 0ull);
+      const uint64_t live_mask = __ballot(live);
+      float xnext = 0.f;  // the old row of the first live vertex, ahead
+      if (live_mask) xnext = X[(uint64_t)(t.v0 + __builtin_ctzll(live_mask)) * BK + lane];
+      for (uint32_t i = 0; i < n; ++i) {
+        if (!((live_mask >> i) & 1ull)) continue;  // wave-uniform
+        const uint32_t v = t.v0 + i;
+        const float xold = xnext;
+        const uint64_t rest = live_mask & ~((2ull << i) - 1ull);
+        if (rest) xnext = X[(uint64_t)(t.v0 + __builtin_ctzll(rest)) * BK + lane];  // the next live row, ahead
         float s = 0.f, c = 0.f;
-        uint64_t e = eb;
-        for (; e + BU <= ee; e += BU) {
-          float vals[BU];
-#pragma unroll
-          for (int i = 0; i < BU; ++i) vals[i] = ld_row(X, ld_edge(row_idx + e + i), lane);
-#pragma unroll
-          for (int i = 0; i < BU; ++i) kahan_add(s, c, vals[i]);
-        }
-        for (; e < ee; ++e) kahan_add(s, c, ld_row(X, ld_edge(row_idx + e), lane));
-        const uint64_t sm = seedmask[v];
-        const float xold = X[(uint64_t)v * BK + lane];
+        if (any_needs) column_sum(X, row_idx, lane_u64(cp, i), lane_u64(cp, i + 1), lane, needs, s, c);
+        const uint64_t sm = lane_u64(smv, i);
+        const uint32_t d = __builtin_amdgcn_readlane(odv, i);
         st_row(Y + (uint64_t)v * BK + lane,
-               batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta));
+               batch_apply(L, damping, (double)s - (double)c, xold, d, (sm >> lane) & 1ull, delta));
       }
     } else {
       float s = 0.f, c = 0.f;
-      uint64_t e = t.e0;
-      for (; e + BU <= t.e1; e += BU) {
-        float vals[BU];
-#pragma unroll
-        for (int i = 0; i < BU; ++i) vals[i] = ld_row(X, ld_edge(row_idx + e + i), lane);
-#pragma unroll
-        for (int i = 0; i < BU; ++i) kahan_add(s, c, vals[i]);
-      }
-      for (; e < t.e1; ++e) kahan_add(s, c, ld_row(X, ld_edge(row_idx + e), lane));
+      if (any_needs) column_sum(X, row_idx, t.e0, t.e1, lane, needs, s, c);
       partial[(uint64_t)t.chunk * BK + lane] = make_float2(s, c);
     }
   }
@@ -283,6 +325,15 @@ __global__ void k_batch_seed_bits(uint64_t* __restrict__ seedmask, const uint32_
   }
 }
 
+// A released slot's seeds stop being seeds: their entries in that column go to 0 in both
+// buffers, so vertices without in-edges keep the all-zero state the gather skips.
+__global__ void k_batch_zero_slot(float* X0, float* X1, const uint32_t* __restrict__ ids, uint32_t n, uint32_t slot) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    X0[(uint64_t)ids[i] * BK + slot] = 0.0f;
+    X1[(uint64_t)ids[i] * BK + slot] = 0.0f;
+  }
+}
+
 __global__ void k_batch_set_slot(uint32_t* status, double* init_dist, uint32_t* iters, double* last_delta,
                                  uint32_t slot, uint32_t st, double dist) {
   status[slot] = st;
@@ -384,7 +435,7 @@ struct pe_ppr_batch_s {
     std::vector<bt_long> l;
     uint64_t chunk = 0;
     uint32_t v = 0;
-    const uint32_t max_group = 32;
+    const uint32_t max_group = PE_BATCH_GROUP;
     while (v < V) {
       const uint64_t deg = cp[v + 1] - cp[v];
       if (deg > C) {
@@ -582,6 +633,7 @@ int pe_ppr_batch_release(pe_ppr_batch_t* b, uint32_t slot) {
       dbuf<uint32_t> d(b->g->ctx, ids.size());
       PE_HIP(hipMemcpyAsync(d.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, s));
       k_batch_seed_bits<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->seedmask.p, d.p, (uint32_t)ids.size(), slot, 0);
+      k_batch_zero_slot<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->X0.p, b->X1.p, d.p, (uint32_t)ids.size(), slot);
       ids.clear();
     }
     k_batch_set_slot<<<1, 1, 0, s>>>(b->status.p, b->init_dist.p, b->iters.p, b->last_delta.p, slot, COL_EMPTY, 0.0);
