@@ -396,6 +396,21 @@ typedef struct {
  * counters, *seconds the device time. */
 int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double* anfs, uint32_t cap, uint32_t* nanfs,
                 double* avg_distance, uint8_t* registers, double* seconds);
+
+/* n-step degrees: plato::algo::nstepdegrees_t::compute (plato/algo/nstepdegrees/
+ * nstepdegrees.hpp:166-489).  For each target vertex (target_words: a bitmap over input
+ * ids, (V >> 6) + 1 words; NULL = every vertex), the in-degree is the HyperLogLog estimate
+ * (2^bits registers, bits in [6, 16]) of the vertices that reach it within `step` hops,
+ * minus 1, and the out-degree the same for the vertices it reaches, stored as
+ * static_cast<vid_t>(estimate) - 1; other vertices get 0.  step = 0xFFFFFFFF (the driver's
+ * -1) runs until no counter grows.  type: PE_NSTEP_IN, PE_NSTEP_OUT or PE_NSTEP_BOTH; the
+ * out pass of a directed graph needs PE_GRAPH_BUILD_CSR, and an undirected graph's out
+ * degrees equal its in degrees.  Arrays in input id order. */
+#define PE_NSTEP_IN 1
+#define PE_NSTEP_OUT 2
+#define PE_NSTEP_BOTH 3
+int pe_nstep_degrees(pe_graph_t* g, const uint64_t* target_words, uint32_t step, uint32_t bits, int type,
+                     uint32_t* in_degrees, uint32_t* out_degrees, double* seconds);
 /* A new graph with every edge of the directed graph g reversed (same input ids), for
  * pulls over out-edges such as inverse PageRank (the reference runs pagerank over its
  * bcsr_t out-edge structure, trustrank.hpp:101-103).  flags: PE_GRAPH_BUILD_CSR,

@@ -84,6 +84,8 @@ def lib():
     L.orc_format_csv.argtypes = [C.c_uint64, _u32p, _u32p, C.c_void_p, C.c_uint64]
     L.orc_format_csv.restype = C.c_int64
     L.orc_topk.argtypes = [C.c_uint32, _f64p, C.c_uint32, _u32p, _f64p]
+    L.orc_nstep_degrees.argtypes = [C.c_uint32, _u64p, _u32p, C.c_uint32, C.c_uint32, _u32p]
+    L.orc_nstep_degrees.restype = C.c_int
     L.orc_rmat_weights.argtypes = [C.c_uint64, C.c_uint64, _u64p, _f32p]
     L.orc_rmat_edges_at.argtypes = [C.c_uint32, C.c_uint64, _u32p, C.c_uint64, _u64p, _u32p, _u32p]
     L.orc_rmat_scan.argtypes = [C.c_uint32, C.c_uint64, _u32p, C.c_uint64, C.c_uint32, _u32p, _u32p,
@@ -336,6 +338,17 @@ def hyperanf(g: CSC, bits=12, iters=20, max_anfs=4096):
     n = lib().orc_hyperanf(g.V, g.col_ptr, np.ascontiguousarray(g.row_idx), bits, iters, regs, anfs, max_anfs)
     anfs = anfs[:min(n, max_anfs)]
     return anfs, float(lib().orc_anf_avg_distance(anfs, anfs.size)), regs
+
+
+def nstep_degrees(g: CSC, step=20, bits=6, direction="in"):
+    """n-step degrees of every vertex: direction "in" over in-edges (CSC), "out" over
+    out-edges (CSR).  Returns (degrees, iterations run)."""
+    out = np.zeros(g.V, np.uint32)
+    ptr, idx = (g.col_ptr, g.row_idx) if direction == "in" else (g.row_ptr, g.col_idx)
+    n = lib().orc_nstep_degrees(g.V, ptr, np.ascontiguousarray(idx), bits, step & 0xFFFFFFFF, out)
+    if n < 0:
+        raise RuntimeError("orc_nstep_degrees failed")
+    return out, n
 
 
 def cc_labels(g: CSC):

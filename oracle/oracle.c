@@ -850,6 +850,42 @@ int orc_hyperanf(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx, u
   return n;
 }
 
+int orc_nstep_degrees(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t P, uint32_t step,
+                      uint32_t* degree) {
+  const uint64_t m = 1ull << P;
+  uint8_t* regs = (uint8_t*)calloc((size_t)V * m + 1, 1);
+  uint8_t* nxt = (uint8_t*)malloc((size_t)V * m + 1);
+  if (!regs || !nxt) { free(regs); free(nxt); return -1; }
+  for (uint32_t v = 0; v < V; ++v) {  /* books[v].add(&v, 4) (nstepdegrees.hpp:331-336) */
+    const uint32_t h = orc_murmur3_32(&v, 4, 0x5f61767au);
+    const uint32_t r0 = h >> (32 - P);
+    uint32_t w = h << P;
+    uint8_t rank = 1;
+    while (rank <= 32 - P && !(w & 0x80000000u)) { ++rank; w <<= 1; }
+    regs[(size_t)v * m + r0] = rank;
+  }
+  int executed = 0;
+  for (uint32_t it = 0; it < step; ++it) {  /* (343-409): stops when no counter grows */
+    memcpy(nxt, regs, (size_t)V * m);
+    int changed = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+      uint8_t* dst = nxt + (size_t)v * m;
+      for (uint64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+        const uint8_t* src = regs + (size_t)idx[e] * m;
+        for (uint64_t i = 0; i < m; ++i)
+          if (src[i] > dst[i]) { dst[i] = src[i]; changed = 1; }
+      }
+    }
+    uint8_t* t = regs; regs = nxt; nxt = t;
+    ++executed;
+    if (!changed) break;
+  }
+  for (uint32_t v = 0; v < V; ++v)  /* static_cast<vid_t>(estimate()) - 1 (412-421) */
+    degree[v] = (uint32_t)orc_hll_estimate(regs + (size_t)v * m, P) - 1u;
+  free(regs); free(nxt);
+  return executed;
+}
+
 double orc_anf_avg_distance(const double* anfs, uint32_t n) {
   double avg = 0.0;
   for (uint32_t i = 1; i < n; ++i) avg += i * (anfs[i] - anfs[i - 1]);

@@ -211,6 +211,20 @@ double orc_hll_estimate(const uint8_t* regs, uint32_t P);
 int orc_hyperanf(uint32_t V, const uint64_t* col_ptr, const uint32_t* row_idx, uint32_t P, uint32_t iters,
                  uint8_t* regs, double* anfs, uint32_t cap);
 
+/* n-step degrees (plato/algo/nstepdegrees/nstepdegrees.hpp:166-489) over the adjacency
+ * the caller passes: in-edges (a CSC: the in-degree pass, whose books gather over in-edges)
+ * or out-edges (a CSR: the out-degree pass on the reversed engine).  Every vertex's book is a
+ * 2^P-register HyperLogLog started with its own id (as orc_hyperanf); each iteration gives
+ * every vertex the register-wise max of its book and its adjacency's books of the previous
+ * iteration, for `step` iterations or until no book grows; degree[v] =
+ * (uint32)estimate - 1.  With every vertex covered this equals the reference's targets'
+ * values (its covered set holds every vertex within step hops of a target).
+ * Pinned: the reference test's known answers (test_nstepdegrees.cc: raw_graph_7_7.csv,
+ * step 1, bits 6: in(0) = 3, out(0) = 1, in(4) = 1, out(4) = 1) and the HLL of each vertex's
+ * step-hop ball built from scratch. */
+int orc_nstep_degrees(uint32_t V, const uint64_t* ptr, const uint32_t* idx, uint32_t P, uint32_t step,
+                      uint32_t* degree);
+
 /* GetAvgDistance (hyperanf.hpp:58-67) of n anfs. */
 double orc_anf_avg_distance(const double* anfs, uint32_t n);
 

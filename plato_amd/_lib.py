@@ -21,6 +21,7 @@ PE_PULL_MERGE_PATH, PE_PULL_BINNED = 0, 1
 PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED, PE_GRAPH_RELABEL_DEGREE = 1, 2, 4, 8
 PE_GRAPH_ENCODE_IDS = 16
 PE_SPREAD_COUNT, PE_SPREAD_ADD = 0, 1
+PE_NSTEP_IN, PE_NSTEP_OUT, PE_NSTEP_BOTH = 1, 2, 3
 
 
 class PlatoHipError(RuntimeError):
@@ -131,6 +132,7 @@ SIGNATURES = {
     "pe_trustrank": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pe_kcore": [_vp, C.c_int, _vp, _vp, _vp, _u32, _vp],
     "pe_connected_components": [_vp, _vp, _vp, _vp, _u32, _vp],
+    "pe_nstep_degrees": [_vp, _vp, _u32, _u32, _i32, _vp, _vp, _P(_dbl)],
     "pe_hyperanf": [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp],
     "pe_graph_create_rmat": [_vp, _u32, _u32, _u64, _dbl, _dbl, _dbl, _u32, _P(_vp)],
     "pe_rmat_edges": [_vp, _u32, _u64, _dbl, _dbl, _dbl, _u64, _u64, _vp, _vp, _vp],

@@ -273,3 +273,40 @@ def hyperanf(graph: Graph, opts: hyperanf_opts_t = hyperanf_opts_t(), registers:
     check(graph.lib.pe_hyperanf(graph.h, C.byref(o), anfs.ctypes.data, max_anfs, C.byref(n), C.byref(avg),
                                 regs.ctypes.data if registers else None, C.byref(sec)), graph.ctx.h)
     return HyperAnfResult(avg.value, anfs[:min(n.value, max_anfs)], regs, sec.value)
+
+
+@dataclass
+class nstepdegree_opts_t:
+    """plato/algo/nstepdegrees/nstepdegrees.hpp:49-53 (+ the HyperLogLog width, a
+    template argument there)"""
+    step: int = 20
+    type: str = "both"
+    is_directed: bool = False
+    bits: int = 12
+
+
+@dataclass
+class NstepResult:
+    in_degrees: np.ndarray | None
+    out_degrees: np.ndarray | None
+    seconds: float
+
+
+def nstep_degrees(graph: Graph, opts: nstepdegree_opts_t = nstepdegree_opts_t(), targets=None) -> NstepResult:
+    """nstepdegrees_t::compute (plato/algo/nstepdegrees/nstepdegrees.hpp:437-466): n-step
+    in/out degrees of the target vertices (all when targets is None), 0 elsewhere."""
+    t = {"in": L.PE_NSTEP_IN, "out": L.PE_NSTEP_OUT, "both": L.PE_NSTEP_BOTH}[opts.type]
+    V = graph.V
+    din = np.zeros(V, np.uint32) if t & L.PE_NSTEP_IN else None
+    dout = np.zeros(V, np.uint32) if t & L.PE_NSTEP_OUT else None
+    words = None
+    if targets is not None:
+        ids = np.asarray(targets, np.uint64)
+        words = np.zeros((V >> 6) + 1, np.uint64)
+        np.bitwise_or.at(words, (ids >> np.uint64(6)).astype(np.int64), np.left_shift(np.uint64(1), ids & np.uint64(63)))
+    secs = C.c_double()
+    check(graph.lib.pe_nstep_degrees(graph.h, None if words is None else words.ctypes.data,
+                                     int(opts.step) & 0xFFFFFFFF, int(opts.bits), t,
+                                     None if din is None else din.ctypes.data,
+                                     None if dout is None else dout.ctypes.data, C.byref(secs)), graph.ctx.h)
+    return NstepResult(din, dout, secs.value)

@@ -345,6 +345,48 @@ __global__ __launch_bounds__(256) void k_hanf_estimate(const uint8_t* __restrict
   if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
+// n-step degrees: the estimate of each target's counter as the reference stores it,
+// static_cast<vid_t>(estimate) - 1 (nstepdegrees.hpp:412-421); 0 for other vertices.
+// One wave per vertex; targets (input ids, a bitmap) are looked up through perm.
+__global__ __launch_bounds__(256) void k_nstep_degree(const uint8_t* __restrict__ regs, uint32_t V, int P,
+                                                      const uint64_t* __restrict__ targets,
+                                                      const uint32_t* __restrict__ perm, uint32_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t m = 1ull << P;
+  const uint64_t wave_id = grid_wave_id(), nwaves = grid_waves();
+  const double alpha = hanf_alpha(P);
+  for (uint64_t v = wave_id; v < V; v += nwaves) {
+    const uint32_t id = perm ? perm[v] : (uint32_t)v;
+    if (targets && !((targets[id >> 6] >> (id & 63)) & 1ull)) {
+      if (lane == 0) out[v] = 0u;
+      continue;
+    }
+    double s = 0.0;
+    uint32_t zeros = 0;
+    for (uint64_t i = lane; i < m; i += 64) {
+      const uint32_t r = regs[v * m + i];
+      s += hanf_pow2_neg(r);
+      zeros += r == 0 ? 1u : 0u;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      s += __shfl_xor(s, off, 64);
+      zeros += __shfl_xor(zeros, off, 64);
+    }
+    if (lane == 0) out[v] = (uint32_t)hanf_estimate(alpha, (double)m, s, zeros) - 1u;
+  }
+}
+
+// Any word of a differs from b: *flag = 1.
+__global__ void k_rows_differ(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t n,
+                              unsigned int* __restrict__ flag) {
+  bool d = false;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    d |= a[i] != b[i];
+  if (__ballot(d) && (threadIdx.x & 63) == 0) *flag = 1u;
+}
+
+__global__ void k_flag_zero(unsigned int* f) { *f = 0u; }
+
 // counters in input-id order
 __global__ void k_hanf_rows_external(const uint8_t* __restrict__ in, const uint32_t* __restrict__ rank_of,
                                      uint32_t V, uint64_t m, uint8_t* __restrict__ out) {
@@ -358,6 +400,34 @@ __global__ void k_hanf_rows_external(const uint8_t* __restrict__ in, const uint3
 }  // namespace pe
 
 using namespace pe;
+
+// One HyperANF pull over the adjacency (ptr, idx): nxt[v] = register-wise max of cur[v]
+// and cur[u] over the adjacency of v.  hubs: the adjacency's hub table.
+static void hanf_pull(pe_ctx_s* ctx, const uint64_t* ptr, const uint32_t* idx, uint32_t V, int P, hub_table& hubs,
+                      const uint8_t* cur, uint8_t* nxt, unsigned grid) {
+  hipStream_t s = ctx->stream;
+  const uint64_t m = 1ull << P;
+  switch (P) {
+    case 6:
+      k_hanf_pull_lowdeg<16, 4><<<grid, 256, 0, s>>>(ptr, idx, V, HANF_LOW_DEG, (const uint32_t*)cur, (uint32_t*)nxt);
+      k_hanf_pull_small<4, 8><<<grid, 256, 0, s>>>(ptr, idx, V, HANF_LOW_DEG, (const uint32_t*)cur, (uint32_t*)nxt);
+      break;
+    case 7:
+      k_hanf_pull_lowdeg<32, 4><<<grid, 256, 0, s>>>(ptr, idx, V, HANF_LOW_DEG, (const uint32_t*)cur, (uint32_t*)nxt);
+      k_hanf_pull_small<2, 4><<<grid, 256, 0, s>>>(ptr, idx, V, HANF_LOW_DEG, (const uint32_t*)cur, (uint32_t*)nxt);
+      break;
+#define HANF_W(PP, W, U)                                                                                 \
+  case PP:                                                                                               \
+    k_hanf_pull_words<W, U><<<grid, 256, 0, s>>>(ptr, idx, V, m / 4, (const uint32_t*)cur, (uint32_t*)nxt); \
+    break;
+    HANF_W(8, 1, 8) HANF_W(9, 2, 4) HANF_W(10, 4, 4) HANF_W(11, 8, 2) HANF_W(12, 16, 2) HANF_W(13, 16, 2)
+    HANF_W(14, 16, 2) HANF_W(15, 16, 2) HANF_W(16, 16, 2)
+#undef HANF_W
+  }
+  k_hanf_pull_hubs<<<grid, 256, 0, s>>>(ptr, idx, hubs.hubs.p, hubs.count.p, hubs.start.p, m / 4,
+                                        (const uint32_t*)cur, (uint32_t*)nxt);
+  PE_HIP(hipGetLastError());
+}
 
 extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double* anfs, uint32_t cap,
                            uint32_t* nanfs, double* avg_distance, uint8_t* registers, double* seconds) {
@@ -389,31 +459,7 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
       return t;
     };
     auto pull = [&](const uint8_t* cur, uint8_t* nxt) {
-      switch (P) {
-        case 6:
-          k_hanf_pull_lowdeg<16, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
-                                                              (const uint32_t*)cur, (uint32_t*)nxt);
-          k_hanf_pull_small<4, 8><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
-                                                            (const uint32_t*)cur, (uint32_t*)nxt);
-          break;
-        case 7:
-          k_hanf_pull_lowdeg<32, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
-                                                              (const uint32_t*)cur, (uint32_t*)nxt);
-          k_hanf_pull_small<2, 4><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, HANF_LOW_DEG,
-                                                            (const uint32_t*)cur, (uint32_t*)nxt);
-          break;
-#define HANF_W(PP, W, U)                                                                                      \
-  case PP:                                                                                                    \
-    k_hanf_pull_words<W, U><<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, V, m / 4,                  \
-                                                      (const uint32_t*)cur, (uint32_t*)nxt);                   \
-    break;
-        HANF_W(8, 1, 8) HANF_W(9, 2, 4) HANF_W(10, 4, 4) HANF_W(11, 8, 2) HANF_W(12, 16, 2) HANF_W(13, 16, 2)
-        HANF_W(14, 16, 2) HANF_W(15, 16, 2) HANF_W(16, 16, 2)
-#undef HANF_W
-      }
-      k_hanf_pull_hubs<<<pull_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, hubs.hubs.p, hubs.count.p,
-                                                 hubs.start.p, m / 4, (const uint32_t*)cur, (uint32_t*)nxt);
-      PE_HIP(hipGetLastError());
+      hanf_pull(ctx, g->col_ptr.p, g->row_idx.p, V, P, hubs, cur, nxt, pull_grid);
     };
     PE_HIP(hipEventRecord(ctx->t0, s));
     hub_table_build(ctx, g->col_ptr.p, V, g->nnz, HANF_HUB_DEG, HANF_HUB_CHUNK, hubs);
@@ -454,6 +500,74 @@ extern "C" int pe_hyperanf(pe_graph_t* g, const pe_hyperanf_opts_t* opts, double
       }
       PE_HIP(hipStreamSynchronize(s));
     }
+    if (seconds) *seconds = ms * 1e-3;
+  });
+}
+
+extern "C" int pe_nstep_degrees(pe_graph_t* g, const uint64_t* target_words, uint32_t step, uint32_t bits,
+                                int type, uint32_t* in_degrees, uint32_t* out_degrees, double* seconds) {
+  if (!g) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    const int P = (int)bits;
+    PE_REQUIRE(P >= 6 && P <= 16, PE_ERR_INVALID, "bits must be in [6, 16] (nstepdegrees supported range)");
+    PE_REQUIRE(type >= PE_NSTEP_IN && type <= PE_NSTEP_BOTH, PE_ERR_INVALID, "type must be PE_NSTEP_IN, OUT or BOTH");
+    PE_REQUIRE(!(type & PE_NSTEP_IN) || in_degrees, PE_ERR_INVALID, "in_degrees is required for the in pass");
+    PE_REQUIRE(!(type & PE_NSTEP_OUT) || out_degrees, PE_ERR_INVALID, "out_degrees is required for the out pass");
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const uint32_t V = g->V;
+    const uint64_t m = 1ull << P, bytes = std::max<uint64_t>(4, (uint64_t)V * m);
+    const uint32_t* perm = g->relabeled ? g->perm.p : nullptr;
+    dbuf<uint64_t> tw;
+    if (target_words) {
+      const uint64_t nw = ((uint64_t)V >> 6) + 1;
+      tw.alloc(ctx, nw);
+      PE_HIP(hipMemcpyAsync(tw.p, target_words, nw * 8, hipMemcpyHostToDevice, s));
+    }
+    dbuf<uint8_t> a(ctx, bytes), b(ctx, bytes);
+    dbuf<uint32_t> dev_deg(ctx, std::max<uint32_t>(1, V));
+    dbuf<unsigned int> flag(ctx, 1);
+    const unsigned grid = persistent_grid(ctx, grid_for(V, 4), 8);
+    PE_HIP(hipEventRecord(ctx->t0, s));
+    // one pass per direction: the in pass gathers over in-edges (the reference's
+    // propagate on the forward engine), the out pass over out-edges (on the reversed one)
+    auto pass = [&](const uint64_t* ptr, const uint32_t* idx, uint32_t* host_out) {
+      hub_table hubs;
+      hub_table_build(ctx, ptr, V, g->nnz, HANF_HUB_DEG, HANF_HUB_CHUNK, hubs);
+      k_hanf_zero<<<grid_for(bytes / 4, 256, 65536), 256, 0, s>>>((uint32_t*)a.p, bytes / 4);
+      k_hanf_init<<<grid_for(V, 256, 8192), 256, 0, s>>>(a.p, V, P, perm);
+      PE_HIP(hipGetLastError());
+      uint8_t* cur = a.p;
+      uint8_t* nxt = b.p;
+      for (uint32_t it = 0; it < step; ++it) {  // nstepdegrees.hpp:343-409
+        hanf_pull(ctx, ptr, idx, V, P, hubs, cur, nxt, grid);
+        k_flag_zero<<<1, 1, 0, s>>>(flag.p);
+        k_rows_differ<<<grid_for(bytes / 4, 256, 8192), 256, 0, s>>>((const uint32_t*)cur, (const uint32_t*)nxt,
+                                                                     bytes / 4, flag.p);
+        PE_HIP(hipGetLastError());
+        unsigned int changed = 0;
+        PE_HIP(hipMemcpyAsync(&changed, flag.p, 4, hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        std::swap(cur, nxt);
+        if (!changed) break;  // no book grew: further steps change nothing
+      }
+      k_nstep_degree<<<grid, 256, 0, s>>>(cur, V, P, tw.p, perm, dev_deg.p);
+      PE_HIP(hipGetLastError());
+      download_external(g, dev_deg.p, host_out, 4);
+    };
+    if (type & PE_NSTEP_IN) pass(g->col_ptr.p, g->row_idx.p, in_degrees);
+    if (type & PE_NSTEP_OUT) {
+      if (!g->directed && (type & PE_NSTEP_IN)) {
+        std::memcpy(out_degrees, in_degrees, (size_t)V * 4);  // undirected: out = in (nstepdegrees.hpp:456-463)
+      } else {
+        PE_REQUIRE(g->has_csr || g->csr_is_csc, PE_ERR_STATE, "the out pass needs out-edges (PE_GRAPH_BUILD_CSR)");
+        pass(g->csr_ptr(), g->csr_idx(), out_degrees);
+      }
+    }
+    PE_HIP(hipEventRecord(ctx->t1, s));
+    PE_HIP(hipEventSynchronize(ctx->t1));
+    float ms = 0.f;
+    PE_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
     if (seconds) *seconds = ms * 1e-3;
   });
 }

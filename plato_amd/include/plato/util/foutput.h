@@ -82,4 +82,16 @@ class thread_local_fs_output {
   std::ostream os_;
 };
 
+// fs_mt_omp_output_t (plato/util/foutput.h): the writer the reference's n-step degrees
+// driver saves through; one file `<path>/<prefix>0.csv[.gz]`, written via ostream().
+class fs_mt_omp_output_t {
+ public:
+  fs_mt_omp_output_t(const std::string& path, const std::string& prefix, bool compressed = true)
+      : out_(path, prefix, compressed) {}
+  std::ostream& ostream() { return out_.local(); }
+
+ private:
+  thread_local_fs_output out_;
+};
+
 }  // namespace plato

@@ -16,7 +16,7 @@ def test_host_programs_build():
     r = subprocess.run(["make", "-s", "-f", "plato_amd/host/Makefile", f"ROOT={ROOT}"], cwd=ROOT,
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "cgm_simple", "hyperanf",
+    for p in ("pagerank_simple", "bfs_simple", "personalized_pagerank", "trustrank", "kcore_simple", "cgm_simple", "hyperanf", "nstepdegrees",
               "test_plato_api"):
         assert os.access(os.path.join(BIN, p), os.X_OK)
 
@@ -180,3 +180,22 @@ def test_hyperanf_driver_reference_case():
     assert r.returncode == 0, r.stderr
     got = float(r.stdout.split("avg_distance: ")[1].split(",")[0])
     assert abs(got - avg) <= 1e-9 * abs(avg)
+
+
+@pytest.mark.gpu
+def test_nstepdegrees_driver_matches_oracle(tmp_path):
+    """example/nstepdegrees.cc's command line on raw_graph_7_7.csv and an RMAT edge list:
+    `v,in,out` for every vertex against the oracle."""
+    path = os.path.join(GOLDEN, "raw_graph_7_7.csv")
+    r = subprocess.run([os.path.join(BIN, "nstepdegrees"), "--input", path, "--output", str(tmp_path),
+                        "--is_directed=true", "--step", "1", "--bits", "6", "--type", "both"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(tmp_path / "0000_0.csv", delimiter=",", dtype=np.int64, ndmin=2)
+    s, d = oracle.read_edge_csv(path)
+    g = oracle.CSC(int(max(s.max(), d.max())) + 1, s, d)
+    din, _ = oracle.nstep_degrees(g, step=1, bits=6, direction="in")
+    dout, _ = oracle.nstep_degrees(g, step=1, bits=6, direction="out")
+    assert np.array_equal(got[:, 0], np.arange(g.V))
+    assert np.array_equal(got[:, 1], din) and np.array_equal(got[:, 2], dout)
+    assert (got[0, 1], got[0, 2], got[4, 1], got[4, 2]) == (3, 1, 1, 1)

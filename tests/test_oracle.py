@@ -621,3 +621,58 @@ def test_ppr_pushpull_switch_boundary():
     # epoch 0: A = od(0) = 2, not above 2 -> push; epoch 1: A = 2 + 3 + 0 = 5 -> pull
     assert modes.tolist() == [1, 0, 0, 0]
     assert np.array_equal(modes, _expected_ppr_modes(g, 0, 4))
+
+
+# ---- n-step degrees (nstepdegrees.hpp:166-489) -------------------------------------------------
+
+def test_nstep_degrees_reference_known_answers():
+    """test_nstepdegrees.cc: raw_graph_7_7.csv, directed, step 1, bits 6, type both:
+    in(0) = 3, out(0) = 1, in(4) = 1, out(4) = 1."""
+    s, d = oracle.read_edge_csv(golden("raw_graph_7_7.csv"))
+    V = int(max(s.max(), d.max())) + 1
+    g = oracle.CSC(V, s, d)
+    din, _ = oracle.nstep_degrees(g, step=1, bits=6, direction="in")
+    dout, _ = oracle.nstep_degrees(g, step=1, bits=6, direction="out")
+    assert (din[0], dout[0], din[4], dout[4]) == (3, 1, 1, 1)
+
+
+def _ball_hll_degree(g, v, step, bits, direction):
+    """From scratch: the set of vertices within `step` hops of v (against the edges for
+    in-degrees), each id hashed into a fresh HyperLogLog, estimate - 1."""
+    V = g.V
+    ptr, idx = (g.col_ptr, g.row_idx) if direction == "in" else (g.row_ptr, g.col_idx)
+    ball, frontier = {v}, {v}
+    for _ in range(step):
+        nxt = set()
+        for u in frontier:
+            nxt.update(int(w) for w in idx[int(ptr[u]):int(ptr[u + 1])])
+        nxt -= ball
+        ball |= nxt
+        frontier = nxt
+        if not frontier:
+            break
+    m = 1 << bits
+    regs = np.zeros(m, np.uint8)
+    for u in ball:
+        h = oracle.murmur3_32(int(u).to_bytes(4, "little"), 0x5F61767A)
+        r, w = 1, (h << bits) & 0xFFFFFFFF
+        while r <= 32 - bits and not (w & 0x80000000):
+            r += 1
+            w = (w << 1) & 0xFFFFFFFF
+        regs[h >> (32 - bits)] = max(regs[h >> (32 - bits)], r)
+    return int(oracle.hll_estimate(regs)) - 1
+
+
+@pytest.mark.parametrize("direction", ["in", "out"])
+def test_nstep_degrees_equal_hll_of_balls(direction):
+    s, d = oracle.rmat_edges(9, 4, 61)
+    g = oracle.CSC(1 << 9, s, d)
+    for step in (1, 2, 3):
+        deg, _ = oracle.nstep_degrees(g, step=step, bits=7, direction=direction)
+        for v in (0, 1, 5, 77, 300, 511):
+            assert deg[v] == _ball_hll_degree(g, v, step, 7, direction), (step, v)
+    # unbounded steps: the closure; the loop stops once nothing grows
+    deg, n = oracle.nstep_degrees(g, step=-1, bits=7, direction=direction)
+    assert n < 64
+    for v in (0, 300):
+        assert deg[v] == _ball_hll_degree(g, v, 1 << 20, 7, direction)
