@@ -237,9 +237,9 @@ struct pb_layout {
   uint32_t n1 = 0, n2 = 0, nsplit = 0;
   bool weighted = false;
   dbuf<uint16_t> src_upd;               // [nupdates] source (in its chunk) of each update
-  dbuf<uint16_t> dst_local;             // [nnz] bin order: destination in its bin, bit 15 =
+  dbuf<uint16_t> dst_local;             // [nnz, padded to 512] bin order: destination in its bin, bit 15 =
                                         // first entry of a new update
-  dbuf<uint32_t> gbase;                 // [nnz/8 + 1] updates before each aligned group of 8
+  dbuf<uint32_t> gbase;                 // updates before each aligned gather step of 512 entries
   dbuf<float> w;                        // [nnz] bin order, weighted graphs
   dbuf<float> values;                   // [nupdates] contributions of the current iteration
   dbuf<uint64_t> rec_start;             // [nrecords] chunk-major run records (update space)

@@ -60,9 +60,7 @@ static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the 
 #ifndef PB_SCATTER_RECORDS
 #define PB_SCATTER_RECORDS 8  // run records per wave at a time in the scatter
 #endif
-#ifndef PB_GATHER_UNROLL
-#define PB_GATHER_UNROLL 4    // 8-entry groups per lane at a time in the gather
-#endif
+
 
 // Adds one converted term into a destination's accumulator word.
 __device__ __forceinline__ void acc_add(unsigned long long* acc, uint64_t v) {
@@ -70,8 +68,41 @@ __device__ __forceinline__ void acc_add(unsigned long long* acc, uint64_t v) {
 }
 
 // ---- scatter ----------------------------------------------------------------------------
-// RB run records per wave at a time (records hold at most run_cap entries; each pass
-// covers 256 entries of every record), so each lane keeps 4 * RB loads in flight.
+// A wave takes RB run records at a time (a record holds at most 256 updates, one pass of
+// 4 x 64 lanes).  The chain per record group -- descriptors, then the sources of its
+// updates, then LDS lookups and stores -- is software-pipelined across groups: the
+// descriptors are loaded two groups ahead and the sources one group ahead, so a wave
+// always has the next group's loads in flight while it stores the current one.
+template <int RB>
+struct pb_rec_group {
+  uint64_t s[RB];
+  uint32_t len[RB];
+};
+
+template <int RB>
+__device__ __forceinline__ void pb_load_group(pb_rec_group<RB>& g, uint64_t r, uint64_t r1,
+                                              const uint64_t* __restrict__ rec_start,
+                                              const uint32_t* __restrict__ rec_len) {
+#pragma unroll
+  for (int q = 0; q < RB; ++q) {
+    const bool ok = r + q < r1;
+    g.s[q] = ok ? rec_start[r + q] : 0ull;
+    g.len[q] = ok ? rec_len[r + q] : 0u;
+  }
+}
+
+template <int RB>
+__device__ __forceinline__ void pb_load_sources(const pb_rec_group<RB>& g, const uint16_t* __restrict__ src_upd,
+                                                int lane, uint32_t (&sl)[RB][4]) {
+#pragma unroll
+  for (int q = 0; q < RB; ++q)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t k = j * 64 + lane;
+      sl[q][j] = k < g.len[q] ? (uint32_t)src_upd[g.s[q] + k] : 0u;
+    }
+}
+
 template <int RB>
 __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
     const pb_unit1* __restrict__ units, uint32_t nunits, const uint64_t* __restrict__ rec_start,
@@ -85,10 +116,18 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
   }
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int WAVES = PB_THREADS / 64;
+  constexpr uint64_t RSTEP = (uint64_t)WAVES * RB;
   for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const pb_unit1 u = units[ui];
     const uint64_t base = (uint64_t)u.chunk << src_bits;
     const uint32_t n = (uint32_t)std::min<uint64_t>(1ull << src_bits, (uint64_t)V - base);
+    uint64_t r = u.r0 + (uint64_t)wave * RB;
+    // the first groups' loads go out before the chunk is staged
+    pb_rec_group<RB> cur, nxt;
+    pb_load_group<RB>(cur, r, u.r1, rec_start, rec_len);
+    pb_load_group<RB>(nxt, r + RSTEP, u.r1, rec_start, rec_len);
+    uint32_t sl[RB][4];
+    pb_load_sources<RB>(cur, src_upd, lane, sl);
     if ((n & 3u) == 0) {  // base is a multiple of 16 sources, so x + base is 16-byte aligned
       const float4* x4 = reinterpret_cast<const float4*>(x + base);
       float4* s4 = reinterpret_cast<float4*>(xs);
@@ -97,54 +136,60 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
       for (uint32_t i = tid; i < n; i += PB_THREADS) xs[i] = x[base + i];
     }
     __syncthreads();
-    for (uint64_t r = u.r0 + (uint64_t)wave * RB; r < u.r1; r += (uint64_t)WAVES * RB) {
-      uint64_t s[RB];
-      uint32_t len[RB], maxlen = 0;
+    for (; r < u.r1; r += RSTEP) {
+      pb_rec_group<RB> nn;
+      pb_load_group<RB>(nn, r + 2 * RSTEP, u.r1, rec_start, rec_len);
+      uint32_t sl2[RB][4];
+      pb_load_sources<RB>(nxt, src_upd, lane, sl2);
 #pragma unroll
-      for (int q = 0; q < RB; ++q) {
-        const bool ok = r + q < u.r1;
-        s[q] = ok ? rec_start[r + q] : 0ull;
-        len[q] = ok ? rec_len[r + q] : 0u;
-        maxlen = len[q] > maxlen ? len[q] : maxlen;
-      }
-      for (uint32_t k0 = 0; k0 < maxlen; k0 += 256) {
-        uint32_t sl[RB][4];
+      for (int q = 0; q < RB; ++q)
 #pragma unroll
-        for (int q = 0; q < RB; ++q)
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t k = j * 64 + lane;
+          if (k < cur.len[q]) values[cur.s[q] + k] = xs[sl[q][j]];
+        }
+      cur = nxt;
+      nxt = nn;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const uint32_t k = k0 + j * 64 + lane;
-            sl[q][j] = k < len[q] ? (uint32_t)src_upd[s[q] + k] : 0u;
-          }
+      for (int q = 0; q < RB; ++q)
 #pragma unroll
-        for (int q = 0; q < RB; ++q)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const uint32_t k = k0 + j * 64 + lane;
-            if (k < len[q]) values[s[q] + k] = xs[sl[q][j]];
-          }
-      }
+        for (int j = 0; j < 4; ++j) sl[q][j] = sl2[q][j];
     }
     __syncthreads();
   }
 }
 
 // ---- gather + apply -------------------------------------------------------------------------
-// Contribution of entry i (unaligned head or tail of a unit): its update is the last
-// one started at or before i, counted from the prefix of its aligned group.
-template <bool WEIGHTED>
-__device__ __forceinline__ uint64_t pb_term(const uint16_t* dst_local, const uint32_t* gbase, const float* values,
-                                            const float* w, uint64_t i, int frac) {
-  uint32_t cnt = 0;
-  for (uint64_t j = i & ~7ull; j <= i; ++j) cnt += (uint32_t)(dst_local[j] >> 15);
-  const float x = values[gbase[i >> 3] + cnt - 1];
-  return WEIGHTED ? fixed64_f64((double)w[i] * (double)x, frac) : fixed64_f32(x, frac);
+// A wave step is PB_STEP = 512 aligned entries, 8 per lane (one 16-byte load of
+// destination words); a wave works on GU steps at a time, all their loads in flight
+// together.  An entry's update index is step_base[step] (updates before the step) plus a
+// wave-wide prefix count of the update flags, so no per-group count is read; each entry
+// then loads its value by update index (entries of one update read one address) and adds
+// its term into its destination's LDS word.  Steps cover the unit's range rounded out to
+// whole steps; entries outside the unit count for the prefix but add nothing.
+static constexpr uint32_t PB_STEP = 512;
+static constexpr int PB_WAVES = PB_THREADS / 64;
+#ifndef PB_GATHER_STEPS
+#define PB_GATHER_STEPS 4  // wave steps in flight per wave
+#endif
+#ifndef PB_GATHER_STEPS_W
+#define PB_GATHER_STEPS_W 2  // the same for weighted graphs (two more float4 per lane and step)
+#endif
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x - v;
 }
 
 template <bool WEIGHTED, int GU>
 __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
     const pb_unit2* __restrict__ units, uint32_t nunits, const float* __restrict__ values,
-    const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ gbase,
+    const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ step_base,
     uint32_t V, int dst_bits, int frac,
     float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
     unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum) {
@@ -161,80 +206,71 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
       y = buf0;
     }
   }
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t pd = 1u << dst_bits;
   double delta = 0.0, rsum = 0.0;
   for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const pb_unit2 u = units[ui];
     for (uint32_t i = tid; i < pd; i += PB_THREADS) sacc[i] = 0ull;
     __syncthreads();
-    // Entries in aligned groups of 8 per lane (one 16-byte load of destinations, the
-    // group's update count, then the 8 values by update index: entries of one update
-    // read the same address), one LDS atomic per entry.  Unaligned head and tail
-    // entries go one by one.
-    const uint64_t a0 = (u.e0 + 7) & ~7ull, a1 = u.e1 & ~7ull;
-    const bool grouped = a0 < a1;
-    {
-      const uint64_t h1 = grouped ? a0 : u.e1;
-      for (uint64_t i = u.e0 + tid; i < h1; i += PB_THREADS) {
-        const uint32_t d = dst_local[i] & 0x7FFFu;
-        acc_add(&sacc[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i, frac));
-      }
-      if (grouped)
-        for (uint64_t i = a1 + tid; i < u.e1; i += PB_THREADS) {
-          const uint32_t d = dst_local[i] & 0x7FFFu;
-          acc_add(&sacc[d], pb_term<WEIGHTED>(dst_local, gbase, values, w, i, frac));
-        }
-    }
-    if (grouped) {
-      const uint64_t ng = (a1 - a0) >> 3, gfirst = a0 >> 3;
-      const uint4* dq = reinterpret_cast<const uint4*>(dst_local + a0);
-      const float4* wq = WEIGHTED ? reinterpret_cast<const float4*>(w + a0) : nullptr;
-      for (uint64_t g0 = tid; g0 < ng; g0 += (uint64_t)PB_THREADS * GU) {
-        uint4 dd[GU];
-        uint32_t gb[GU];
-        float4 wa[GU], wb[GU];
+    const uint64_t s0 = u.e0 / PB_STEP, s1 = (u.e1 + PB_STEP - 1) / PB_STEP;
+    for (uint64_t sb = s0 + wave; sb < s1; sb += (uint64_t)PB_WAVES * GU) {
+      uint4 dd[GU];
+      uint32_t base[GU];
+      float4 wa[GU], wb[GU];
 #pragma unroll
-        for (int q = 0; q < GU; ++q) {
-          const uint64_t g = g0 + (uint64_t)q * PB_THREADS;
-          if (g < ng) {
-            dd[q] = dq[g];
-            gb[q] = gbase[gfirst + g];
-            if (WEIGHTED) {
-              wa[q] = wq[2 * g];
-              wb[q] = wq[2 * g + 1];
-            }
-          }
-        }
-        float xv[GU][8];
-#pragma unroll
-        for (int q = 0; q < GU; ++q) {
-          if (g0 + (uint64_t)q * PB_THREADS >= ng) break;
-          const uint32_t raw[8] = {dd[q].x & 0xFFFFu, dd[q].x >> 16, dd[q].y & 0xFFFFu, dd[q].y >> 16,
-                                   dd[q].z & 0xFFFFu, dd[q].z >> 16, dd[q].w & 0xFFFFu, dd[q].w >> 16};
-          uint32_t idx = gb[q] - 1;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            idx += raw[k] >> 15;
-            xv[q][k] = values[idx];
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < GU; ++q) {
-          if (g0 + (uint64_t)q * PB_THREADS >= ng) break;
-          const uint32_t d[8] = {dd[q].x & 0x7FFFu, (dd[q].x >> 16) & 0x7FFFu, dd[q].y & 0x7FFFu,
-                                 (dd[q].y >> 16) & 0x7FFFu, dd[q].z & 0x7FFFu, (dd[q].z >> 16) & 0x7FFFu,
-                                 dd[q].w & 0x7FFFu, (dd[q].w >> 16) & 0x7FFFu};
-          float wv[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+      for (int q = 0; q < GU; ++q) {
+        const uint64_t st = sb + (uint64_t)q * PB_WAVES;
+        if (st < s1) {
+          dd[q] = reinterpret_cast<const uint4*>(dst_local + st * PB_STEP)[lane];
+          base[q] = step_base[st];
           if (WEIGHTED) {
-            wv[0] = wa[q].x; wv[1] = wa[q].y; wv[2] = wa[q].z; wv[3] = wa[q].w;
-            wv[4] = wb[q].x; wv[5] = wb[q].y; wv[6] = wb[q].z; wv[7] = wb[q].w;
+            const float4* wq = reinterpret_cast<const float4*>(w + st * PB_STEP) + 2 * lane;
+            wa[q] = wq[0];
+            wb[q] = wq[1];
           }
+        }
+      }
+      float xv[GU][8];
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
+      for (int q = 0; q < GU; ++q) {
+        if (sb + (uint64_t)q * PB_WAVES >= s1) break;
+        const uint32_t raw[8] = {dd[q].x & 0xFFFFu, dd[q].x >> 16, dd[q].y & 0xFFFFu, dd[q].y >> 16,
+                                 dd[q].z & 0xFFFFu, dd[q].z >> 16, dd[q].w & 0xFFFFu, dd[q].w >> 16};
+        uint32_t heads = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) heads += raw[k] >> 15;
+        uint32_t idx = base[q] + wave_excl_scan(heads, lane) - 1u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          idx += raw[k] >> 15;
+          xv[q][k] = values[idx];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < GU; ++q) {
+        const uint64_t st = sb + (uint64_t)q * PB_WAVES;
+        if (st >= s1) break;
+        const uint64_t e0 = st * PB_STEP + 8u * (uint32_t)lane;
+        float wv[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+        if (WEIGHTED) {
+          wv[0] = wa[q].x; wv[1] = wa[q].y; wv[2] = wa[q].z; wv[3] = wa[q].w;
+          wv[4] = wb[q].x; wv[5] = wb[q].y; wv[6] = wb[q].z; wv[7] = wb[q].w;
+        }
+        const uint32_t d[8] = {dd[q].x & 0x7FFFu, (dd[q].x >> 16) & 0x7FFFu, dd[q].y & 0x7FFFu,
+                               (dd[q].y >> 16) & 0x7FFFu, dd[q].z & 0x7FFFu, (dd[q].z >> 16) & 0x7FFFu,
+                               dd[q].w & 0x7FFFu, (dd[q].w >> 16) & 0x7FFFu};
+        if (e0 >= u.e0 && e0 + 8 <= u.e1) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
             acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)xv[q][k], frac)
                                           : fixed64_f32(xv[q][k], frac));
-          }
+        } else {  // the unit's first or last group
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (e0 + k >= u.e0 && e0 + k < u.e1)
+              acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)xv[q][k], frac)
+                                            : fixed64_f32(xv[q][k], frac));
         }
       }
     }
@@ -414,7 +450,7 @@ __global__ __launch_bounds__(1024) void k_pb_write(const uint64_t* __restrict__ 
       const uint64_t k = keys[e];
       const uint64_t u_ex = ou + bu + (uint64_t)__popcll(mu & below);  // updates before e
       dst_local[e] = (uint16_t)((k & dmask) | (fu ? 0x8000ull : 0ull));
-      if ((e & 7ull) == 0) gbase[e >> 3] = (uint32_t)u_ex;
+      if ((e % PB_STEP) == 0) gbase[e / PB_STEP] = (uint32_t)u_ex;
       if (fu) src_upd[u_ex] = (uint16_t)((k >> db) & smask);
       if (fr) {
         const uint64_t r = orr + br + (uint64_t)__popcll(mr & below);
@@ -458,7 +494,8 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   }
   PE_REQUIRE(o.src_bits >= 4 && o.src_bits <= (uint32_t)PB_MAX_SRC_BITS, PE_ERR_INVALID, "src_bits must be in [4, 15]");
   PE_REQUIRE(o.dst_bits >= 4 && o.dst_bits <= (uint32_t)PB_MAX_DST_BITS, PE_ERR_INVALID, "dst_bits must be in [4, 14]");
-  PE_REQUIRE(o.run_cap >= 64, PE_ERR_INVALID, "run_cap must be at least 64");
+  PE_REQUIRE(o.run_cap >= 64 && o.run_cap <= 256, PE_ERR_INVALID,
+             "run_cap must be in [64, 256] (a scatter pass covers 256 updates of a record)");
   PE_REQUIRE(o.unit_log2 >= 10 && o.unit_log2 <= 30, PE_ERR_INVALID, "unit_log2 must be in [10, 30]");
   if (g->pb && g->pb->src_bits == o.src_bits && g->pb->dst_bits == o.dst_bits && g->pb->run_cap == o.run_cap &&
       g->pb->unit_log2 == o.unit_log2)
@@ -488,14 +525,20 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
       PE_HIP(hipGetLastError());
     }
     if (L->weighted) {
-      L->w.alloc(ctx, std::max<uint64_t>(1, nnz));
+      const uint64_t wpad = (nnz + PB_STEP - 1) / PB_STEP * PB_STEP + PB_STEP;
+      L->w.alloc(ctx, wpad);
+      k_pb_zero<<<grid_for(wpad / 2, 256, 65536), 256, 0, s>>>(reinterpret_cast<unsigned long long*>(L->w.p), wpad / 2);
       if (nnz) PE_HIP(hipMemcpyAsync(L->w.p, g->wn.p, nnz * sizeof(float), hipMemcpyDeviceToDevice, s));
       sort_keys(ctx, keys, &L->w, nnz, end_bit);
     } else {
       sort_keys(ctx, keys, nullptr, nnz, end_bit);
     }
-    L->dst_local.alloc(ctx, std::max<uint64_t>(1, nnz));
-    L->gbase.alloc(ctx, nnz / 8 + 1);
+    // padded to whole gather steps with zero words (no update flag, masked by range)
+    const uint64_t padded = (nnz + PB_STEP - 1) / PB_STEP * PB_STEP + PB_STEP;
+    L->dst_local.alloc(ctx, padded);
+    k_pb_zero<<<grid_for(padded / 4, 256, 65536), 256, 0, s>>>(reinterpret_cast<unsigned long long*>(L->dst_local.p),
+                                                               padded / 4);
+    L->gbase.alloc(ctx, padded / PB_STEP + 1);
     if (nnz) {
       const uint64_t nseg = (nnz + PB_SEG - 1) / PB_SEG;
       PE_REQUIRE(nseg < 0x7FFFFFFFull, PE_ERR_RANGE, "too many entries for the update scan");
@@ -656,11 +699,11 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_GATHER);
     if (weighted)
-      k_pb_gather<true, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
+      k_pb_gather<true, PB_GATHER_STEPS_W><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
                                                   (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                   part_rsum);
     else
-      k_pb_gather<false, PB_GATHER_UNROLL><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
+      k_pb_gather<false, PB_GATHER_STEPS><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
                                                    (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                    part_rsum);
     PE_HIP(hipGetLastError());

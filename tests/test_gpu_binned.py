@@ -79,7 +79,7 @@ def test_binned_layout_info_and_release(s14):
 def test_binned_rejects_bad_options(s14):
     from plato_amd._lib import PlatoHipError
     g, _ = s14
-    for bad in (dict(src_bits=16), dict(dst_bits=15), dict(src_bits=3), dict(run_cap=32),
+    for bad in (dict(src_bits=16), dict(dst_bits=15), dict(src_bits=3), dict(run_cap=32), dict(run_cap=512),
                 dict(unit_log2=31)):
         with pytest.raises(PlatoHipError):
             g.set_pull_layout("binned", **bad)
