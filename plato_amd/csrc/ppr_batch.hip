@@ -125,23 +125,41 @@ __device__ __forceinline__ float ld_row(const float* X, uint32_t u, int lane) {
   return *p;
 }
 
-// Sum over the in-edges [eb, ee) of one column in ascending source order, BU rows in
-// flight per wave: every group of BU edges (the last one masked) costs one round trip
-// for its ids (scalar loads: the range is wave-uniform) and one for its rows.  Lanes
-// whose column needs no sum issue no row loads.
+// Sum over the in-edges [eb, ee) of one column in ascending source order.  The ids of up
+// to 64 edges come in one vector load (lane i holds edge i; each row address is read back
+// with readlane), and the rows go in groups of BU, double-buffered: the next group's rows
+// are in flight while the current group is added, so a wave waits on one row latency per
+// group pair instead of an id load plus a row load per group.  Lanes whose column needs
+// no sum issue no row loads.  The adds stay in edge order, so the sum does not change.
+__device__ __forceinline__ void load_rows(const float* X, uint32_t idv, uint32_t j, uint32_t m, int lane, bool needs,
+                                          float (&r)[BU]) {
+#pragma unroll
+  for (int i = 0; i < BU; ++i) {
+    const uint32_t k = j + (uint32_t)i;  // wave-uniform
+    r[i] = 0.0f;
+    if (k < m) {
+      const uint32_t u = __builtin_amdgcn_readlane(idv, (int)k);
+      if (needs) r[i] = ld_row(X, u, lane);
+    }
+  }
+}
+
 __device__ __forceinline__ void column_sum(const float* X, const uint32_t* __restrict__ row_idx, uint64_t eb,
                                            uint64_t ee, int lane, bool needs, float& s, float& c) {
-  for (uint64_t e = eb; e < ee; e += BU) {
-    const uint32_t n = (uint32_t)std::min<uint64_t>(BU, ee - e);  // wave-uniform
-    uint32_t u[BU];
+  for (uint64_t e = eb; e < ee; e += 64) {
+    const uint32_t m = (uint32_t)std::min<uint64_t>(64, ee - e);  // wave-uniform
+    const uint32_t idv = (uint32_t)lane < m ? ld_edge(row_idx + e + lane) : 0u;
+    float cur[BU];
+    load_rows(X, idv, 0, m, lane, needs, cur);
+    for (uint32_t j = 0; j < m; j += BU) {
+      float nxt[BU];
+      if (j + BU < m) load_rows(X, idv, j + BU, m, lane, needs, nxt);
 #pragma unroll
-    for (int i = 0; i < BU; ++i) u[i] = ld_edge(row_idx + e + ((uint32_t)i < n ? i : 0));
-    float vals[BU];
+      for (int i = 0; i < BU; ++i)
+        if (j + (uint32_t)i < m) kahan_add(s, c, cur[i]);
 #pragma unroll
-    for (int i = 0; i < BU; ++i) vals[i] = (needs && (uint32_t)i < n) ? ld_row(X, u[i], lane) : 0.0f;
-#pragma unroll
-    for (int i = 0; i < BU; ++i)
-      if ((uint32_t)i < n) kahan_add(s, c, vals[i]);
+      for (int i = 0; i < BU; ++i) cur[i] = nxt[i];
+    }
   }
 }
 

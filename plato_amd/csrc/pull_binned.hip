@@ -47,7 +47,13 @@
 
 namespace pe {
 
-static constexpr int PB_THREADS = 1024;
+static constexpr int PB_THREADS = 1024;  // layout build kernels
+#ifndef PB_SCATTER_THREADS
+#define PB_SCATTER_THREADS 1024
+#endif
+#ifndef PB_GATHER_THREADS
+#define PB_GATHER_THREADS 1024
+#endif
 #ifndef PB_MAX_SRC_BITS
 #define PB_MAX_SRC_BITS 15  // 128 KiB of FP32 sources in LDS
 #endif
@@ -103,19 +109,20 @@ __device__ __forceinline__ void pb_load_sources(const pb_rec_group<RB>& g, const
     }
 }
 
-template <int RB>
-__global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
+template <int RB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_pb_scatter(
     const pb_unit1* __restrict__ units, uint32_t nunits, const uint64_t* __restrict__ rec_start,
     const uint32_t* __restrict__ rec_len, const uint16_t* __restrict__ src_upd, uint32_t V, int src_bits,
     const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl, float* __restrict__ values) {
-  __shared__ float xs[1 << PB_MAX_SRC_BITS];
+  extern __shared__ float pb_scatter_lds[];  // the chunk's 2^src_bits sources
+  float* xs = pb_scatter_lds;
   const float* x = buf0;
   if (ctl) {
     if (ctl->done) return;
     if (ctl->cur) x = buf1;
   }
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  constexpr int WAVES = PB_THREADS / 64;
+  constexpr int WAVES = THREADS / 64;
   constexpr uint64_t RSTEP = (uint64_t)WAVES * RB;
   for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const pb_unit1 u = units[ui];
@@ -131,9 +138,9 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
     if ((n & 3u) == 0) {  // base is a multiple of 16 sources, so x + base is 16-byte aligned
       const float4* x4 = reinterpret_cast<const float4*>(x + base);
       float4* s4 = reinterpret_cast<float4*>(xs);
-      for (uint32_t i = tid; i < (n >> 2); i += PB_THREADS) s4[i] = x4[i];
+      for (uint32_t i = tid; i < (n >> 2); i += THREADS) s4[i] = x4[i];
     } else {
-      for (uint32_t i = tid; i < n; i += PB_THREADS) xs[i] = x[base + i];
+      for (uint32_t i = tid; i < n; i += THREADS) xs[i] = x[base + i];
     }
     __syncthreads();
     for (; r < u.r1; r += RSTEP) {
@@ -168,7 +175,6 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_scatter(
 // its term into its destination's LDS word.  Steps cover the unit's range rounded out to
 // whole steps; entries outside the unit count for the prefix but add nothing.
 static constexpr uint32_t PB_STEP = 512;
-static constexpr int PB_WAVES = PB_THREADS / 64;
 #ifndef PB_GATHER_STEPS
 #define PB_GATHER_STEPS 4  // wave steps in flight per wave
 #endif
@@ -186,15 +192,17 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
   return x - v;
 }
 
-template <bool WEIGHTED, int GU>
-__global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
+template <bool WEIGHTED, int GU, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_pb_gather(
     const pb_unit2* __restrict__ units, uint32_t nunits, const float* __restrict__ values,
     const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ step_base,
     uint32_t V, int dst_bits, int frac,
     float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
     unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum) {
-  __shared__ unsigned long long sacc[1 << PB_MAX_DST_BITS];
-  __shared__ double s_red[2 * PB_THREADS / 64];
+  extern __shared__ unsigned long long pb_gather_lds[];  // 2^dst_bits accumulators, then the reduction
+  constexpr int PB_WAVES = THREADS / 64;
+  unsigned long long* sacc = pb_gather_lds;
+  double* s_red = reinterpret_cast<double*>(pb_gather_lds + (1u << dst_bits));
   bool final_iter = a.final_iter;
   const float* x = buf0;
   float* y = buf1;
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
   double delta = 0.0, rsum = 0.0;
   for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const pb_unit2 u = units[ui];
-    for (uint32_t i = tid; i < pd; i += PB_THREADS) sacc[i] = 0ull;
+    for (uint32_t i = tid; i < pd; i += THREADS) sacc[i] = 0ull;
     __syncthreads();
     const uint64_t s0 = u.e0 / PB_STEP, s1 = (u.e1 + PB_STEP - 1) / PB_STEP;
     for (uint64_t sb = s0 + wave; sb < s1; sb += (uint64_t)PB_WAVES * GU) {
@@ -277,19 +285,19 @@ __global__ __launch_bounds__(PB_THREADS) void k_pb_gather(
     __syncthreads();
     const uint64_t v0 = (uint64_t)u.bin << dst_bits;
     if (u.slot == 0xFFFFFFFFu) {
-      for (uint32_t i = tid; i < pd; i += PB_THREADS) {
+      for (uint32_t i = tid; i < pd; i += THREADS) {
         const uint64_t v = v0 + i;
         if (v >= V) break;
         delta += apply_row<float>(a, final_iter, (uint32_t)v, from_fixed64(sacc[i], frac), x, y, od, rsum);
       }
     } else {
       unsigned long long* ga = acc + ((uint64_t)u.slot << dst_bits);
-      for (uint32_t i = tid; i < pd; i += PB_THREADS)
+      for (uint32_t i = tid; i < pd; i += THREADS)
         if (sacc[i]) atomicAdd(&ga[i], sacc[i]);
     }
     __syncthreads();
   }
-  block_reduce2<PB_THREADS>(delta, rsum, s_red);
+  block_reduce2<THREADS>(delta, rsum, s_red);
   if (tid == 0) {
     part_delta[blockIdx.x] = delta;
     part_rsum[blockIdx.x] = rsum;
@@ -672,8 +680,20 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   g->pb = std::move(L);
 }
 
+// Blocks of `threads` that fit one CU with `lds` bytes each (160 KiB of LDS, 2048 threads).
+static unsigned pb_blocks_per_cu(size_t lds, int threads) {
+  const unsigned by_lds = (unsigned)std::max<size_t>(1, (160u << 10) / std::max<size_t>(1, lds));
+  const unsigned by_threads = (unsigned)std::max(1, 2048 / threads);
+  return std::max(1u, std::min(by_lds, by_threads));
+}
+
+static size_t pb_gather_lds_bytes(uint32_t dst_bits) {
+  return ((size_t)8 << dst_bits) + (size_t)16 * (PB_GATHER_THREADS / 64);
+}
+
 unsigned binned_grid_gather(pe_graph_s* g) {
-  return persistent_grid(g->ctx, std::max<uint32_t>(1, g->pb->n2), 1);
+  return persistent_grid(g->ctx, std::max<uint32_t>(1, g->pb->n2),
+                         pb_blocks_per_cu(pb_gather_lds_bytes(g->pb->dst_bits), PB_GATHER_THREADS));
 }
 
 unsigned binned_grid_split(pe_graph_s* g) {
@@ -688,22 +708,24 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
   PE_REQUIRE(a.mass > 0.0, PE_ERR_INTERNAL, "binned iteration without a bound on the contributions");
   const int frac = pb_frac_bits(a.mass);
   hipStream_t s = g->ctx->stream;
-  const unsigned g1 = persistent_grid(g->ctx, std::max<uint32_t>(1, L.n1), 1);
+  const size_t lds1 = (size_t)4 << L.src_bits;
+  const unsigned g1 = persistent_grid(g->ctx, std::max<uint32_t>(1, L.n1), pb_blocks_per_cu(lds1, PB_SCATTER_THREADS));
+  const size_t lds2 = pb_gather_lds_bytes(L.dst_bits);
   const unsigned g2 = binned_grid_gather(g), gs = binned_grid_split(g);
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_SCATTER);
-    k_pb_scatter<PB_SCATTER_RECORDS><<<g1, PB_THREADS, 0, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V,
+    k_pb_scatter<PB_SCATTER_RECORDS, PB_SCATTER_THREADS><<<g1, PB_SCATTER_THREADS, lds1, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V,
                                            (int)L.src_bits, b0, b1, ctl, L.values.p);
     PE_HIP(hipGetLastError());
   }
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_GATHER);
     if (weighted)
-      k_pb_gather<true, PB_GATHER_STEPS_W><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
+      k_pb_gather<true, PB_GATHER_STEPS_W, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
                                                   (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                   part_rsum);
     else
-      k_pb_gather<false, PB_GATHER_STEPS><<<g2, PB_THREADS, 0, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
+      k_pb_gather<false, PB_GATHER_STEPS, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
                                                    (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                    part_rsum);
     PE_HIP(hipGetLastError());
