@@ -34,11 +34,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu_info():
+    """The host the CPU baseline runs on: cores this process may use (its affinity set, the
+    share a GPU box gives one GPU), all cores, CPU model and NUMA nodes."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit()])
+    except OSError:
+        numa = None
+    return {"usable_cores": usable, "cpu_count": os.cpu_count(), "model": model, "numa_nodes": numa}
+
+
 def cpu_baseline(g, seconds_target=10.0, threads=None):
-    """Time the oracle's pull iteration (FP64, OpenMP) on a sample of the graph's columns:
-    16 evenly spaced ranges of V/256 columns each (1/16 of the columns)."""
+    """Time the oracle's pull iteration (FP64, OpenMP over every core this process may use)
+    on a sample of the graph's columns: 16 evenly spaced ranges of V/256 columns each."""
     import oracle
-    threads = threads or min(16, os.cpu_count() or 1)
+    info = host_cpu_info()
+    threads = threads or info["usable_cores"]
     cp, ri = g.download_csc()
     od = g.out_degrees()
     if g.info.relabeled:  # out_degrees() is in input order; the CSC is internal
@@ -61,7 +81,7 @@ def cpu_baseline(g, seconds_target=10.0, threads=None):
         spent += time.perf_counter() - t0
         done_edges += edges
         passes += 1
-    return {"value": done_edges / spent / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+    return {"value": done_edges / spent / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port", "host": info,
             "sample": f"{passes} passes of one PageRank pull iteration (FP64, OpenMP) over 16 evenly "
                       f"spaced ranges of {width} columns ({edges} in-edges) of the same graph, "
                       f"{spent:.1f} s"}
@@ -115,6 +135,42 @@ def measure_pull(ctx, g, layout, steps, warmup, weighted=False):
     log(f"[bench] {'weighted ' if weighted else ''}{layout}: {out['ms_per_step']:.3f} ms/iteration (timed), {out['graph_ms']:.3f} ms "
         f"(graph replay), kernel {out['kernel_ms']:.3f} ms")
     return out
+
+
+def run_c1(ctx, scale=16, iters=20, seed=1):
+    """The CPU-runnable configuration: PageRank, RMAT scale 16, 20 iterations, eps 0, FP64
+    ranks (example/pagerank_simple.cc:67-112 with --is_directed=true --iterations 20
+    --eps 0).  The device's FP64 run against the oracle's full-graph FP64 run on every
+    usable host core, both timed, and their L1 / L-infinity difference."""
+    import oracle  # CPU baseline and checker
+    from plato_amd import Graph
+    from plato_amd.algo import pagerank, pagerank_opts_t
+    g = Graph.rmat(ctx, scale, 16, seed=seed)
+    E = g.nnz
+    pagerank(g, pagerank_opts_t(iters, 0.85, 0.0), precision=64)  # warm-up
+    best = None
+    for _ in range(5):
+        r = pagerank(g, pagerank_opts_t(iters, 0.85, 0.0), precision=64)
+        best = r if best is None or r.seconds < best.seconds else best
+    g.close()
+    s, d = oracle.rmat_edges(scale, 16, seed)
+    og = oracle.CSC(1 << scale, s, d, csr=False)
+    info = host_cpu_info()
+    oracle.pagerank(og, iters, 0.85, 0.0, nthreads=info["usable_cores"])  # warm-up
+    t_cpu, reps = 0.0, 0
+    while t_cpu < 3.0 and reps < 50:
+        t0 = time.perf_counter()
+        ref, _ = oracle.pagerank(og, iters, 0.85, 0.0, nthreads=info["usable_cores"])
+        t_cpu += time.perf_counter() - t0
+        reps += 1
+    diff = np.abs(best.rank - ref)
+    gpu_ms = best.seconds * 1e3 / iters
+    cpu_ms = t_cpu * 1e3 / reps / iters
+    return {"workload": f"pagerank_rmat{scale}_fp64_{iters}_iterations", "E": int(E),
+            "gpu_ms_per_iteration": round(gpu_ms, 4), "gpu_gteps": round(E / gpu_ms / 1e6, 2),
+            "cpu_ms_per_iteration": round(cpu_ms, 3), "cpu_gteps": round(E / cpu_ms / 1e6, 3),
+            "cpu_cores": info["usable_cores"], "cpu_kind": "port", "cpu_runs": reps,
+            "l1_vs_oracle": float(diff.sum()), "linf_rel_vs_oracle": float(np.max(diff / np.abs(ref)))}
 
 
 def run_weighted(ctx, args, layouts):
@@ -539,6 +595,8 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g)
+        log("[bench] C1: PageRank RMAT-16 FP64, 20 iterations, device and oracle")
+        out["c1_pagerank_s16_fp64"] = run_c1(ctx)
     g.close()
     if not args.no_weighted:
         out["weighted"] = run_weighted(ctx, args, layouts)

@@ -414,7 +414,12 @@ int pe_nstep_degrees(pe_graph_t* g, const uint64_t* target_words, uint32_t step,
 /* A new graph with every edge of the directed graph g reversed (same input ids), for
  * pulls over out-edges such as inverse PageRank (the reference runs pagerank over its
  * bcsr_t out-edge structure, trustrank.hpp:101-103).  flags: PE_GRAPH_BUILD_CSR,
- * PE_GRAPH_RELABEL_DEGREE. */
+ * PE_GRAPH_RELABEL_DEGREE, PE_GRAPH_SAME_IDS.
+ * PE_GRAPH_SAME_IDS keeps g's internal ids (and its degree relabeling, if any): device
+ * vectors, bitmaps and BFS states of g are then valid for the reverse, which is how the
+ * engine's reverse() walks a directed graph's edges backwards (dualmode.hpp:151-153,
+ * 299-322).  It excludes PE_GRAPH_RELABEL_DEGREE. */
+#define PE_GRAPH_SAME_IDS 32u
 int pe_graph_create_reverse(pe_graph_t* g, uint32_t flags, pe_graph_t** out);
 
 typedef struct {
@@ -531,6 +536,13 @@ int pe_bfs_reset(pe_bfs_t* b, uint32_t root, pe_bitmap_t* frontier);
 int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, uint64_t* activated,
                 uint64_t* next_active_edges);
 int pe_bfs_get_levels(pe_bfs_t* b, int32_t* levels);
+/* Later steps of b walk the edges of g, a graph over the same vertex ids as the one b
+ * was created on (the graph itself, or its reverse from pe_graph_create_reverse with
+ * PE_GRAPH_SAME_IDS; PE_ERR_INVALID otherwise).  Levels and the visited set carry over.
+ * The engine binds its search to the reversed edges after reverse() on a directed
+ * graph, as the reference's push and pull then walk in_edges_ and out_edges_
+ * (dualmode.hpp:299-322). */
+int pe_bfs_bind(pe_bfs_t* b, pe_graph_t* g);
 
 /* ===========================================================================
  * Autotuner (no reference counterpart: the reference has no tunables on this

@@ -20,6 +20,7 @@ PE_KT_PULL_BIN_SCATTER, PE_KT_PULL_BIN_GATHER = 5, 6
 PE_PULL_MERGE_PATH, PE_PULL_BINNED = 0, 1
 PE_GRAPH_SYMMETRIZE, PE_GRAPH_BUILD_CSR, PE_GRAPH_WEIGHTED, PE_GRAPH_RELABEL_DEGREE = 1, 2, 4, 8
 PE_GRAPH_ENCODE_IDS = 16
+PE_GRAPH_SAME_IDS = 32
 PE_SPREAD_COUNT, PE_SPREAD_ADD = 0, 1
 PE_NSTEP_IN, PE_NSTEP_OUT, PE_NSTEP_BOTH = 1, 2, 3
 
@@ -183,6 +184,7 @@ SIGNATURES = {
     "pe_bfs_reset": [_vp, _u32, _vp],
     "pe_bfs_step": [_vp, _i32, _vp, _vp, _P(_u64), _P(_u64)],
     "pe_bfs_get_levels": [_vp, _vp],
+    "pe_bfs_bind": [_vp, _vp],
     "pe_autotune": [_vp, _u32, _P(TuneResult)],
     "pe_autotune_save": [_vp, _P(TuneResult), C.c_char_p],
     "pe_autotune_load": [_vp, C.c_char_p, _P(TuneResult), _P(_i32)],

@@ -532,14 +532,9 @@ struct bfs_state {
   dbuf<uint64_t> pref;  // scan of the queued vertices' out-degrees (edge-balanced push)
   dbuf<uint64_t> bsum;  // its per-block sums
 
-  explicit bfs_state(pe_graph_s* graph) : g(graph) {
+  explicit bfs_state(pe_graph_s* graph) : g(nullptr) {
+    bind(graph);
     pe_ctx_s* ctx = g->ctx;
-    if (!g->first_in.p) {  // kept with the graph for later searches
-      g->first_in.alloc(ctx, std::max<uint32_t>(1, g->V));
-      k_first_in<<<grid_for(g->V, 256, 8192), 256, 0, ctx->stream>>>(g->col_ptr.p, g->row_idx.p, g->V,
-                                                                      g->first_in.p);
-      PE_HIP(hipGetLastError());
-    }
     const uint32_t V = std::max<uint32_t>(1, g->V);
     nwords = ((uint64_t)g->V >> 6) + 1;
     level.alloc(ctx, V);
@@ -554,6 +549,17 @@ struct bfs_state {
     red.alloc(ctx, 2);
     pref.alloc(ctx, (uint64_t)V + 1);
     bsum.alloc(ctx, ((uint64_t)V + DEG_BLOCK - 1) / DEG_BLOCK);
+  }
+
+  // Walk the edges of `graph` from now on (a graph over the same vertex ids).
+  void bind(pe_graph_s* graph) {
+    g = graph;
+    if (!g->first_in.p) {  // kept with the graph for later searches
+      g->first_in.alloc(g->ctx, std::max<uint32_t>(1, g->V));
+      k_first_in<<<grid_for(g->V, 256, 8192), 256, 0, g->ctx->stream>>>(g->col_ptr.p, g->row_idx.p, g->V,
+                                                                         g->first_in.p);
+      PE_HIP(hipGetLastError());
+    }
   }
 
   // One top-down level from the queue qc (n vertices, A active edges; A = 0 if unknown)
@@ -752,6 +758,18 @@ int pe_bfs_step(pe_bfs_t* b, int push, pe_bitmap_t* current, pe_bitmap_t* next, 
 int pe_bfs_get_levels(pe_bfs_t* b, int32_t* levels) {
   if (!b || !levels) return PE_ERR_INVALID;
   return guarded(b->g->ctx, [&] { download_external(b->g, b->st->level.p, levels, 4); });
+}
+
+int pe_bfs_bind(pe_bfs_t* b, pe_graph_t* g) {
+  if (!b || !g) return PE_ERR_INVALID;
+  return guarded(b->g->ctx, [&] {
+    PE_REQUIRE(g->ctx == b->g->ctx, PE_ERR_INVALID, "graph of another context");
+    PE_REQUIRE(g->id_space == b->g->id_space && g->V == b->g->V, PE_ERR_INVALID,
+               "graph over other vertex ids (bind g or its reverse made with PE_GRAPH_SAME_IDS)");
+    b->g = g;
+    b->st->bind(g);
+    PE_HIP(hipStreamSynchronize(g->ctx->stream));
+  });
 }
 
 int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint8_t* modes,

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -289,6 +290,15 @@ struct pe_graph_s {
   pe::dbuf<uint32_t> fix_t0;     // [F] first tile holding edges of the row
   pe::dbuf<uint32_t> fix_t1;     // [F] tile holding the row's end
   uint64_t num_fix_long = 0;     // fix rows [num_fix - num_fix_long, num_fix) span > 8 tiles
+
+  // Graphs with one id_space number number their vertices alike (same V, same internal
+  // ids): a graph and its reverse built with PE_GRAPH_SAME_IDS.  Device vectors, bitmaps
+  // and search states of one are valid for the other.
+  uint64_t id_space = next_id_space();
+  static uint64_t next_id_space() {
+    static std::atomic<uint64_t> n{1};
+    return n.fetch_add(1);
+  }
 
   // degree relabeling: internal id = rank_of[input id], input id = perm[internal id]
   bool relabeled = false;

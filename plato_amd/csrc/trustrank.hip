@@ -115,6 +115,9 @@ int pe_graph_create_reverse(pe_graph_t* g, uint32_t flags, pe_graph_t** out) {
   *out = nullptr;
   return guarded(g->ctx, [&] {
     PE_REQUIRE(g->directed, PE_ERR_STATE, "a symmetrized graph is its own reverse");
+    const bool same_ids = flags & PE_GRAPH_SAME_IDS;
+    PE_REQUIRE(!(same_ids && (flags & PE_GRAPH_RELABEL_DEGREE)), PE_ERR_INVALID,
+               "PE_GRAPH_SAME_IDS keeps g's ids: it excludes PE_GRAPH_RELABEL_DEGREE");
     pe_ctx_s* ctx = g->ctx;
     hipStream_t s = ctx->stream;
     auto* r = new pe_graph_s();
@@ -125,11 +128,28 @@ int pe_graph_create_reverse(pe_graph_t* g, uint32_t flags, pe_graph_t** out) {
     dbuf<uint32_t> ds(ctx, std::max<uint64_t>(1, g->nnz)), dd(ctx, std::max<uint64_t>(1, g->nnz));
     dbuf<float> dw;
     if (g->nnz) {
-      k_reverse_edges<<<grid_for(g->V, 256, 1u << 20), 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->V,
-                                                                    g->relabeled ? g->perm.p : nullptr, ds.p, dd.p);
+      // same ids: the edges stay in g's internal ids; else they go back to input ids
+      k_reverse_edges<<<grid_for(g->V, 256, 1u << 20), 256, 0, s>>>(
+          g->col_ptr.p, g->row_idx.p, g->V, (g->relabeled && !same_ids) ? g->perm.p : nullptr, ds.p, dd.p);
       PE_HIP(hipGetLastError());
     }
     build_from_edges(r, ds, dd, dw, flags & (PE_GRAPH_BUILD_CSR | PE_GRAPH_RELABEL_DEGREE));
+    if (same_ids) {
+      r->id_space = g->id_space;
+      if (g->relabeled) {
+        r->relabeled = true;
+        r->perm.alloc(ctx, g->V);
+        r->rank_of.alloc(ctx, g->V);
+        PE_HIP(hipMemcpyAsync(r->perm.p, g->perm.p, (size_t)g->V * 4, hipMemcpyDeviceToDevice, s));
+        PE_HIP(hipMemcpyAsync(r->rank_of.p, g->rank_of.p, (size_t)g->V * 4, hipMemcpyDeviceToDevice, s));
+      }
+    }
+    if (g->encoded) {
+      r->encoded = true;
+      r->vid_decode.alloc(ctx, g->V);
+      PE_HIP(hipMemcpyAsync(r->vid_decode.p, g->vid_decode.p, (size_t)g->V * 8, hipMemcpyDeviceToDevice, s));
+    }
+    PE_HIP(hipStreamSynchronize(s));
     *out = guard.release();
   });
 }

@@ -163,9 +163,12 @@ class Graph:
         with open(path, "rb") as f:
             return cls.from_csv_bytes(ctx, f.read(), **kw)
 
-    def reverse(self, build_csr=False, relabel=False):
-        """A new graph with every edge reversed (pe_graph_create_reverse), same input ids."""
-        flags = (L.PE_GRAPH_BUILD_CSR if build_csr else 0) | (L.PE_GRAPH_RELABEL_DEGREE if relabel else 0)
+    def reverse(self, build_csr=False, relabel=False, same_ids=False):
+        """A new graph with every edge reversed (pe_graph_create_reverse), same input ids.
+        same_ids keeps this graph's internal ids too, so its device states, bitmaps and
+        BFS states stay valid for the reverse (excludes relabel)."""
+        flags = (L.PE_GRAPH_BUILD_CSR if build_csr else 0) | (L.PE_GRAPH_RELABEL_DEGREE if relabel else 0) | \
+            (L.PE_GRAPH_SAME_IDS if same_ids else 0)
         h = C.c_void_p()
         check(self.lib.pe_graph_create_reverse(self.h, flags, C.byref(h)), self.ctx.h)
         return Graph(self.ctx, h)
@@ -425,6 +428,32 @@ class dualmode_engine_t:
         self.ctx = graph.ctx
         self.lib = graph.lib
         self.push_threshold_ = push_threshold  # dualmode_engine_opts_t (dualmode.hpp:41-43)
+        self.reversed_ = False
+        self._rev = None  # the directed graph's reverse over the same ids, built on first use
+
+    def reverse(self):
+        """Walk the edges backwards from now on (dualmode.hpp:151-153): pulls sum over
+        out-edges and pushes follow in-edges (dualmode.hpp:299-322).  The push/pull choice
+        keeps the graph's own out-degrees, as the reference's does (dualmode.hpp:250-278).
+        A symmetrized graph is its own reverse; a directed one is reversed on the device
+        once, keeping its ids, on the first reversed call."""
+        self.reversed_ = not self.reversed_
+
+    def is_reversed(self):
+        return self.reversed_
+
+    def walk_graph(self) -> Graph:
+        """The graph whose in-edges a pull and whose out-edges a push walk."""
+        if not self.reversed_ or not self.graph.info.is_directed:
+            return self.graph
+        if self._rev is None:
+            self._rev = self.graph.reverse(build_csr=True, same_ids=True)
+        return self._rev
+
+    def close(self):
+        if self._rev is not None:
+            self._rev.close()
+            self._rev = None
 
     def alloc_v_state(self, dtype=np.float32) -> DenseState:
         return DenseState(self.ctx, self.graph.V, dtype)
@@ -444,7 +473,7 @@ class dualmode_engine_t:
         return bool(p.value), a.value
 
     def pull_sum(self, x: DenseState, y: DenseState, weighted=False):
-        check(self.lib.pe_pull_sum(self.graph.h, x.h, y.h, int(weighted)), self.ctx.h)
+        check(self.lib.pe_pull_sum(self.walk_graph().h, x.h, y.h, int(weighted)), self.ctx.h)
 
     @staticmethod
     def _params(damping, base, seed_mass, seeds, divide_by_od, final_iter):
@@ -458,16 +487,29 @@ class dualmode_engine_t:
                    final_iter=False):
         p = self._params(damping, base, seed_mass, seeds, divide_by_od, final_iter)
         d = C.c_double()
-        check(self.lib.pe_rank_apply(self.graph.h, x.h, y.h, C.byref(p), C.byref(d)), self.ctx.h)
+        check(self.lib.pe_rank_apply(self.walk_graph().h, x.h, y.h, C.byref(p), C.byref(d)), self.ctx.h)
         return d.value
 
     def rank_iterate(self, x, y, damping, base, seed_mass=0.0, seeds=None, divide_by_od=True,
                      final_iter=False, weighted=False):
         p = self._params(damping, base, seed_mass, seeds, divide_by_od, final_iter)
         d = C.c_double()
-        check(self.lib.pe_rank_iterate(self.graph.h, x.h, y.h, int(weighted), C.byref(p),
+        check(self.lib.pe_rank_iterate(self.walk_graph().h, x.h, y.h, int(weighted), C.byref(p),
                                        C.byref(d)), self.ctx.h)
         return d.value
+
+
+    def bfs_step(self, state: "BfsEngineState", push: bool, current: Bitmap, nxt: Bitmap):
+        """One BFS epoch over the edges this engine walks (reversed after reverse())."""
+        state.bind(self.walk_graph())
+        return state.step(push, current, nxt)
+
+    def foreach_edges_bfs(self, state: "BfsEngineState", current: Bitmap, nxt: Bitmap):
+        """The 5-argument foreach_edges with the BFS operators (bfs.hpp:95-137): push or
+        pull by the active-edge rule, then one epoch.  Returns (push, activated)."""
+        push, _ = self.choose_push(current)
+        activated, _ = self.bfs_step(state, push, current, nxt)
+        return push, activated
 
 
 class BfsEngineState:
@@ -495,6 +537,11 @@ class BfsEngineState:
 
     def reset(self, root, frontier: Bitmap):
         check(self.lib.pe_bfs_reset(self.h, int(root), frontier.h), self.ctx.h)
+
+    def bind(self, graph: Graph):
+        """Step over `graph`'s edges from now on: this search's graph or its reverse made
+        with same_ids (pe_bfs_bind)."""
+        check(self.lib.pe_bfs_bind(self.h, graph.h), self.ctx.h)
 
     def step(self, push: bool, current: Bitmap, nxt: Bitmap):
         a, n = C.c_uint64(), C.c_uint64()

@@ -10,6 +10,11 @@
 //   push  bfs_push_signal/slot      bfs.hpp:98-118                  -> pe_bfs_step(push)
 // The 5-argument form keeps the reference's rule: A = sum of out-degrees over the
 // active set, push iff (double)A / E' < push_threshold_ (dualmode.hpp:261-279).
+// After reverse() pulls sum over out-edges and pushes follow in-edges
+// (dualmode.hpp:299-322): a directed graph's reverse is built on the device once, over
+// the same ids (pe_graph_create_reverse with PE_GRAPH_SAME_IDS), and every operator
+// then walks it; the push/pull choice keeps the graph's own out-degrees, as the
+// reference's does (dualmode.hpp:250-278).
 // Any other functor type fails to compile with a message: there is no host
 // fallback.
 #pragma once
@@ -46,6 +51,7 @@ struct bfs_state {
   pe_bfs_t* h = nullptr;
   bitmap_t<>* next = nullptr;
   eid_t next_active_edges = 0;
+  pe_graph_t* walked = nullptr;  // graph the search last stepped over
 };
 struct bfs_push_signal { bfs_state* s; };
 struct bfs_push_slot { bfs_state* s; };
@@ -110,7 +116,6 @@ class dualmode_engine_t {
   // pull only
   template <typename MSG, typename R, typename SIGNAL, typename SLOT>
   R foreach_edges(SIGNAL&& signal, SLOT&& slot) {
-    if (reversed_ && graph_info_.is_directed_) fatal("foreach_edges", PE_ERR_STATE, "reversed pull needs a symmetric graph");
     return pull_dispatch<R>(signal, slot);
   }
 
@@ -118,6 +123,20 @@ class dualmode_engine_t {
   std::shared_ptr<OUTGOING> out_edges() { return out_edges_; }
   void reverse() { reversed_ = !reversed_; }
   bool is_reversed() { return reversed_; }
+
+  // the graph whose in-edges a pull and whose out-edges a push walk
+  pe_graph_t* walk_graph() {
+    if (!reversed_) return graph();
+    pe_graph_info_t gi = device_graph_handle_info();
+    if (!gi.is_directed) return graph();  // a symmetrized graph is its own reverse
+    if (!rev_) {
+      pe_graph_t* r = nullptr;
+      check_status(pe_graph_create_reverse(graph(), PE_GRAPH_BUILD_CSR | PE_GRAPH_SAME_IDS, &r),
+                   "pe_graph_create_reverse");
+      rev_.reset(r);
+    }
+    return rev_.get();
+  }
 
   // decision of the last 5-argument call (the reference logs it under __DUALMODE_DEBUG__)
   bool last_was_push() const { return last_was_push_; }
@@ -142,7 +161,7 @@ class dualmode_engine_t {
 
   template <typename R, typename T>
   R pull_dispatch(ops::gather_sum<T>& sig, ops::write_add<T>& slot) {
-    check_status(pe_pull_sum(graph(), sig.src.handle(), slot.dst.handle(), sig.weighted ? 1 : 0), "pe_pull_sum");
+    check_status(pe_pull_sum(walk_graph(), sig.src.handle(), slot.dst.handle(), sig.weighted ? 1 : 0), "pe_pull_sum");
     slot.dst.invalidate();
     return R();  // the PageRank slot returns 0 for every message
   }
@@ -168,6 +187,11 @@ class dualmode_engine_t {
   template <typename R>
   R bfs_step(ops::bfs_state* s, int push, v_subset_t* current) {
     if (!current) fatal("foreach_edges", PE_ERR_STATE, "BFS pull needs the active set of the 5-argument call");
+    pe_graph_t* wg = walk_graph();
+    if (s->walked != wg) {
+      check_status(pe_bfs_bind(s->h, wg), "pe_bfs_bind");
+      s->walked = wg;
+    }
     uint64_t activated = 0, next_a = 0;
     check_status(pe_bfs_step(s->h, push, current->handle(), s->next->handle(), &activated, &next_a), "pe_bfs_step");
     s->next_active_edges = next_a;
@@ -175,9 +199,14 @@ class dualmode_engine_t {
   }
 
  private:
+  struct graph_deleter {
+    void operator()(pe_graph_t* g) const { pe_graph_destroy(g); }
+  };
+
   std::shared_ptr<INCOMING> in_edges_;
   std::shared_ptr<OUTGOING> out_edges_;
   bool reversed_ = false;
+  std::unique_ptr<pe_graph_t, graph_deleter> rev_;  // reverse of a directed graph, same ids
   graph_info_t graph_info_;
   dualmode_engine_opts_t opts_;
   v_subset_t* current_frontier_ = nullptr;

@@ -319,6 +319,58 @@ static void Reference_PagerankSimpleCallSites() {
   (void)cluster_info;
 }
 
+// dualmode.hpp:151-153, 299-322: after reverse() the pull sums over out-edges and the
+// push follows in-edges.  Chain 0->1->...->9: every vertex but 9 has one out-edge, and a
+// search from 9 over the reversed edges reaches v at level 9 - v, in push mode and in
+// pull mode; reversing again restores the forward edges.
+static void Engine_Reverse() {
+  using in_t = plato::dcsc_t<plato::empty_t>;
+  using out_t = plato::bcsr_t<plato::empty_t>;
+  plato::graph_info_t graph_info(true);
+  auto g = plato::create_dualmode_seq_from_path<plato::empty_t>(&graph_info, dir + "/graph_10_9.csv");
+  for (double threshold : {1e9, 0.0}) {  // every epoch push, then every epoch pull
+    plato::dualmode_engine_opts_t eo;
+    eo.push_threshold_ = threshold;
+    plato::dualmode_engine_t<in_t, out_t> engine(std::shared_ptr<in_t>(&g.second, [](in_t*) {}),
+                                                 std::shared_ptr<out_t>(&g.first, [](out_t*) {}), graph_info, eo);
+    engine.reverse();
+    EXPECT(engine.is_reversed());
+    auto ones = engine.alloc_v_state<float>();
+    auto values = engine.alloc_v_state<float>();
+    ones.fill(1.0f);
+    engine.foreach_edges<float, int>(plato::ops::gather_sum<float>{ones}, plato::ops::write_add<float>{values});
+    for (plato::vid_t v = 0; v < 9; ++v) EXPECT(values[v] == 1.0f);
+    EXPECT(values[9] == 0.0f);
+
+    plato::ops::bfs_state st;
+    plato::check_status(pe_bfs_create(g.second.handle(), &st.h), "pe_bfs_create");
+    auto current = engine.alloc_v_subset();
+    auto next = engine.alloc_v_subset();
+    plato::check_status(pe_bfs_reset(st.h, 9, current.handle()), "pe_bfs_reset");
+    plato::vid_t visited = 1, actives = 1;
+    while (actives) {
+      st.next = &next;
+      actives = engine.foreach_edges<plato::vid_t, plato::vid_t>(
+          plato::ops::bfs_push_signal{&st}, plato::ops::bfs_push_slot{&st}, plato::ops::bfs_pull_signal{&st},
+          plato::ops::bfs_pull_slot{&st}, current);
+      EXPECT(engine.last_was_push() == (threshold > 1.0));
+      visited += actives;
+      std::swap(current, next);
+    }
+    EXPECT(visited == 10);
+    std::vector<int32_t> lv(10, -2);
+    plato::check_status(pe_bfs_get_levels(st.h, lv.data()), "pe_bfs_get_levels");
+    for (int v = 0; v < 10; ++v) EXPECT(lv[v] == 9 - v);
+    pe_bfs_destroy(st.h);
+
+    engine.reverse();
+    EXPECT(!engine.is_reversed());
+    engine.foreach_edges<float, int>(plato::ops::gather_sum<float>{ones}, plato::ops::write_add<float>{values});
+    EXPECT(values[0] == 0.0f);
+    for (plato::vid_t v = 1; v < 10; ++v) EXPECT(values[v] == 1.0f);
+  }
+}
+
 int main(int argc, char** argv) {
   dir = argc > 1 ? argv[1] : "tests/golden";
   struct { const char* name; void (*fn)(); } cases[] = {
@@ -332,6 +384,7 @@ int main(int argc, char** argv) {
       {"CGM.CalcComponent", CGM_CalcComponent},
       {"MessagePassing.SpreadMessage", MessagePassing_SpreadMessage},
       {"Reference.PagerankSimpleCallSites", Reference_PagerankSimpleCallSites},
+      {"Engine.Reverse", Engine_Reverse},
   };
   for (auto& c : cases) {
     const int before = failures;

@@ -39,7 +39,7 @@ def test_cpp_api_known_answers():
                        text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("  OK  ") == 10
+    assert r.stdout.count("  OK  ") == 11
 
 
 @pytest.mark.gpu
