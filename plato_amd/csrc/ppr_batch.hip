@@ -117,46 +117,102 @@ __device__ __forceinline__ void st_row(float* p, float v) {
 #endif
 }
 
-__device__ __forceinline__ float ld_row(const float* X, uint32_t u, int lane) {
-  const float* p = X + (uint64_t)u * BK + lane;
+// entry c of row u of a block with `stride` floats per row (the wide block: 64, packed: 8)
+__device__ __forceinline__ float ld_row(const float* X, uint32_t u, uint32_t stride, uint32_t c) {
+  const float* p = X + (uint64_t)u * stride + c;
 #if PE_BATCH_COLD_ROW
   if (u >= (uint32_t)PE_BATCH_COLD_ROW) return __builtin_nontemporal_load(p);
 #endif
   return *p;
 }
 
-// Sum over the in-edges [eb, ee) of one column in ascending source order.  The ids of up
-// to 64 edges come in one vector load (lane i holds edge i; each row address is read back
-// with readlane), and the rows go in groups of BU, double-buffered: the next group's rows
-// are in flight while the current group is added, so a wave waits on one row latency per
-// group pair instead of an id load plus a row load per group.  Lanes whose column needs
-// no sum issue no row loads.  The adds stay in edge order, so the sum does not change.
-__device__ __forceinline__ void load_rows(const float* X, uint32_t idv, uint32_t j, uint32_t m, int lane, bool needs,
-                                          float (&r)[BU]) {
+// A column's sum (or a long column's chunk sum) runs as BS interleaved Kahan sums: edge k
+// of the column (counted from its first edge, or the chunk's) goes to sub-sum k % BS, and
+// the sub-sums are combined in order 0..BS-1 with TwoSum.  Every pass kernel below, however
+// many edges it puts in one row load, keeps exactly this arithmetic, so a column's bits do
+// not depend on how many other columns are in the pass.
+static constexpr int BS = 4;
+
+// Edges per row load P: lane group p = lane / (64 / P) takes edge t * P + p of step t, so a
+// wave-wide load covers P edges for up to 64 / P columns.  P = 1 is lane j = column j.
+// Each lane keeps the BS / P sub-sums of its edge residue: sub-sum a * P + p in slot a.
+template <int P>
+struct col_sum {
+  static constexpr int NA = BS / P;
+  float s[NA], c[NA];
+  __device__ __forceinline__ void zero() {
 #pragma unroll
-  for (int i = 0; i < BU; ++i) {
-    const uint32_t k = j + (uint32_t)i;  // wave-uniform
+    for (int a = 0; a < NA; ++a) s[a] = c[a] = 0.f;
+  }
+  // the combined sum (s - c) on the lanes of group 0
+  __device__ __forceinline__ void combine(float& S, float& C) const {
+    constexpr int W = BK / P;
+    S = 0.f;
+    C = 0.f;
+#pragma unroll
+    for (int k = 0; k < BS; ++k) {
+      float vs = s[k / P], vc = c[k / P];
+      if (P > 1 && (k % P) != 0) {
+        vs = __shfl_down(vs, (k % P) * W, 64);
+        vc = __shfl_down(vc, (k % P) * W, 64);
+      }
+      pair_add(S, C, vs, vc);
+    }
+  }
+};
+
+// Rows of steps [t0, t0 + NB) into r (0 where there is no edge or no sum is needed).
+// The ids of up to 64 edges sit one per lane in idv: P = 1 reads them back with readlane,
+// wider steps with a lane shuffle.
+template <int P, int NB>
+__device__ __forceinline__ void load_rows(const float* X, uint32_t stride, uint32_t idv, uint32_t t0, uint32_t m, int p,
+                                          uint32_t col, bool needs, float (&r)[NB]) {
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const uint32_t t = t0 + (uint32_t)i;  // wave-uniform
     r[i] = 0.0f;
-    if (k < m) {
-      const uint32_t u = __builtin_amdgcn_readlane(idv, (int)k);
-      if (needs) r[i] = ld_row(X, u, lane);
+    if (t * P < m) {
+      const uint32_t k = t * P + (uint32_t)p;
+      uint32_t u;
+      if (P == 1) u = __builtin_amdgcn_readlane(idv, (int)t);
+      else u = (uint32_t)__shfl((int)idv, (int)(k & 63u), 64);
+      if (needs && k < m) r[i] = ld_row(X, u, stride, col);
     }
   }
 }
 
-__device__ __forceinline__ void column_sum(const float* X, const uint32_t* __restrict__ row_idx, uint64_t eb,
-                                           uint64_t ee, int lane, bool needs, float& s, float& c) {
+// The rows of BU steps starting at step t (a multiple of BU) into their sub-sums: step
+// t + i carries sub-sums (t + i) % (BS / P) of the lane's residue, i.e. slot (OFF + i) % NA
+// with OFF = t % NA (0, or BU when a lane keeps more sub-sums than a group has steps).
+template <int P, int OFF>
+__device__ __forceinline__ void add_rows(col_sum<P>& acc, const float (&r)[BU], uint32_t t, uint32_t m, int p) {
+  constexpr int NA = col_sum<P>::NA;
+#pragma unroll
+  for (int i = 0; i < BU; ++i)
+    if ((t + (uint32_t)i) * P + (uint32_t)p < m) kahan_add(acc.s[(OFF + i) % NA], acc.c[(OFF + i) % NA], r[i]);
+}
+
+// Sum over the in-edges [eb, ee) of one column in ascending source order per sub-sum.  The
+// ids of up to 64 edges come in one vector load (lane i holds edge i), and the rows go in
+// groups of BU steps, double-buffered: the next group's rows are in flight while the
+// current group is added.  Lanes whose column needs no sum issue no row loads.
+template <int P>
+__device__ __forceinline__ void column_sum(const float* X, uint32_t stride, const uint32_t* __restrict__ row_idx,
+                                           uint64_t eb, uint64_t ee, int lane, int p, uint32_t col, bool needs,
+                                           col_sum<P>& acc) {
+  constexpr int NA = col_sum<P>::NA;
+  static_assert(NA <= BU ? BU % NA == 0 : NA == 2 * BU, "sub-sum slots must tile the step groups");
   for (uint64_t e = eb; e < ee; e += 64) {
     const uint32_t m = (uint32_t)std::min<uint64_t>(64, ee - e);  // wave-uniform
     const uint32_t idv = (uint32_t)lane < m ? ld_edge(row_idx + e + lane) : 0u;
+    const uint32_t steps = (m + P - 1) / P;
     float cur[BU];
-    load_rows(X, idv, 0, m, lane, needs, cur);
-    for (uint32_t j = 0; j < m; j += BU) {
+    load_rows<P, BU>(X, stride, idv, 0, m, p, col, needs, cur);
+    for (uint32_t t = 0; t < steps; t += BU) {
       float nxt[BU];
-      if (j + BU < m) load_rows(X, idv, j + BU, m, lane, needs, nxt);
-#pragma unroll
-      for (int i = 0; i < BU; ++i)
-        if (j + (uint32_t)i < m) kahan_add(s, c, cur[i]);
+      if (t + BU < steps) load_rows<P, BU>(X, stride, idv, t + BU, m, p, col, needs, nxt);
+      if (NA > BU && ((t / BU) & 1u)) add_rows<P, (NA > BU ? BU : 0)>(acc, cur, t, m, p);
+      else add_rows<P, 0>(acc, cur, t, m, p);
 #pragma unroll
       for (int i = 0; i < BU; ++i) cur[i] = nxt[i];
     }
@@ -176,21 +232,72 @@ __device__ __forceinline__ uint64_t lane_u64(uint64_t v, int i) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Which columns a pass must write, and which pass kernel runs.  A column is covered while
+// LOADING, ACTIVE or FINAL, and for the first pass after it turns DONE (that pass copies its
+// final value into the other buffer; after it both buffers hold it).  EMPTY and settled DONE
+// columns are left alone: nothing reads an EMPTY column, and a settled one is the same in
+// both buffers.  At most BATCH_PACKED_W covered columns run on the packed block (it may be
+// read when the last pass was packed over the same columns: else the pass reads the wide
+// block); at most 16 or 32 put 4 or 2 edges in a row load of the wide block; more run one
+// edge per load, every column written.
+__global__ void k_batch_plan(batch_ctl* __restrict__ ctl, const uint32_t* __restrict__ status) {
+  const int j = threadIdx.x;  // one wave, lane j = column j
+  const uint32_t st = status[j];
+  const bool covered = st == COL_LOADING || st == COL_ACTIVE || st == COL_FINAL || (st == COL_DONE && ctl->pend[j]);
+  const uint64_t mask = __ballot(covered);
+  const uint32_t n = (uint32_t)__popcll(mask);
+  const uint32_t r = (uint32_t)__popcll(mask & ((1ull << j) - 1ull));
+  const bool packed = n >= 1 && n <= (uint32_t)BATCH_PACKED_W;
+  // the packed block is valid if the last pass was packed over exactly these columns
+  const uint32_t last_n = ctl->pk_n;
+  const bool same = !covered || (r < (uint32_t)BATCH_PACKED_W && ctl->pk_cols[r] == (uint8_t)j);
+  const bool valid = packed && last_n == n && __ballot(!same) == 0ull;
+  __syncthreads();  // every lane has read the last plan before it is overwritten
+  ctl->qof[j] = covered ? (uint8_t)r : (uint8_t)0xFF;
+  if (covered) {
+    ctl->cols[r] = (uint8_t)j;
+    if (packed) ctl->pk_cols[r] = (uint8_t)j;
+  }
+  if (j == 0) {
+    ctl->ncols = n;
+    ctl->mode = packed ? BATCH_MODE_PACKED : n <= 16 ? 4u : n <= 32 ? 2u : 1u;
+    ctl->pk_valid = valid ? 1u : 0u;
+    ctl->pk_n = packed ? n : 0xFFFFFFFFu;
+  }
+}
+
+// One pass over the tasks with P edges per row load; runs only if the plan chose it.
+// PK: the packed pass (P = 4, at most BATCH_PACKED_W columns): rows come from the packed
+// block when the plan allows it, else from the wide one, and every new value goes to both.
+template <int P, bool PK>
 __global__ __launch_bounds__(256) void k_batch_tasks(
     const uint64_t* __restrict__ col_ptr, const uint32_t* __restrict__ row_idx,
     const uint32_t* __restrict__ od, const uint64_t* __restrict__ seedmask,
-    const bt_task* __restrict__ tasks, uint64_t ntasks, float* X0, float* X1,
+    const bt_task* __restrict__ tasks, uint64_t ntasks, float* X0, float* X1, float* P0, float* P1,
     const batch_ctl* __restrict__ ctl, const uint32_t* __restrict__ status,
     const double* __restrict__ init_dist, double damping, float2* __restrict__ partial,
     double* __restrict__ part_delta) {
+  static_assert(!PK || P == 4, "the packed pass puts 4 edges in a load");
+  if (ctl->mode != (PK ? BATCH_MODE_PACKED : (uint32_t)P)) return;
+  constexpr int W = BK / P;
   __shared__ double s_red[4][BK];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = lane / W, q = lane % W;
   const uint32_t cur = ctl->cur;
   const float* X = cur ? X1 : X0;
   float* Y = cur ? X0 : X1;
+  float* YP = cur ? P0 : P1;
+  const bool has_col = P == 1 || (uint32_t)q < ctl->ncols;
+  const uint32_t col = P == 1 ? (uint32_t)lane : (has_col ? (uint32_t)ctl->cols[q] : 0u);
+  // where this pass reads rows: the packed block (entry q) or the wide one (entry col)
+  const bool from_packed = PK && ctl->pk_valid;
+  const float* XR = from_packed ? (cur ? P1 : P0) : X;
+  const uint32_t rstride = from_packed ? (uint32_t)BATCH_PACKED_W : (uint32_t)BK;
+  const uint32_t rc = from_packed ? (uint32_t)q : col;
+  const bool writer = has_col && p == 0;  // the lane that applies and stores its column
   lane_cfg L;
-  L.status = status[lane];
-  L.init_dist = init_dist[lane];
+  L.status = has_col ? status[col] : COL_EMPTY;
+  L.init_dist = has_col ? init_dist[col] : 0.0;
   L.restart = (1.0 - damping) * L.init_dist;
   // only ACTIVE and FINAL columns use the gathered sum; when none does, no row is read
   const bool needs = L.status == COL_ACTIVE || L.status == COL_FINAL;
@@ -198,6 +305,7 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
   double delta = 0.0;
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
+  col_sum<P> acc;
   for (uint64_t ti = wave_id; ti < ntasks; ti += nwaves) {
     const bt_task t = tasks[ti];
     if (t.chunk == 0xFFFFFFFFu) {
@@ -214,27 +322,40 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
       const bool live = (uint32_t)lane < n && (cp != cp_next || smv != 0ull);
       const uint64_t live_mask = __ballot(live);
       float xnext = 0.f;  // the old row of the first live vertex, ahead
-      if (live_mask) xnext = X[(uint64_t)(t.v0 + __builtin_ctzll(live_mask)) * BK + lane];
+      if (live_mask && writer) xnext = ld_row(XR, t.v0 + __builtin_ctzll(live_mask), rstride, rc);
       for (uint32_t i = 0; i < n; ++i) {
         if (!((live_mask >> i) & 1ull)) continue;  // wave-uniform
         const uint32_t v = t.v0 + i;
         const float xold = xnext;
         const uint64_t rest = live_mask & ~((2ull << i) - 1ull);
-        if (rest) xnext = X[(uint64_t)(t.v0 + __builtin_ctzll(rest)) * BK + lane];  // the next live row, ahead
-        float s = 0.f, c = 0.f;
-        if (any_needs) column_sum(X, row_idx, lane_u64(cp, i), lane_u64(cp, i + 1), lane, needs, s, c);
+        if (rest && writer) xnext = ld_row(XR, t.v0 + __builtin_ctzll(rest), rstride, rc);  // the next live row
+        float S = 0.f, C = 0.f;
+        if (any_needs) {
+          acc.zero();
+          column_sum<P>(XR, rstride, row_idx, lane_u64(cp, i), lane_u64(cp, i + 1), lane, p, rc, needs, acc);
+          acc.combine(S, C);
+        }
         const uint64_t sm = lane_u64(smv, i);
         const uint32_t d = __builtin_amdgcn_readlane(odv, i);
-        st_row(Y + (uint64_t)v * BK + lane,
-               batch_apply(L, damping, (double)s - (double)c, xold, d, (sm >> lane) & 1ull, delta));
+        if (writer) {
+          const float y = batch_apply(L, damping, (double)S - (double)C, xold, d, (sm >> col) & 1ull, delta);
+          st_row(Y + (uint64_t)v * BK + col, y);
+          if (PK) st_row(YP + (uint64_t)v * BATCH_PACKED_W + q, y);
+        }
       }
     } else {
-      float s = 0.f, c = 0.f;
-      if (any_needs) column_sum(X, row_idx, t.e0, t.e1, lane, needs, s, c);
-      partial[(uint64_t)t.chunk * BK + lane] = make_float2(s, c);
+      float S = 0.f, C = 0.f;
+      if (any_needs) {
+        acc.zero();
+        column_sum<P>(XR, rstride, row_idx, t.e0, t.e1, lane, p, rc, needs, acc);
+        acc.combine(S, C);
+      }
+      if (writer) partial[(uint64_t)t.chunk * BK + col] = make_float2(S, C);
     }
   }
-  s_red[wave][lane] = delta;
+  s_red[wave][lane] = 0.0;
+  __syncthreads();
+  if (writer) s_red[wave][col] = delta;
   __syncthreads();
   if (threadIdx.x < BK) {
     double d = 0.0;
@@ -246,7 +367,7 @@ __global__ __launch_bounds__(256) void k_batch_tasks(
 // Long columns: chunk partials combined in chunk order, then applied.
 __global__ __launch_bounds__(256) void k_batch_fixup(
     const uint32_t* __restrict__ od, const uint64_t* __restrict__ seedmask,
-    const bt_long* __restrict__ longs, uint64_t nlong, float* X0, float* X1,
+    const bt_long* __restrict__ longs, uint64_t nlong, float* X0, float* X1, float* P0, float* P1,
     const batch_ctl* __restrict__ ctl, const uint32_t* __restrict__ status,
     const double* __restrict__ init_dist, double damping, const float2* __restrict__ partial,
     double* __restrict__ part_delta) {
@@ -259,6 +380,9 @@ __global__ __launch_bounds__(256) void k_batch_fixup(
   L.status = status[lane];
   L.init_dist = init_dist[lane];
   L.restart = (1.0 - damping) * L.init_dist;
+  const bool packed = ctl->mode == BATCH_MODE_PACKED;
+  const uint32_t qk = ctl->qof[lane];
+  float* YP = cur ? P0 : P1;
   double delta = 0.0;
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
@@ -279,9 +403,10 @@ __global__ __launch_bounds__(256) void k_batch_fixup(
     }
     const uint32_t v = lc.v;
     const uint64_t sm = seedmask[v];
-    const float xold = X[(uint64_t)v * BK + lane];
-    Y[(uint64_t)v * BK + lane] =
-        batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta);
+    const float xold = X[(uint64_t)v * BK + lane];  // the wide block holds every covered column
+    const float y = batch_apply(L, damping, (double)s - (double)c, xold, od[v], (sm >> lane) & 1ull, delta);
+    Y[(uint64_t)v * BK + lane] = y;
+    if (packed && qk != 0xFFu) YP[(uint64_t)v * BATCH_PACKED_W + qk] = y;
   }
   s_red[wave][lane] = delta;
   __syncthreads();
@@ -326,6 +451,9 @@ __global__ __launch_bounds__(FIN_GROUPS * BK) void k_batch_finish(const double* 
     } else if (st == COL_FINAL) {
       iters[j] = iters[j] + 1;
       status[j] = COL_DONE;
+      ctl->pend[j] = 1u;  // the next pass copies the final value into the other buffer
+    } else if (st == COL_DONE) {
+      ctl->pend[j] = 0u;
     }
   }
   __syncthreads();
@@ -345,10 +473,21 @@ __global__ void k_batch_seed_bits(uint64_t* __restrict__ seedmask, const uint32_
 
 // A released slot's seeds stop being seeds: their entries in that column go to 0 in both
 // buffers, so vertices without in-edges keep the all-zero state the gather skips.
-__global__ void k_batch_zero_slot(float* X0, float* X1, const uint32_t* __restrict__ ids, uint32_t n, uint32_t slot) {
+// In the packed block a row holds whatever columns the last packed pass covered, so a seed
+// that is left without in-edges and without seeds (a vertex the passes skip) gets its whole
+// packed row cleared; the rows of other vertices are rewritten by every pass.
+__global__ void k_batch_zero_slot(float* X0, float* X1, float* P0, float* P1, const uint64_t* __restrict__ col_ptr,
+                                  const uint64_t* __restrict__ seedmask, const uint32_t* __restrict__ ids, uint32_t n,
+                                  uint32_t slot) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    X0[(uint64_t)ids[i] * BK + slot] = 0.0f;
-    X1[(uint64_t)ids[i] * BK + slot] = 0.0f;
+    const uint32_t v = ids[i];
+    X0[(uint64_t)v * BK + slot] = 0.0f;
+    X1[(uint64_t)v * BK + slot] = 0.0f;
+    if (seedmask[v] == 0ull && col_ptr[v] == col_ptr[v + 1])
+      for (int k = 0; k < BATCH_PACKED_W; ++k) {
+        P0[(uint64_t)v * BATCH_PACKED_W + k] = 0.0f;
+        P1[(uint64_t)v * BATCH_PACKED_W + k] = 0.0f;
+      }
   }
 }
 
@@ -427,6 +566,7 @@ struct pe_ppr_batch_s {
   pe_graph_s* g = nullptr;
   pe_ppr_batch_opts_t opts{};
   dbuf<float> X0, X1;
+  dbuf<float> P0, P1;  // packed rows [V][BATCH_PACKED_W], same parity as X0 / X1
   dbuf<uint64_t> seedmask;
   dbuf<bt_task> tasks;
   dbuf<bt_long> longs;
@@ -507,11 +647,22 @@ struct pe_ppr_batch_s {
     hipStream_t s = g->ctx->stream;
     {
       ktime_scope kt(g->ctx, PE_KT_PPR_BLOCK);
-      k_batch_tasks<<<task_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->od.p, seedmask.p, tasks.p, ntasks,
-                                              X0.p, X1.p, ctl.p, status.p, init_dist.p, opts.damping, partial.p,
-                                              part_delta.p);
+      // the plan picks one of the four pass kernels; the others return at once
+      k_batch_plan<<<1, BK, 0, s>>>(ctl.p, status.p);
+      k_batch_tasks<4, true><<<task_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->od.p, seedmask.p, tasks.p,
+                                                       ntasks, X0.p, X1.p, P0.p, P1.p, ctl.p, status.p, init_dist.p,
+                                                       opts.damping, partial.p, part_delta.p);
+      k_batch_tasks<4, false><<<task_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->od.p, seedmask.p, tasks.p,
+                                                        ntasks, X0.p, X1.p, P0.p, P1.p, ctl.p, status.p, init_dist.p,
+                                                        opts.damping, partial.p, part_delta.p);
+      k_batch_tasks<2, false><<<task_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->od.p, seedmask.p, tasks.p,
+                                                        ntasks, X0.p, X1.p, P0.p, P1.p, ctl.p, status.p, init_dist.p,
+                                                        opts.damping, partial.p, part_delta.p);
+      k_batch_tasks<1, false><<<task_grid, 256, 0, s>>>(g->col_ptr.p, g->row_idx.p, g->od.p, seedmask.p, tasks.p,
+                                                        ntasks, X0.p, X1.p, P0.p, P1.p, ctl.p, status.p, init_dist.p,
+                                                        opts.damping, partial.p, part_delta.p);
     }
-    k_batch_fixup<<<fix_grid, 256, 0, s>>>(g->od.p, seedmask.p, longs.p, nlong, X0.p, X1.p, ctl.p, status.p,
+    k_batch_fixup<<<fix_grid, 256, 0, s>>>(g->od.p, seedmask.p, longs.p, nlong, X0.p, X1.p, P0.p, P1.p, ctl.p, status.p,
                                            init_dist.p, opts.damping, partial.p,
                                            part_delta.p + (uint64_t)task_grid * BK);
     k_batch_finish<<<1, FIN_GROUPS * BK, 0, s>>>(part_delta.p, task_grid + fix_grid, ctl.p, status.p, iters.p,
@@ -577,6 +728,11 @@ int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_b
     const uint64_t cells = (uint64_t)std::max<uint32_t>(1, g->V) * BK;
     b->X0.alloc(ctx, cells);
     b->X1.alloc(ctx, cells);
+    const uint64_t pcells = (uint64_t)std::max<uint32_t>(1, g->V) * BATCH_PACKED_W;
+    b->P0.alloc(ctx, pcells);
+    b->P1.alloc(ctx, pcells);
+    PE_HIP(hipMemsetAsync(b->P0.p, 0, pcells * 4, s));
+    PE_HIP(hipMemsetAsync(b->P1.p, 0, pcells * 4, s));
     b->seedmask.alloc(ctx, std::max<uint32_t>(1, g->V));
     PE_HIP(hipMemsetAsync(b->X0.p, 0, cells * 4, s));
     PE_HIP(hipMemsetAsync(b->X1.p, 0, cells * 4, s));
@@ -590,7 +746,9 @@ int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_b
     b->iters.alloc(ctx, BK);
     b->init_dist.alloc(ctx, BK);
     b->last_delta.alloc(ctx, BK);
-    PE_HIP(hipMemsetAsync(b->ctl.p, 0, sizeof(batch_ctl), s));
+    batch_ctl c0{};
+    c0.pk_n = 0xFFFFFFFFu;  // no packed pass yet
+    PE_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof(batch_ctl), hipMemcpyHostToDevice, s));
     PE_HIP(hipMemsetAsync(b->status.p, 0, BK * 4, s));
     PE_HIP(hipMemsetAsync(b->iters.p, 0, BK * 4, s));
     PE_HIP(hipMemsetAsync(b->init_dist.p, 0, BK * 8, s));
@@ -611,7 +769,7 @@ int pe_ppr_batch_destroy(pe_ppr_batch_t* b) {
 
 int pe_ppr_batch_bytes(const pe_ppr_batch_t* b, uint64_t* bytes) {
   if (!b || !bytes) return PE_ERR_INVALID;
-  *bytes = b->X0.bytes() + b->X1.bytes() + b->seedmask.bytes() + b->tasks.bytes() + b->longs.bytes() +
+  *bytes = b->X0.bytes() + b->X1.bytes() + b->P0.bytes() + b->P1.bytes() + b->seedmask.bytes() + b->tasks.bytes() + b->longs.bytes() +
            b->partial.bytes() + b->part_delta.bytes();
   return PE_OK;
 }
@@ -651,7 +809,9 @@ int pe_ppr_batch_release(pe_ppr_batch_t* b, uint32_t slot) {
       dbuf<uint32_t> d(b->g->ctx, ids.size());
       PE_HIP(hipMemcpyAsync(d.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, s));
       k_batch_seed_bits<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->seedmask.p, d.p, (uint32_t)ids.size(), slot, 0);
-      k_batch_zero_slot<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->X0.p, b->X1.p, d.p, (uint32_t)ids.size(), slot);
+      k_batch_zero_slot<<<grid_for(ids.size(), 256, 64), 256, 0, s>>>(b->X0.p, b->X1.p, b->P0.p, b->P1.p,
+                                                                     b->g->col_ptr.p, b->seedmask.p, d.p,
+                                                                     (uint32_t)ids.size(), slot);
       ids.clear();
     }
     k_batch_set_slot<<<1, 1, 0, s>>>(b->status.p, b->init_dist.p, b->iters.p, b->last_delta.p, slot, COL_EMPTY, 0.0);

@@ -27,6 +27,21 @@ struct bt_long {
 struct batch_ctl {
   uint32_t cur;        // 0: state in X0, 1: in X1
   uint32_t iteration;  // passes run since creation
+  // the pass plan (k_batch_plan): the pass kernel to run (1, 2, 4: edges per row load
+  // over the wide block; BATCH_MODE_PACKED: the packed block), and the columns it writes
+  uint32_t mode;
+  uint32_t ncols;
+  uint8_t cols[64];   // covered columns, ascending
+  uint8_t qof[64];    // position of a column in cols (0xFF: not covered)
+  uint32_t pend[64];  // 1: the column turned DONE last pass and is not yet in both buffers
+  // packed block: its column list as of the last packed pass (pk_n = 0xFFFFFFFF: the last
+  // pass was not packed) and whether this pass may read it
+  uint32_t pk_n;
+  uint32_t pk_valid;
+  uint8_t pk_cols[64];
 };
+
+static constexpr uint32_t BATCH_MODE_PACKED = 5u;
+static constexpr int BATCH_PACKED_W = 8;  // columns in a packed row (32 B)
 
 }  // namespace pe

@@ -70,6 +70,65 @@ def test_alone_equals_in_mix_bitwise(s13):
     assert float(np.abs(r_mix - ref).sum()) <= 1e-6
 
 
+def _alone(g, q, slot=7):
+    from plato_amd.batch import PprBatch
+    a = PprBatch(g, D, tol=1e-7, max_iters=300)
+    a.admit(slot, q)
+    a.run_until_done([slot])
+    r = a.results([slot])[0]
+    a.close()
+    return r
+
+
+def test_pass_modes_bitwise(s13):
+    """Admissions staggered so that passes run on the packed block (at most 8 columns),
+    with 4 and 2 edges per row load (at most 16 and 32 columns) and with one (more), and
+    back down as columns finish: every column equals its query run alone, bit for bit."""
+    from plato_amd.batch import PprBatch
+    g, og = s13
+    seeds = _seeds(og, 48, np.random.default_rng(4))
+    b = PprBatch(g, D, tol=1e-7, max_iters=300)
+    for a0, a1, n in [(0, 3, 6), (3, 12, 6), (12, 30, 6), (30, 48, 6)]:
+        for slot in range(a0, a1):
+            b.admit(slot, [seeds[slot]])
+        b.step(n)
+    b.run_until_done(range(48))
+    res = b.results(list(range(48)))
+    b.close()
+    for slot in (0, 2, 5, 11, 12, 29, 30, 47):
+        assert np.array_equal(res[slot], _alone(g, [seeds[slot]])), slot
+    ref, _ = oracle.ppr(og, [seeds[0]], 2000, D, 0.0)
+    assert float(np.abs(res[0] - ref).sum()) <= 1e-6
+
+
+def test_packed_slot_refilled_in_place(s13):
+    """A column finishes and its slot is refilled before the next pass while another column
+    keeps running: the pass after sees the same covered slots as the last packed pass and
+    reads the packed block, whose entries for the slot still hold the old query."""
+    from plato_amd.batch import DONE, PprBatch
+    g, og = s13
+    seeds = _seeds(og, 3, np.random.default_rng(5))
+    b = PprBatch(g, D, tol=1e-7, max_iters=300)
+    b.admit(0, [seeds[0]])
+    b.step(20)
+    b.admit(1, [seeds[1]])
+    for _ in range(400):
+        b.step(1)
+        st, _, _ = b.poll()
+        if st[0] == DONE:
+            break
+    assert st[0] == DONE and st[1] != DONE
+    first = b.results([0])[0]
+    b.release(0)
+    b.admit(0, [seeds[2]])
+    b.run_until_done([0, 1])
+    res = b.results([0, 1])
+    b.close()
+    assert np.array_equal(first, _alone(g, [seeds[0]]))
+    assert np.array_equal(res[0], _alone(g, [seeds[2]]))
+    assert np.array_equal(res[1], _alone(g, [seeds[1]]))
+
+
 def test_continuous_refill_and_topk(s13):
     from plato_amd.batch import run_queries
     g, og = s13
@@ -82,6 +141,8 @@ def test_continuous_refill_and_topk(s13):
     for qi in range(0, 150, 7):
         ref, _ = oracle.ppr(og, queries[qi], 2000, D, 0.0)
         assert float(np.abs(ranks[qi] - ref).sum()) <= 1e-6, qi
+        if qi % 21 == 0:  # the same bits as the query alone, whatever the mix around it
+            assert np.array_equal(ranks[qi], _alone(g, list(queries[qi]))), qi
         ids, vals = tops[qi]
         order = np.lexsort((np.arange(g.V), -ranks[qi].astype(np.float32)))[:100]
         assert np.array_equal(ids, order.astype(np.uint32))
