@@ -219,17 +219,25 @@ def run_bfs(ctx, scale, roots, seed=11):
     rng = np.random.default_rng(seed)
     cand = np.flatnonzero(od > 0)
     rs = rng.choice(cand, roots, replace=False)
-    teps, secs = [], []
+    teps, secs, alg = [], [], []
     for r in rs:
         res = breadth_first_search(g, bfs_opts_t(int(r)))
         m = res.reached_edges / 2  # undirected edges in the component (Graph500)
         teps.append(m / res.seconds / 1e9)
         secs.append(res.seconds)
+        alg.append(res.alg_bytes)
     teps = np.array(teps)
+    # algorithmic bytes per level (push 8|F| + 4 sum_F deg + 4|new|, pull 8|U| + 4 examined
+    # + V/8), summed over the levels of each search, against the search's time
+    rate = float(np.sum(alg) / np.sum(secs) / 1e9)
     out = {"scale": scale, "roots": roots, "harmonic_mean_gteps": float(len(teps) / np.sum(1 / teps)),
            "min_gteps": float(teps.min()), "max_gteps": float(teps.max()),
            "mean_ms": float(np.mean(secs) * 1e3), "build_s": round(build_s, 2),
-           "relabel_by_degree": True}
+           "relabel_by_degree": True,
+           "roofline": {"bound": "hbm", "achieved": round(rate, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(rate / 8000.0, 4), "alg_bytes_per_root_mean": int(np.mean(alg)),
+                        "definition": "per level: push 8|F| + 4 sum_F deg + 4|new|; pull 8|U| + "
+                                      "4 examined in-edges (each list up to its first frontier member) + V/8"}}
     g.close()
     return out
 
@@ -372,6 +380,29 @@ def ppr_seeds(od, rng, n=64):
     return [int(rng.choice(hi)) if i % 2 == 0 else int(rng.choice(lo)) for i in range(n)]
 
 
+def partial_passes(ctx, g, seeds, counts=(64, 48, 32, 16, 8, 1), passes=8):
+    """Kernel time of one block pass with n columns active (tolerance 0, so no column
+    retires), after two passes that load them."""
+    from plato_amd import _lib as L
+    from plato_amd.batch import PprBatch
+    out = {}
+    for n in counts:
+        b = PprBatch(g, 0.85, 0.0, 100000)
+        for slot in range(n):
+            b.admit(slot, [seeds[slot]])
+        b.step(2)
+        ctx.sync()
+        ctx.set_kernel_timing(True)
+        ctx.kernel_time(L.PE_KT_PPR_BLOCK)
+        b.step(passes)
+        ctx.sync()
+        ms, k = ctx.kernel_time(L.PE_KT_PPR_BLOCK)
+        ctx.set_kernel_timing(False)
+        b.close()
+        out[str(n)] = round(ms / max(k, 1), 4)
+    return out
+
+
 def run_batched_ppr(ctx, scale, tol=1e-7, max_iters=200, seed=24, single_runs=True):
     """64 PPR queries as one 64-column block vs the same queries one at a time."""
     from plato_amd import Graph
@@ -397,6 +428,7 @@ def run_batched_ppr(ctx, scale, tol=1e-7, max_iters=200, seed=24, single_runs=Tr
     sample = [0, 1, 2, 3, 32, 33, 62, 63]
     batch_ranks = b.results(sample)
     b.close()
+    partial = partial_passes(ctx, g, seeds)
     # the plain one-seed GPU engine: pull PPR, same tolerance rule, one query at a time,
     # on each pull layout (the binned one is the faster one-seed engine)
     single = {}
@@ -419,7 +451,7 @@ def run_batched_ppr(ctx, scale, tol=1e-7, max_iters=200, seed=24, single_runs=Tr
     if not single_runs:  # profiling runs: the block alone
         g.close()
         return {"scale": scale, "k": 64, "tol": tol, "batched_qps": 64 / t_batch,
-                "batched_seconds": t_batch, "passes": passes}
+                "batched_seconds": t_batch, "passes": passes, "pass_ms_by_active_columns": partial}
     l1_batch = [float(np.abs(batch_ranks[i] - ref[j]).sum()) for i, j in enumerate(sample)]
     # push/pull single-seed variant on 8 of the seeds
     t_pp = 0.0
@@ -439,16 +471,25 @@ def run_batched_ppr(ctx, scale, tol=1e-7, max_iters=200, seed=24, single_runs=Tr
             "mean_iterations": float(np.mean(iters)), "max_iterations": int(np.max(iters)),
             "single_mean_iterations": float(np.mean(single_iters)),
             "max_l1_vs_fp64_batched": max(l1_batch), "max_l1_vs_fp64_single": max(l1_single),
-            "l1_sample_columns": len(sample)}
+            "l1_sample_columns": len(sample), "pass_ms_by_active_columns": partial}
 
 
 def run_serving(ctx, scale, qps_list, n_queries, seed=25):
-    """Poisson request stream against a resident graph, 1-256 seeds per query, top-100."""
+    """Poisson request stream against a resident graph, 1-256 seeds per query, top-100.
+    Per point: HBM admitted (blocks opened x one block's device bytes).  After the sweep:
+    the top-100 of sampled queries of the busiest point run again alone in a block (max L1
+    difference of the top-100 values, and whether the ids agree), and the latency of the
+    one-seed engine (binned pull) on the same graph for comparison."""
     from plato_amd import Graph
+    from plato_amd.algo import multi_seed_ppr
+    from plato_amd.batch import PprBatch
     from plato_amd.serve import Server, poisson_trace
     g = Graph.rmat(ctx, scale, 16, seed=seed, relabel=True)
     cand = np.flatnonzero(g.out_degrees() > 0)
+    alone = PprBatch(g, 0.85, 1e-7, 200)
+    block_bytes = alone.device_bytes()
     out = []
+    last = None
     for qps in qps_list:
         rng = np.random.default_rng(seed + int(qps))
         arrivals, queries = poisson_trace(rng, n_queries, qps, cand)
@@ -458,10 +499,32 @@ def run_serving(ctx, scale, qps_list, n_queries, seed=25):
         srv.close()
         row = {"offered_qps": qps}
         row.update(res.summary())
+        row["admitted_hbm_bytes"] = int(row["blocks_used"] * block_bytes)
         out.append(row)
+        last = (queries, res)
         log(f"[bench] serving qps={qps}: {row}")
+    queries, res = last
+    sample = sorted({0, len(queries) // 3, 2 * len(queries) // 3, len(queries) - 1})
+    l1, ids_equal = 0.0, True
+    for qi in sample:
+        alone.admit(0, queries[qi])
+        alone.run_until_done([0], chunk=4)
+        ids, vals = alone.topk([0], 100)
+        alone.release(0)
+        ids_equal &= bool(np.array_equal(ids[0], res.topk_ids[qi]))
+        l1 = max(l1, float(np.abs(vals[0].astype(np.float64) - res.topk_vals[qi].astype(np.float64)).sum()))
+    alone.close()
+    g.set_pull_layout("binned")
+    one = []
+    for sd in cand[np.random.default_rng(seed).choice(cand.size, 6, replace=False)]:
+        r = multi_seed_ppr(g, [int(sd)], 200, 0.85, 1e-7, precision=32)
+        one.append(r.seconds)
     g.close()
-    return {"scale": scale, "queries_per_point": n_queries, "points": out}
+    return {"scale": scale, "queries_per_point": n_queries, "points": out, "block_bytes": int(block_bytes),
+            "alone_vs_mix": {"queries": len(sample), "max_l1_top100": l1, "top100_ids_equal": ids_equal,
+                             "offered_qps": qps_list[-1]},
+            "one_seed_engine_ms": {"mean": float(np.mean(one[1:]) * 1e3), "runs": len(one) - 1,
+                                   "engine": "binned pull, FP32, tol 1e-7"}}
 
 
 def main():

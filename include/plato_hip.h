@@ -427,6 +427,10 @@ typedef struct {
   uint32_t visited;       /* |visited| (the reference's return value) */
   uint64_t reached_edges; /* sum of od over reached vertices */
   double seconds;
+  /* algorithmic bytes of the search, per level: push 8|F| + 4 sum_F deg + 4|new|; pull
+   * 8|U| + 4 * (in-edges a sequential scan reads, each unvisited list up to its first
+   * frontier member) + V/8 */
+  uint64_t alg_bytes;
 } pe_bfs_stats_t;
 /* plato::algo::breadth_first_search (plato/algo/bfs/bfs.hpp:55-155) with the
  * engine switch (dualmode.hpp:261-279).  levels: host int32 [V] (root 0,

@@ -80,7 +80,7 @@ class RankStats(C.Structure):
 
 class BfsStats(C.Structure):
     _fields_ = [("epochs", C.c_uint32), ("visited", C.c_uint32), ("reached_edges", C.c_uint64),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("alg_bytes", C.c_uint64)]
 
 
 class PprBatchOpts(C.Structure):

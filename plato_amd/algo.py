@@ -130,6 +130,7 @@ class BfsResult:
     visited: int
     reached_edges: int
     seconds: float
+    alg_bytes: int = 0  # per level: push 8|F| + 4 sum_F deg + 4|new|, pull 8|U| + 4 examined + V/8
 
 
 def breadth_first_search(graph: Graph, opts: bfs_opts_t = bfs_opts_t(),
@@ -145,7 +146,7 @@ def breadth_first_search(graph: Graph, opts: bfs_opts_t = bfs_opts_t(),
                            max_epochs, C.byref(st)), graph.ctx.h)
     k = min(st.epochs, max_epochs)
     return BfsResult(levels, modes[:k], frontier[:k], active[:k], st.epochs, st.visited,
-                     st.reached_edges, st.seconds)
+                     st.reached_edges, st.seconds, st.alg_bytes)
 
 
 def _tr_opts(opts, precision):

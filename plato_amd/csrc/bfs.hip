@@ -36,7 +36,7 @@ struct bfs_counters {
   unsigned long long next_count;
   unsigned long long next_active;
   unsigned long long heavy_count;
-  unsigned long long pad;
+  unsigned long long examined;  // pull: in-edges a sequential scan reads, each list up to its first hit
 };
 
 __device__ __forceinline__ unsigned wave_prefix(uint64_t mask) {
@@ -62,6 +62,7 @@ __global__ void k_bfs_init(uint32_t* __restrict__ level, uint64_t* __restrict__ 
     cnt->next_count = 1;
     cnt->next_active = od[root];
     cnt->heavy_count = 0;
+    cnt->examined = 0;
   }
 }
 
@@ -93,6 +94,7 @@ __global__ void k_bfs_reset(bfs_counters* __restrict__ cnt) {
   cnt->next_count = 0;
   cnt->next_active = 0;
   cnt->heavy_count = 0;
+  cnt->examined = 0;
 }
 
 // wave-aggregated append of the lanes in `mask`; returns this lane's slot
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
   for (uint64_t i = threadIdx.x; i < nh; i += blockDim.x) hot[i] = cur_bits[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  unsigned long long active = 0, found = 0;
+  unsigned long long active = 0, found = 0, examined = 0;
   for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
        word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     const uint64_t v = word * 64 + lane;
@@ -353,6 +355,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
       const uint32_t f = first_in[v];
       if (f != UNVIS) {
         const uint64_t fw = (f >> 6) < nh ? hot[f >> 6] : cur_bits[f >> 6];
+        examined += 1;  // the list's first entry
         if ((fw >> (f & 63)) & 1ull) {
           claimed = true;
         } else {
@@ -368,10 +371,14 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
               const uint32_t u = us[k];
               if (u != UNVIS) {
                 const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
-                if ((wd >> (u & 63)) & 1ull) claimed = true;
+                if (((wd >> (u & 63)) & 1ull) && !claimed) {
+                  claimed = true;
+                  examined += i + k - b;  // entries b+1 .. i+k
+                }
               }
             }
           }
+          if (!claimed) examined += lim - b - 1;  // every entry up to the cap
           is_heavy = !claimed && e > lim;
         }
       }
@@ -391,7 +398,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
       active += od[v];
     }
   }
-  block_add2(active, found, &cnt->next_active, &cnt->next_count);
+  block_add3(active, found, examined, &cnt->next_active, &cnt->next_count, &cnt->examined);
 }
 
 // second pull pass: one wave per heavy vertex, 64 in-edges per step, stop at the first hit
@@ -404,11 +411,12 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
   const uint64_t wave_id = grid_wave_id();
   const uint64_t nwaves = grid_waves();
   const unsigned long long n = cnt->heavy_count;
-  unsigned long long active = 0;
+  unsigned long long active = 0, examined = 0;
   for (uint64_t h = wave_id; h < n; h += nwaves) {
     const uint32_t v = heavy[h];
     const uint64_t b = col_ptr[v] + PULL_CAP, e = col_ptr[v + 1];
     bool found = false;
+    uint64_t scanned = e - b;  // entries past the cap a sequential scan reads
     for (uint64_t base = b; base < e && !found; base += 64) {
       const uint64_t i = base + lane;
       bool hit = false;
@@ -416,8 +424,11 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
         const uint32_t u = row_idx[i];
         hit = (cur_bits[u >> 6] >> (u & 63)) & 1ull;
       }
-      found = __ballot(hit) != 0ull;
+      const uint64_t hits = __ballot(hit);
+      found = hits != 0ull;
+      if (found) scanned = base - b + (uint64_t)__builtin_ctzll(hits) + 1;
     }
+    if (lane == 0) examined += scanned;
     if (found && lane == 0) {
       level[v] = L;
       atomicOr((unsigned long long*)&next_bits[v >> 6], 1ull << (v & 63));
@@ -426,6 +437,7 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(
     }
   }
   if (lane == 0 && active) atomicAdd(&cnt->next_active, active);
+  if (lane == 0 && examined) atomicAdd(&cnt->examined, examined);
 }
 
 static constexpr unsigned COMPACT_BLOCKS_PER_CU = 4;
@@ -589,6 +601,11 @@ struct bfs_state {
     return false;
   }
 
+  // Algorithmic bytes of the search (counted per level): push 8|F| + 4 sum_F deg + 4|new|
+  // (offsets, edge ids, level writes), pull 8|U| + 4 examined + V/8 (offsets of the unvisited
+  // vertices, the in-edges a sequential scan reads up to its first hit, the next bitmap).
+  uint64_t alg_bytes = 0;
+
   int run(uint32_t root, double threshold, uint8_t* modes, uint64_t* frontier, uint64_t* actv,
           uint32_t max_epochs, double* seconds) {
     pe_ctx_s* ctx = g->ctx;
@@ -615,6 +632,8 @@ struct bfs_state {
     uint64_t* bn = b1.p;
     int epoch = 0;
     bool queued = true;  // qc holds the current frontier as a queue
+    alg_bytes = 0;
+    uint64_t visited = n;  // visited so far, the current frontier included
     while (n != 0) {
       const bool push = ((double)A / (double)Ep) < threshold;  // dualmode.hpp:279
       if ((uint32_t)epoch < max_epochs) {
@@ -643,8 +662,11 @@ struct bfs_state {
       PE_HIP(hipGetLastError());
       h = reinterpret_cast<const bfs_counters*>(mail_post(ctx, reinterpret_cast<const unsigned long long*>(c), CW,
                                                           reinterpret_cast<unsigned long long*>(set(epoch + 1)), CW));
+      if (push) alg_bytes += 8 * n + 4 * A + 4 * h->next_count;
+      else alg_bytes += 8 * ((uint64_t)g->V - visited) + 4 * h->examined + (uint64_t)g->V / 8;
       n = h->next_count;
       A = h->next_active;
+      visited += n;
       std::swap(qc, qn);
       std::swap(bc, bn);
       ++epoch;
@@ -795,6 +817,7 @@ int pe_bfs(pe_graph_t* g, uint32_t root, double threshold, int32_t* levels, uint
       stats->visited = (uint32_t)(r[1] & 0xFFFFFFFFull);
       stats->reached_edges = r[0];
       stats->seconds = secs;
+      stats->alg_bytes = st.alg_bytes;
     }
   });
 }

@@ -218,6 +218,35 @@ __device__ __forceinline__ void block_add2(unsigned long long a, unsigned long l
   }
 }
 
+// three sums, as block_add2
+__device__ __forceinline__ void block_add3(unsigned long long a, unsigned long long b, unsigned long long c,
+                                           unsigned long long* da, unsigned long long* db, unsigned long long* dc) {
+  __shared__ unsigned long long part3[3][16];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part3[0][wave] = a;
+    part3[1][wave] = b;
+    part3[2][wave] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sa = 0, sb = 0, sc = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      sa += part3[0][w];
+      sb += part3[1][w];
+      sc += part3[2][w];
+    }
+    if (sa) atomicAdd(da, sa);
+    if (sb) atomicAdd(db, sb);
+    if (sc) atomicAdd(dc, sc);
+  }
+}
+
 }  // namespace pe
 
 // ---- binned pull layout (pull_binned.hip) -------------------------------------------

@@ -32,6 +32,8 @@ def test_bfs_rmat_bit_exact(ctx, directed):
         assert np.array_equal(res.active_edges, ref["active_edges"])
         assert res.visited == ref["visited"]
         assert res.reached_edges == int(og.od[ref["levels"] >= 0].sum())
+        assert res.alg_bytes == _alg_bytes(og, ref["levels"], ref["modes"], ref["frontier"],
+                                           ref["active_edges"]), f"root {root}"
 
 
 def test_bfs_modes_cover_push_and_pull(ctx):
@@ -114,6 +116,32 @@ def test_engine_step_loop_mirrors_reference(ctx):
         assert np.array_equal(st.levels(), ref["levels"])
 
 
+def _alg_bytes(og, levels, modes, frontier, active):
+    """Algorithmic bytes of a search from its levels, counted per level as the bench's BFS
+    roofline counts them: push 8|F| + 4 sum_F deg + 4|new|; pull 8|U| + 4 examined + V/8,
+    where examined is what a sequential scan of each unvisited vertex's ascending in-edge
+    list reads up to its first frontier member (the whole list if there is none)."""
+    V = og.V
+    cp = og.col_ptr.astype(np.int64)
+    ri = og.row_idx[: int(cp[-1])]
+    deg = np.diff(cp)
+    pos = np.arange(ri.size, dtype=np.int64) - np.repeat(cp[:-1], deg)
+    nz = np.flatnonzero(deg > 0)
+    total = 0
+    for e, m in enumerate(modes.tolist()):
+        n = int(frontier[e])
+        nxt = int(frontier[e + 1]) if e + 1 < len(frontier) else 0
+        if m == 1:
+            total += 8 * n + 4 * int(active[e]) + 4 * nxt
+            continue
+        cand = (levels > e) | (levels < 0)
+        hit = levels[ri] == e
+        first = np.minimum.reduceat(np.where(hit, pos, np.repeat(deg, deg)), cp[nz])
+        ex = np.where(first < deg[nz], first + 1, deg[nz])
+        total += 8 * int(cand.sum()) + 4 * int(ex[cand[nz]].sum()) + V // 8
+    return total
+
+
 def _check_bfs(g, og, root, threshold):
     from plato_amd.algo import bfs_opts_t, breadth_first_search
     ref = oracle.bfs(og, int(root), threshold=threshold)
@@ -123,6 +151,7 @@ def _check_bfs(g, og, root, threshold):
     assert np.array_equal(res.frontier, ref["frontier"])
     assert np.array_equal(res.active_edges, ref["active_edges"])
     assert res.visited == ref["visited"]
+    assert res.alg_bytes == _alg_bytes(og, ref["levels"], ref["modes"], ref["frontier"], ref["active_edges"])
     return res
 
 
