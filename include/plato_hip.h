@@ -135,6 +135,8 @@ int pe_graph_create_rmat(pe_ctx_t* ctx, uint32_t scale, uint32_t edge_factor, ui
 /* Generate RMAT edges [e0, e0+n) into host arrays (same generator). */
 int pe_rmat_edges(pe_ctx_t* ctx, uint32_t scale, uint64_t seed, double a, double b, double c,
                   uint64_t e0, uint64_t n, uint32_t* src, uint32_t* dst, float* w);
+/* Objects built on g (BFS states, PPR blocks, servers, reverses) must not be used after g is
+ * destroyed; destroying them afterwards is allowed. */
 int pe_graph_destroy(pe_graph_t* g);
 int pe_graph_get_info(const pe_graph_t* g, pe_graph_info_t* info);
 /* Build from CSV text in host memory, parsed on the device with the reference's
@@ -458,7 +460,13 @@ typedef struct {
   double tol;           /* per-column delta threshold (the reference's eps_) */
   uint32_t max_iters;   /* iteration cap per column (the reference's iteration_) */
   uint32_t task_edges;  /* edges per wave task (0: 4096); longer columns are split */
+  /* Pass plan: a pass with n covered columns (LOADING, ACTIVE, FINAL, or DONE for one copy
+   * pass) runs on the packed block if n <= plan_packed (at most 8), with 4 edges per row
+   * load if n <= plan_edges4 (at most 16), 2 if n <= plan_edges2 (at most 32), else 1.
+   * 0 takes the widest (8, 16, 32); PE_BATCH_PLAN_OFF never uses that kernel. */
+  uint32_t plan_packed, plan_edges4, plan_edges2;
 } pe_ppr_batch_opts_t;
+#define PE_BATCH_PLAN_OFF 0xFFFFFFFFu
 
 int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_batch_t** out);
 int pe_ppr_batch_destroy(pe_ppr_batch_t* b);
@@ -551,17 +559,28 @@ int pe_bfs_bind(pe_bfs_t* b, pe_graph_t* g);
 /* ===========================================================================
  * Autotuner (no reference counterpart: the reference has no tunables on this
  * path beyond dualmode_engine_opts_t::push_threshold_).  Times candidates on
- * the resident graph and keeps the fastest: pull tile size (applied to the
- * graph), the 64-column PPR block's task size, and the BFS push/pull ratio.
+ * the resident graph and keeps the fastest:
+ *   PE_TUNE_PULL    merge-path items per pull tile (applied to the graph);
+ *   PE_TUNE_BATCH   the 64-column PPR block's task size, and its pass plan: which
+ *                   pass kernel (packed block, 4, 2 or 1 edges per row load) runs
+ *                   passes with 8, 16 and 32 covered columns;
+ *   PE_TUNE_BFS     the BFS push/pull ratio;
+ *   PE_TUNE_BINNED  the binned layout's source chunk and destination bin widths
+ *                   (applied to the graph: the best layout is left built).
  * ========================================================================= */
 #define PE_TUNE_PULL 1u
 #define PE_TUNE_BATCH 2u
 #define PE_TUNE_BFS 4u
+#define PE_TUNE_BINNED 8u
 typedef struct {
   uint32_t tile_items;        /* chosen merge-path items per pull tile */
   uint32_t batch_task_edges;  /* pe_ppr_batch_opts_t::task_edges */
   double bfs_push_threshold;  /* threshold for pe_bfs */
   double pull_ms, batch_ms, bfs_ms;  /* best times seen (per iteration / per search) */
+  uint32_t binned_src_bits, binned_dst_bits;  /* pe_binned_opts_t of the best binned layout */
+  double binned_ms;                           /* its iteration time */
+  uint32_t batch_plan_packed, batch_plan_edges4, batch_plan_edges2;  /* pe_ppr_batch_opts_t plan */
+  double batch_plan_ms[3];  /* best pass times with 8, 16 and 32 covered columns */
 } pe_tune_result_t;
 int pe_autotune(pe_graph_t* g, uint32_t what, pe_tune_result_t* out);
 /* append / look up a choice keyed by (V, nnz, directed, relabeled) in a text file */

@@ -85,7 +85,11 @@ class BfsStats(C.Structure):
 
 class PprBatchOpts(C.Structure):
     _fields_ = [("k", C.c_uint32), ("damping", C.c_double), ("tol", C.c_double),
-                ("max_iters", C.c_uint32), ("task_edges", C.c_uint32)]
+                ("max_iters", C.c_uint32), ("task_edges", C.c_uint32),
+                ("plan_packed", C.c_uint32), ("plan_edges4", C.c_uint32), ("plan_edges2", C.c_uint32)]
+
+
+PE_BATCH_PLAN_OFF = 0xFFFFFFFF
 
 
 class ServerOpts(C.Structure):
@@ -103,10 +107,13 @@ class RequestStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tile_items", C.c_uint32), ("batch_task_edges", C.c_uint32),
                 ("bfs_push_threshold", C.c_double), ("pull_ms", C.c_double),
-                ("batch_ms", C.c_double), ("bfs_ms", C.c_double)]
+                ("batch_ms", C.c_double), ("bfs_ms", C.c_double),
+                ("binned_src_bits", C.c_uint32), ("binned_dst_bits", C.c_uint32), ("binned_ms", C.c_double),
+                ("batch_plan_packed", C.c_uint32), ("batch_plan_edges4", C.c_uint32),
+                ("batch_plan_edges2", C.c_uint32), ("batch_plan_ms", C.c_double * 3)]
 
 
-PE_TUNE_PULL, PE_TUNE_BATCH, PE_TUNE_BFS = 1, 2, 4
+PE_TUNE_PULL, PE_TUNE_BATCH, PE_TUNE_BFS, PE_TUNE_BINNED = 1, 2, 4, 8
 
 _vp = C.c_void_p
 _u32, _u64, _i32, _dbl = C.c_uint32, C.c_uint64, C.c_int, C.c_double

@@ -16,14 +16,18 @@ K = 64
 class PprBatch:
     """A resident rank block of 64 query columns on one graph."""
 
-    def __init__(self, graph: Graph, damping=0.85, tol=1e-7, max_iters=200, task_edges=0):
+    def __init__(self, graph: Graph, damping=0.85, tol=1e-7, max_iters=200, task_edges=0, plan=(0, 0, 0)):
+        """plan: (packed, edges4, edges2) limits of the pass plan (pe_ppr_batch_opts_t): 0 the
+        kernel's widest, L.PE_BATCH_PLAN_OFF never."""
         self.graph, self.ctx, self.lib = graph, graph.ctx, graph.lib
         o = L.PprBatchOpts()
         o.k, o.damping, o.tol, o.max_iters, o.task_edges = K, damping, tol, max_iters, task_edges
+        o.plan_packed, o.plan_edges4, o.plan_edges2 = plan
         h = C.c_void_p()
         check(self.lib.pe_ppr_batch_create(graph.h, C.byref(o), C.byref(h)), self.ctx.h)
         self.h = h
         self.ctx._adopt(self)
+        graph._depend(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -691,6 +691,7 @@ extern "C" {
 
 struct pe_bfs_s {
   pe_graph_s* g;
+  pe_ctx_s* ctx = nullptr;  // the graph's context; destroy uses it, not the graph (which may be gone)
   std::unique_ptr<bfs_state> st;
   uint32_t epoch = 0;
 };
@@ -703,6 +704,7 @@ int pe_bfs_create(pe_graph_t* g, pe_bfs_t** out) {
   return guarded(g->ctx, [&] {
     auto b = std::make_unique<pe_bfs_s>();
     b->g = g;
+    b->ctx = g->ctx;
     b->st = std::make_unique<bfs_state>(g);
     *out = b.release();
   });
@@ -710,7 +712,7 @@ int pe_bfs_create(pe_graph_t* g, pe_bfs_t** out) {
 
 int pe_bfs_destroy(pe_bfs_t* b) {
   if (!b) return PE_ERR_INVALID;
-  (void)hipStreamSynchronize(b->g->ctx->stream);
+  (void)hipStreamSynchronize(b->ctx->stream);
   delete b;
   return PE_OK;
 }

@@ -239,7 +239,8 @@ __device__ __forceinline__ uint64_t lane_u64(uint64_t v, int i) {
 // both buffers.  At most BATCH_PACKED_W covered columns run on the packed block (it may be
 // read when the last pass was packed over the same columns: else the pass reads the wide
 // block); at most 16 or 32 put 4 or 2 edges in a row load of the wide block; more run one
-// edge per load, every column written.
+// edge per load, every column written.  The limits come from the batch options (these
+// widths by default; a kernel can be turned off).
 __global__ void k_batch_plan(batch_ctl* __restrict__ ctl, const uint32_t* __restrict__ status) {
   const int j = threadIdx.x;  // one wave, lane j = column j
   const uint32_t st = status[j];
@@ -247,7 +248,7 @@ __global__ void k_batch_plan(batch_ctl* __restrict__ ctl, const uint32_t* __rest
   const uint64_t mask = __ballot(covered);
   const uint32_t n = (uint32_t)__popcll(mask);
   const uint32_t r = (uint32_t)__popcll(mask & ((1ull << j) - 1ull));
-  const bool packed = n >= 1 && n <= (uint32_t)BATCH_PACKED_W;
+  const bool packed = n >= 1 && n <= ctl->max_packed;
   // the packed block is valid if the last pass was packed over exactly these columns
   const uint32_t last_n = ctl->pk_n;
   const bool same = !covered || (r < (uint32_t)BATCH_PACKED_W && ctl->pk_cols[r] == (uint8_t)j);
@@ -260,7 +261,7 @@ __global__ void k_batch_plan(batch_ctl* __restrict__ ctl, const uint32_t* __rest
   }
   if (j == 0) {
     ctl->ncols = n;
-    ctl->mode = packed ? BATCH_MODE_PACKED : n <= 16 ? 4u : n <= 32 ? 2u : 1u;
+    ctl->mode = packed ? BATCH_MODE_PACKED : n <= ctl->max_edges4 ? 4u : n <= ctl->max_edges2 ? 2u : 1u;
     ctl->pk_valid = valid ? 1u : 0u;
     ctl->pk_n = packed ? n : 0xFFFFFFFFu;
   }
@@ -564,6 +565,7 @@ using namespace pe;
 
 struct pe_ppr_batch_s {
   pe_graph_s* g = nullptr;
+  pe_ctx_s* ctx = nullptr;  // the graph's context; destroy uses it, not the graph (which may be gone)
   pe_ppr_batch_opts_t opts{};
   dbuf<float> X0, X1;
   dbuf<float> P0, P1;  // packed rows [V][BATCH_PACKED_W], same parity as X0 / X1
@@ -723,6 +725,7 @@ int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_b
     hipStream_t s = ctx->stream;
     auto b = std::make_unique<pe_ppr_batch_s>();
     b->g = g;
+    b->ctx = g->ctx;
     b->opts = *opts;
     if (!b->opts.task_edges) b->opts.task_edges = 4096;
     const uint64_t cells = (uint64_t)std::max<uint32_t>(1, g->V) * BK;
@@ -748,6 +751,12 @@ int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_b
     b->last_delta.alloc(ctx, BK);
     batch_ctl c0{};
     c0.pk_n = 0xFFFFFFFFu;  // no packed pass yet
+    auto limit = [](uint32_t v, uint32_t widest) {
+      return v == 0 ? widest : v == PE_BATCH_PLAN_OFF ? 0u : std::min(v, widest);
+    };
+    c0.max_packed = limit(opts->plan_packed, (uint32_t)BATCH_PACKED_W);
+    c0.max_edges4 = limit(opts->plan_edges4, 16u);
+    c0.max_edges2 = limit(opts->plan_edges2, 32u);
     PE_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof(batch_ctl), hipMemcpyHostToDevice, s));
     PE_HIP(hipMemsetAsync(b->status.p, 0, BK * 4, s));
     PE_HIP(hipMemsetAsync(b->iters.p, 0, BK * 4, s));
@@ -762,7 +771,7 @@ int pe_ppr_batch_create(pe_graph_t* g, const pe_ppr_batch_opts_t* opts, pe_ppr_b
 
 int pe_ppr_batch_destroy(pe_ppr_batch_t* b) {
   if (!b) return PE_ERR_INVALID;
-  (void)hipStreamSynchronize(b->g->ctx->stream);
+  (void)hipStreamSynchronize(b->ctx->stream);
   delete b;
   return PE_OK;
 }

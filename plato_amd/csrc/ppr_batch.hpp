@@ -39,6 +39,8 @@ struct batch_ctl {
   uint32_t pk_n;
   uint32_t pk_valid;
   uint8_t pk_cols[64];
+  // the plan's limits (pe_ppr_batch_opts_t, resolved): largest covered count per kernel, 0 = off
+  uint32_t max_packed, max_edges4, max_edges2;
 };
 
 static constexpr uint32_t BATCH_MODE_PACKED = 5u;

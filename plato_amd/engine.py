@@ -119,6 +119,11 @@ class Graph:
         self.info = L.GraphInfo()
         check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), ctx.h)
         ctx._adopt(self)
+        # objects built on this graph (BFS states, PPR blocks, servers): closed before it
+        self._dependents = weakref.WeakSet()
+
+    def _depend(self, obj):
+        self._dependents.add(obj)
 
     @classmethod
     def from_edges(cls, ctx, src, dst, V, weights=None, directed=True, build_csr=False,
@@ -196,6 +201,8 @@ class Graph:
 
     def close(self):
         if getattr(self, "h", None):
+            for dep in list(getattr(self, "_dependents", ())):
+                dep.close()
             if self.ctx.h:
                 self.lib.pe_graph_destroy(self.h)
             self.h = None
@@ -242,15 +249,20 @@ class Graph:
         check(self.lib.pe_graph_set_pull_layout(self.h, code, C.byref(o)), self.ctx.h)
         check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
 
-    def autotune(self, pull=True, batch=False, bfs=False):
-        """Time the engine's tunables on this graph (pe_autotune); the pull tile
-        choice is applied to the graph.  Returns the chosen values."""
+    def autotune(self, pull=True, batch=False, bfs=False, binned=False):
+        """Time the engine's tunables on this graph (pe_autotune); the pull tile choice
+        and the best binned layout (binned=True) are applied to the graph.  Returns the
+        chosen values."""
         what = (L.PE_TUNE_PULL if pull else 0) | (L.PE_TUNE_BATCH if batch else 0) | \
-            (L.PE_TUNE_BFS if bfs else 0)
+            (L.PE_TUNE_BFS if bfs else 0) | (L.PE_TUNE_BINNED if binned else 0)
         r = L.TuneResult()
         check(self.lib.pe_autotune(self.h, what, C.byref(r)), self.ctx.h)
         check(self.lib.pe_graph_get_info(self.h, C.byref(self.info)), self.ctx.h)
-        return {f: getattr(r, f) for f, _ in L.TuneResult._fields_}
+        out = {}
+        for f, _ in L.TuneResult._fields_:
+            v = getattr(r, f)
+            out[f] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def download_csc(self, weights=False):
         cp = np.zeros(self.V + 1, np.uint64)
@@ -522,6 +534,7 @@ class BfsEngineState:
         check(self.lib.pe_bfs_create(graph.h, C.byref(h)), self.ctx.h)
         self.h = h
         self.ctx._adopt(self)
+        graph._depend(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -50,6 +50,7 @@ class Server:
         check(self.lib.pe_server_create(graph.h, C.byref(o), C.byref(h)), self.ctx.h)
         self.h = h
         self.ctx._adopt(self)
+        graph._depend(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -149,6 +149,29 @@ def test_continuous_refill_and_topk(s13):
         assert np.array_equal(vals, ranks[qi].astype(np.float32)[order])
 
 
+def test_batch_outlives_its_graph(ctx):
+    """A block (or BFS state) destroyed after its graph: destroy touches only the context.
+    The Python wrappers close a graph's dependents before the graph itself."""
+    from plato_amd import Graph
+    from plato_amd.batch import PprBatch
+    from plato_amd.engine import BfsEngineState
+    g = Graph.rmat(ctx, 10, 8, seed=1, build_csr=True)
+    b = PprBatch(g, D, tol=1e-7, max_iters=50)
+    st = BfsEngineState(g)
+    b.admit(0, [1])
+    b.step(2)
+    hb, hs = b.h, st.h
+    assert g.lib.pe_graph_destroy(g.h) == 0
+    g.h = None
+    assert g.lib.pe_ppr_batch_destroy(hb) == 0
+    assert g.lib.pe_bfs_destroy(hs) == 0
+    b.h = st.h = None
+    g2 = Graph.rmat(ctx, 10, 8, seed=2)
+    b2 = PprBatch(g2, D, tol=1e-7, max_iters=50)
+    g2.close()
+    assert b2.h is None
+
+
 def test_slot_errors(s13):
     from plato_amd._lib import PlatoHipError
     from plato_amd.batch import PprBatch
