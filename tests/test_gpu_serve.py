@@ -44,6 +44,7 @@ def test_serving_mix_equals_alone(ctx):
 
 def test_serving_opens_more_blocks_under_load(ctx):
     from plato_amd import Graph
+    from plato_amd.batch import PprBatch
     from plato_amd.serve import Server, poisson_trace
     g = Graph.rmat(ctx, 11, 16, seed=6, relabel=True)
     od = g.out_degrees()
@@ -52,5 +53,17 @@ def test_serving_opens_more_blocks_under_load(ctx):
     srv = Server(g, D, tol=1e-7, max_iters=300, topk=10, iters_per_pass=2, max_batches=3,
                  hbm_reserve_bytes=0)
     res = srv.run_trace(arrivals, queries)
+    srv.close()
     assert res.batch.max() == 2          # 200 queries at once fill three 64-column blocks
     assert (res.done >= res.admit).all()
+    # results from every block, several of them in flight together, equal the query alone
+    alone = PprBatch(g, D, tol=1e-7, max_iters=300)
+    for blk in range(3):
+        for qi in np.flatnonzero(res.batch == blk)[::17][:4]:
+            alone.admit(3, queries[qi])
+            alone.run_until_done([3], chunk=2)
+            ids, vals = alone.topk([3], 10)
+            alone.release(3)
+            assert np.array_equal(ids[0], res.topk_ids[qi]), (blk, qi)
+            assert np.array_equal(vals[0], res.topk_vals[qi]), (blk, qi)
+    alone.close()
