@@ -268,6 +268,20 @@ int pe_pull_sum(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted);
 #define PE_PULL_SUM_NOALLOC_MAX_BYTES (64u << 20)
 int pe_pull_sum_noalloc(pe_graph_t* g, pe_vec_t* x, pe_vec_t* y, int weighted);
 
+/* The initial state of the rank algorithms (the curt_rank.foreach initialisers of
+ * pagerank.hpp:92-100 and personalized_pagerank.hpp:73-83): x[v] = value for every v
+ * (seeds NULL) or for v in seeds (0 elsewhere), divided by od(v) when divide_by_od and
+ * od(v) > 0.  x: PE_F32 or PE_F64, length V. */
+int pe_rank_init(pe_graph_t* g, pe_vec_t* x, const pe_bitmap_t* seeds, double value, int divide_by_od);
+
+/* Push-mode broadcast_message with the push operators of push/pull PPR
+ * (personalized_pagerank.hpp:222-239): every vertex in `actives` adds x[v] to y[w] for
+ * each out-edge v->w (FP atomic adds, the reference's write_add, so the last bits can
+ * vary run to run), and each such w joins `actives` once the step is over (the step
+ * pushes from the set it started with).  x, y: both PE_F32 or both PE_F64, length V.
+ * Needs PE_GRAPH_BUILD_CSR. */
+int pe_push_add(pe_graph_t* g, pe_bitmap_t* actives, pe_vec_t* x, pe_vec_t* y);
+
 /* Restart-apply of the rank algorithms (the dense_state_t::foreach lambdas of
  * pagerank.hpp:118-140, weighted_pagerank.hpp:130-148,
  * personalized_pagerank.hpp:101-125, trustrank.hpp:226-250). */

@@ -192,6 +192,8 @@ SIGNATURES = {
     "pe_bfs_step": [_vp, _i32, _vp, _vp, _P(_u64), _P(_u64)],
     "pe_bfs_get_levels": [_vp, _vp],
     "pe_bfs_bind": [_vp, _vp],
+    "pe_push_add": [_vp, _vp, _vp, _vp],
+    "pe_rank_init": [_vp, _vp, _vp, _dbl, _i32],
     "pe_autotune": [_vp, _u32, _P(TuneResult)],
     "pe_autotune_save": [_vp, _P(TuneResult), C.c_char_p],
     "pe_autotune_load": [_vp, C.c_char_p, _P(TuneResult), _P(_i32)],

@@ -19,7 +19,7 @@ __global__ void k_init_rank(uint32_t V, const uint32_t* __restrict__ od, const u
                             double init_dist, int divide_by_od, T* __restrict__ x) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
-    double val = seeds ? (((seeds[v >> 6] >> (v & 63)) & 1ull) ? init_dist : 0.0) : 1.0;
+    double val = seeds ? (((seeds[v >> 6] >> (v & 63)) & 1ull) ? init_dist : 0.0) : init_dist;
     if (divide_by_od && od[v] > 0) val = val / (double)od[v];
     x[v] = (T)val;
   }
@@ -432,6 +432,62 @@ int pe_ppr(pe_graph_t* g, const uint32_t* seeds, uint32_t nseeds, const pe_rank_
     run.run(opts->iterations, opts->eps, opts->l1_norm_tol);
     run.download(out, deltas);
     run.fill_stats(stats);
+  });
+}
+
+int pe_rank_init(pe_graph_t* g, pe_vec_t* x, const pe_bitmap_t* seeds, double value, int divide_by_od) {
+  if (!g || !x) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(x->dtype == PE_F32 || x->dtype == PE_F64, PE_ERR_INVALID, "x must be PE_F32 or PE_F64");
+    PE_REQUIRE(x->n >= g->V, PE_ERR_INVALID, "vector shorter than V");
+    PE_REQUIRE(!seeds || seeds->bits >= g->V, PE_ERR_INVALID, "bitmap smaller than the vertex set");
+    hipStream_t s = g->ctx->stream;
+    const uint64_t* sw = seeds ? seeds->p : nullptr;
+    if (x->dtype == PE_F64)
+      k_init_rank<double><<<grid_for(g->V, 256, 8192), 256, 0, s>>>(g->V, g->od.p, sw, value, divide_by_od,
+                                                                  static_cast<double*>(x->p));
+    else
+      k_init_rank<float><<<grid_for(g->V, 256, 8192), 256, 0, s>>>(g->V, g->od.p, sw, value, divide_by_od,
+                                                                 static_cast<float*>(x->p));
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int pe_push_add(pe_graph_t* g, pe_bitmap_t* actives, pe_vec_t* x, pe_vec_t* y) {
+  if (!g || !actives || !x || !y) return PE_ERR_INVALID;
+  return guarded(g->ctx, [&] {
+    PE_REQUIRE(x->dtype == y->dtype && (x->dtype == PE_F32 || x->dtype == PE_F64), PE_ERR_INVALID,
+               "x and y must both be PE_F32 or both PE_F64");
+    PE_REQUIRE(x->n >= g->V && y->n >= g->V, PE_ERR_INVALID, "vectors shorter than V");
+    PE_REQUIRE(actives->bits >= g->V, PE_ERR_INVALID, "bitmap smaller than the vertex set");
+    PE_REQUIRE(g->has_csr, PE_ERR_STATE, "push mode needs out-edges (PE_GRAPH_BUILD_CSR)");
+    pe_ctx_s* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    const uint32_t V = g->V;
+    const uint64_t nwords = ((uint64_t)V >> 6) + 1;
+    dbuf<uint64_t> newly(ctx, nwords);
+    k_zero_t<uint64_t><<<grid_for(nwords, 256, 4096), 256, 0, s>>>(newly.p, nwords);
+    hub_table hubs;  // out-edge hubs, pushed in chunks
+    hub_table_build(ctx, g->csr_ptr(), V, g->nnz, PPR_HUB_DEG, PPR_HUB_CHUNK, hubs);
+    const unsigned pg = grid_for(nwords * 64, 256, 4096);
+    const unsigned hg = persistent_grid(ctx, 1u << 20, 4);
+    if (x->dtype == PE_F64) {
+      k_ppr_push<double><<<pg, 256, 0, s>>>(actives->p, nwords, V, g->csr_ptr(), g->csr_idx(),
+                                           static_cast<const double*>(x->p), static_cast<double*>(y->p), newly.p);
+      k_ppr_push_hubs<double><<<hg, 256, 0, s>>>(actives->p, g->csr_ptr(), g->csr_idx(), hubs.hubs.p, hubs.count.p,
+                                                 hubs.start.p, static_cast<const double*>(x->p),
+                                                 static_cast<double*>(y->p), newly.p);
+    } else {
+      k_ppr_push<float><<<pg, 256, 0, s>>>(actives->p, nwords, V, g->csr_ptr(), g->csr_idx(),
+                                          static_cast<const float*>(x->p), static_cast<float*>(y->p), newly.p);
+      k_ppr_push_hubs<float><<<hg, 256, 0, s>>>(actives->p, g->csr_ptr(), g->csr_idx(), hubs.hubs.p, hubs.count.p,
+                                                hubs.start.p, static_cast<const float*>(x->p),
+                                                static_cast<float*>(y->p), newly.p);
+    }
+    k_or_words<<<grid_for(nwords, 256, 4096), 256, 0, s>>>(actives->p, newly.p, nwords);
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipStreamSynchronize(s));
   });
 }
 

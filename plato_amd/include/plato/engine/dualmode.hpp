@@ -46,6 +46,18 @@ struct write_add {
   dense_state_t<T, sequence_balanced_by_destination_t>& dst;
 };
 
+// Push operators of push/pull PPR (personalized_pagerank.hpp:222-239): every active v
+// sends src[v]; the slot adds it to dst[w] for each out-neighbour w and activates w (the
+// active set grows after the step).
+template <typename T>
+struct push_signal {
+  dense_state_t<T, sequence_balanced_by_destination_t>& src;
+};
+template <typename T>
+struct push_add_slot {
+  dense_state_t<T, sequence_balanced_by_destination_t>& dst;
+};
+
 // BFS operators share one device BFS state; `next` receives the next frontier.
 struct bfs_state {
   pe_bfs_t* h = nullptr;
@@ -124,6 +136,13 @@ class dualmode_engine_t {
   void reverse() { reversed_ = !reversed_; }
   bool is_reversed() { return reversed_; }
 
+  // the engine's out-degrees (generate_dense_out_degrees_fg, kept by the engine)
+  eid_t active_edges(v_subset_t& actives) {
+    uint64_t a = 0;
+    check_status(pe_active_edges(graph(), actives.handle(), &a), "pe_active_edges");
+    return (eid_t)a;
+  }
+
   // the graph whose in-edges a pull and whose out-edges a push walk
   pe_graph_t* walk_graph() {
     if (!reversed_) return graph();
@@ -161,8 +180,9 @@ class dualmode_engine_t {
 
   template <typename R, typename T>
   R pull_dispatch(ops::gather_sum<T>& sig, ops::write_add<T>& slot) {
-    check_status(pe_pull_sum(walk_graph(), sig.src.handle(), slot.dst.handle(), sig.weighted ? 1 : 0), "pe_pull_sum");
-    slot.dst.invalidate();
+    // runs when dst is next used: fused with a restart-apply that follows (pe_rank_iterate),
+    // else alone (pe_pull_sum)
+    slot.dst.defer_pull(walk_graph(), sig.src, sig.weighted);
     return R();  // the PageRank slot returns 0 for every message
   }
   template <typename R>
@@ -178,6 +198,12 @@ class dualmode_engine_t {
   template <typename R>
   R push_dispatch(ops::bfs_push_signal& sig, ops::bfs_push_slot&, v_subset_t& actives) {
     return bfs_step<R>(sig.s, 1, &actives);
+  }
+  template <typename R, typename T>
+  R push_dispatch(ops::push_signal<T>& sig, ops::push_add_slot<T>& slot, v_subset_t& actives) {
+    check_status(pe_push_add(walk_graph(), actives.handle(), sig.src.handle(), slot.dst.handle()), "pe_push_add");
+    slot.dst.invalidate();
+    return R();
   }
   template <typename R, typename A, typename B>
   R push_dispatch(A&, B&, v_subset_t&) {

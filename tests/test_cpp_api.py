@@ -72,6 +72,36 @@ def test_pagerank_driver_binned_layout(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+def test_ppr_driver_forms_match_oracle(tmp_path, precision):
+    """Both PPR forms of the C++ API (the reference's control flow over the engine's
+    operators: pull-only, and push/pull with the sticky switch) on a directed RMAT-11
+    from a CSV file, against the oracle's pull PPR.  FP64: relative 1e-10 (the push steps
+    add with FP64 atomics, so the last bits vary); FP32: L1 <= 1e-6."""
+    s, d = oracle.rmat_edges(11, 8, 3)
+    V = int(max(s.max(), d.max())) + 1
+    path = tmp_path / "g.csv"
+    path.write_bytes(oracle.format_csv(s, d))
+    og = oracle.CSC(V, s, d)
+    src = int(np.argmax(og.od))
+    ref, _ = oracle.ppr(og, [src], 40, 0.85, 0.0)
+    for form in ("pull", "pushpull"):
+        out = tmp_path / form
+        out.mkdir()
+        r = subprocess.run([os.path.join(BIN, "personalized_pagerank"), "--input", str(path), "--is_directed=true",
+                            "--src", str(src), "--iterations", "40", "--eps", "0", "--type", form,
+                            "--precision", str(precision), "--output", str(out)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = np.loadtxt(out / "0000_0.csv", delimiter=",")
+        assert np.array_equal(got[:, 0], np.arange(V))
+        if precision == 64:
+            np.testing.assert_allclose(got[:, 1], ref, rtol=1e-10, atol=1e-18)
+        else:
+            assert np.abs(got[:, 1] - ref).sum() <= 1e-6, form
+
+
+@pytest.mark.gpu
 def test_bfs_driver_levels(tmp_path):
     path = os.path.join(GOLDEN, "v100_e2150_ua_c3.csv")
     s, d = oracle.read_edge_csv(path)
