@@ -1,7 +1,8 @@
 // plato::algo::trustrank on the device engine (plato/algo/trustrank/trustrank.hpp:23-259):
 // generate_rank_pairs, select_good_seeds and trustrank with the reference's options and
-// result shapes.  Seed ranking and the iteration run on the device (pe_trustrank_*);
-// inverse PageRank pulls over a reversed copy of the graph built for the call.
+// result shapes.  Seed ranking runs on the device (pe_trustrank_select; inverse PageRank
+// pulls over a reversed copy of the graph built for the call), and trustrank follows the
+// reference's loop over the engine's operators.
 #pragma once
 
 #include <vector>
@@ -89,20 +90,53 @@ bitmap_t<> select_good_seeds(INCOMING& in_edges, OUTGOING& /*out_edges*/, const 
   return out;
 }
 
-// trustrank (trustrank.hpp:163-259): each vertex's rank, dense
+namespace detail {
+template <typename T, typename INCOMING, typename OUTGOING>
+dense_state_t<double, typename INCOMING::partition_t> trustrank_loop(INCOMING& in_edges, OUTGOING& out_edges,
+                                                                     const graph_info_t& graph_info,
+                                                                     const bitmap_t<>& good_vertices,
+                                                                     const trustrank_opts_t& opts) {
+  using rank_state_t = dense_state_t<T, typename INCOMING::partition_t>;
+  dualmode_engine_t<INCOMING, OUTGOING> engine(std::shared_ptr<INCOMING>(&in_edges, [](INCOMING*) {}),
+                                               std::shared_ptr<OUTGOING>(&out_edges, [](OUTGOING*) {}), graph_info);
+  pe_graph_t* g = engine.walk_graph();
+  // select good seeds by opts.select_method
+  bitmap_t<> good_seeds = select_good_seeds(in_edges, out_edges, graph_info, good_vertices, opts);
+  if (good_seeds.count() == 0) fatal("trustrank", PE_ERR_STATE, "no good seeds were selected");
+
+  rank_state_t curt_rank = engine.template alloc_v_state<T>();
+  rank_state_t next_rank = engine.template alloc_v_state<T>();
+  const double init_dist = 1. / good_seeds.count();
+  // [v in seeds] * init_dist / od (trustrank.hpp:197-206)
+  double delta = curt_rank.template foreach<double>(ops::rank_init{g, init_dist, &good_seeds, true});
+
+  for (uint32_t epoch_i = 0; epoch_i < opts.iteration_; ++epoch_i) {
+    next_rank.fill(0.0);
+    engine.template foreach_edges<T, int>(ops::gather_sum<T>{curt_rank}, ops::write_add<T>{next_rank});
+    const bool final_iter = opts.iteration_ - 1 == epoch_i;
+    // use good seeds to compute: ([v in seeds] ? (1 - d) * init_dist : 0) + d * sum
+    delta = next_rank.template foreach<double>(ops::rank_apply<rank_state_t>{
+        g, &curt_rank, opts.damping_, 0.0, (1.0 - opts.damping_) * init_dist, &good_seeds, true, final_iter});
+    if (!final_iter && opts.eps_ > 0.0 && delta < opts.eps_) epoch_i = opts.iteration_ - 2;
+    std::swap(curt_rank, next_rank);
+  }
+  return to_fp64(curt_rank);
+}
+}  // namespace detail
+
+// trustrank (trustrank.hpp:163-259): each vertex's rank, dense.  The reference's control
+// flow over the engine's operators: seeds chosen by select_good_seeds, the state started
+// on them, then per epoch a pull-only foreach_edges and a restart-apply foreach on the
+// seed set, with the eps jump.
 template <typename INCOMING, typename OUTGOING>
-dense_state_t<double, typename INCOMING::partition_t> trustrank(INCOMING& in_edges, OUTGOING& /*out_edges*/,
+dense_state_t<double, typename INCOMING::partition_t> trustrank(INCOMING& in_edges, OUTGOING& out_edges,
                                                              const graph_info_t& graph_info,
                                                              const bitmap_t<>& good_vertices,
                                                              const trustrank_opts_t& opts = trustrank_opts_t()) {
-  const uint32_t V = in_edges.info().V;
-  auto good = detail::good_words(good_vertices, V);
-  detail::reverse_guard rev(in_edges.handle(), graph_info, opts.select_method_);
-  const pe_trustrank_opts_t o = detail::to_c(opts);
-  std::vector<double> out(V);
-  check_status(pe_trustrank(in_edges.handle(), rev.h, good.data(), &o, out.data(), nullptr, nullptr),
-               "pe_trustrank");
-  return dense_state_t<double, typename INCOMING::partition_t>::from_host(std::move(out));
+  if (opts.precision_ == 32) return detail::trustrank_loop<float>(in_edges, out_edges, graph_info, good_vertices, opts);
+  if (opts.precision_ != 64 && opts.precision_ != 0)
+    fatal("trustrank", PE_ERR_INVALID, "precision_ must be 32 or 64");
+  return detail::trustrank_loop<double>(in_edges, out_edges, graph_info, good_vertices, opts);
 }
 
 }  // namespace algo
