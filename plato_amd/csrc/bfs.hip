@@ -340,7 +340,8 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
   for (uint64_t i = threadIdx.x; i < nh; i += blockDim.x) hot[i] = cur_bits[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  unsigned long long active = 0, found = 0, examined = 0;
+  unsigned long long active = 0, found = 0;
+  uint32_t examined = 0;  // a lane's share stays far below 2^32 (at most PULL_CAP per vertex)
   for (uint64_t word = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; word < nwords;
        word += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     const uint64_t v = word * 64 + lane;
@@ -366,19 +367,21 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
             uint32_t us[PULL_SCAN_UNROLL];
 #pragma unroll
             for (int k = 0; k < PULL_SCAN_UNROLL; ++k) us[k] = i + k < lim ? row_idx[i + k] : UNVIS;
+            uint32_t hits = 0;  // bit k: entry i + k is in the frontier
 #pragma unroll
             for (int k = 0; k < PULL_SCAN_UNROLL; ++k) {
               const uint32_t u = us[k];
               if (u != UNVIS) {
                 const uint64_t wd = (u >> 6) < nh ? hot[u >> 6] : cur_bits[u >> 6];
-                if (((wd >> (u & 63)) & 1ull) && !claimed) {
-                  claimed = true;
-                  examined += i + k - b;  // entries b+1 .. i+k
-                }
+                hits |= (uint32_t)((wd >> (u & 63)) & 1ull) << k;
               }
             }
+            if (hits) {
+              claimed = true;
+              examined += (uint32_t)(i - b) + (uint32_t)__builtin_ctz(hits);  // entries b+1 .. the first hit
+            }
           }
-          if (!claimed) examined += lim - b - 1;  // every entry up to the cap
+          if (!claimed) examined += (uint32_t)(lim - b - 1);  // every entry up to the cap
           is_heavy = !claimed && e > lim;
         }
       }
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(PULL_THREADS) void k_bfs_pull(
       active += od[v];
     }
   }
-  block_add3(active, found, examined, &cnt->next_active, &cnt->next_count, &cnt->examined);
+  block_add3(active, found, (unsigned long long)examined, &cnt->next_active, &cnt->next_count, &cnt->examined);
 }
 
 // second pull pass: one wave per heavy vertex, 64 in-edges per step, stop at the first hit
