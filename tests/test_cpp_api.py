@@ -21,6 +21,20 @@ def test_host_programs_build():
         assert os.access(os.path.join(BIN, p), os.X_OK)
 
 
+def test_deferred_pull_call_sequences(tmp_path):
+    """The engine's deferred pull against a call-logging double of the C-ABI (CPU only)."""
+    exe = str(tmp_path / "test_deferred_pull")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{ROOT}/include", f"-I{ROOT}/plato_amd/include",
+                        os.path.join(ROOT, "tests", "cpp", "test_deferred_pull.cc"),
+                        os.path.join(ROOT, "tests", "cpp", "abi_call_log.cc"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert "0 failure(s)" in r.stdout
+
+
 def test_unsupported_functor_is_a_compile_error():
     src = ('#include "plato/engine/dualmode.hpp"\n'
            'int main(){ plato::graph_info_t gi(true);\n'
