@@ -278,10 +278,12 @@ struct pb_layout {
   dbuf<pb_unit2> units2;
   dbuf<uint32_t> split_bin;             // [nsplit] bins summed by several units
   dbuf<unsigned long long> acc;         // [nsplit << dst_bits] fixed-point sums of split bins
+  dbuf<uint32_t> work;                  // [4] unit queues: scatter next / finished, gather next /
+                                        // finished (zero between launches)
   uint64_t bytes() const {
     return src_upd.bytes() + dst_local.bytes() + gbase.bytes() + w.bytes() + values.bytes() +
            rec_start.bytes() + rec_len.bytes() + units1.bytes() + units2.bytes() + split_bin.bytes() +
-           acc.bytes();
+           acc.bytes() + work.bytes();
   }
 };
 }  // namespace pe

@@ -68,6 +68,25 @@ static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the 
 #endif
 
 
+// Units are taken from a queue (q[0]: next unit, q[1]: blocks finished): the unit sizes
+// vary, and a block that has finished its unit takes the next one.  The last block to
+// finish sets the queue back to zero for the next launch.
+__device__ __forceinline__ uint32_t pb_take_unit(uint32_t* q, uint32_t* s_unit) {
+  if (threadIdx.x == 0) *s_unit = atomicAdd(&q[0], 1u);
+  __syncthreads();
+  return *s_unit;
+}
+
+__device__ __forceinline__ void pb_queue_done(uint32_t* q) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&q[1], 1u) == gridDim.x - 1) {
+      q[0] = 0u;
+      q[1] = 0u;
+    }
+  }
+}
+
 // Adds one converted term into a destination's accumulator word.
 __device__ __forceinline__ void acc_add(unsigned long long* acc, uint64_t v) {
   if (v) atomicAdd(acc, (unsigned long long)v);
@@ -113,8 +132,9 @@ template <int RB, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_pb_scatter(
     const pb_unit1* __restrict__ units, uint32_t nunits, const uint64_t* __restrict__ rec_start,
     const uint32_t* __restrict__ rec_len, const uint16_t* __restrict__ src_upd, uint32_t V, int src_bits,
-    const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl, float* __restrict__ values) {
-  extern __shared__ float pb_scatter_lds[];  // the chunk's 2^src_bits sources
+    const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl, float* __restrict__ values,
+    uint32_t* __restrict__ queue) {
+  extern __shared__ float pb_scatter_lds[];  // the chunk's 2^src_bits sources, then the unit slot
   float* xs = pb_scatter_lds;
   const float* x = buf0;
   if (ctl) {
@@ -124,7 +144,10 @@ __global__ __launch_bounds__(THREADS) void k_pb_scatter(
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int WAVES = THREADS / 64;
   constexpr uint64_t RSTEP = (uint64_t)WAVES * RB;
-  for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+  uint32_t* s_unit = reinterpret_cast<uint32_t*>(pb_scatter_lds + (1u << src_bits));
+  for (;;) {
+    const uint32_t ui = pb_take_unit(queue, s_unit);
+    if (ui >= nunits) break;
     const pb_unit1 u = units[ui];
     const uint64_t base = (uint64_t)u.chunk << src_bits;
     const uint32_t n = (uint32_t)std::min<uint64_t>(1ull << src_bits, (uint64_t)V - base);
@@ -164,6 +187,7 @@ __global__ __launch_bounds__(THREADS) void k_pb_scatter(
     }
     __syncthreads();
   }
+  pb_queue_done(queue);
 }
 
 // ---- gather + apply -------------------------------------------------------------------------
@@ -198,11 +222,14 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
     const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ step_base,
     uint32_t V, int dst_bits, int frac,
     float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
-    unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum) {
-  extern __shared__ unsigned long long pb_gather_lds[];  // 2^dst_bits accumulators, then the reduction
+    unsigned long long* __restrict__ acc, double* __restrict__ part_delta, double* __restrict__ part_rsum,
+    uint32_t* __restrict__ queue) {
+  // 2^dst_bits accumulators, then the reduction, then the unit slot
+  extern __shared__ unsigned long long pb_gather_lds[];
   constexpr int PB_WAVES = THREADS / 64;
   unsigned long long* sacc = pb_gather_lds;
   double* s_red = reinterpret_cast<double*>(pb_gather_lds + (1u << dst_bits));
+  uint32_t* s_unit = reinterpret_cast<uint32_t*>(s_red + 2 * PB_WAVES);
   bool final_iter = a.final_iter;
   const float* x = buf0;
   float* y = buf1;
@@ -217,7 +244,9 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t pd = 1u << dst_bits;
   double delta = 0.0, rsum = 0.0;
-  for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+  for (;;) {
+    const uint32_t ui = pb_take_unit(queue, s_unit);
+    if (ui >= nunits) break;
     const pb_unit2 u = units[ui];
     for (uint32_t i = tid; i < pd; i += THREADS) sacc[i] = 0ull;
     __syncthreads();
@@ -297,6 +326,7 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
     }
     __syncthreads();
   }
+  pb_queue_done(queue);
   block_reduce2<THREADS>(delta, rsum, s_red);
   if (tid == 0) {
     part_delta[blockIdx.x] = delta;
@@ -676,6 +706,9 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
   L->acc.alloc(ctx, nacc);
   k_pb_zero<<<grid_for(nacc, 256, 65536), 256, 0, s>>>(L->acc.p, nacc);
   PE_HIP(hipGetLastError());
+  L->work.alloc(ctx, 4);
+  k_pb_zero<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(L->work.p), 2);
+  PE_HIP(hipGetLastError());
   PE_HIP(hipStreamSynchronize(s));
   g->pb = std::move(L);
 }
@@ -688,7 +721,7 @@ static unsigned pb_blocks_per_cu(size_t lds, int threads) {
 }
 
 static size_t pb_gather_lds_bytes(uint32_t dst_bits) {
-  return ((size_t)8 << dst_bits) + (size_t)16 * (PB_GATHER_THREADS / 64);
+  return ((size_t)8 << dst_bits) + (size_t)16 * (PB_GATHER_THREADS / 64) + 16;
 }
 
 unsigned binned_grid_gather(pe_graph_s* g) {
@@ -708,14 +741,14 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
   PE_REQUIRE(a.mass > 0.0, PE_ERR_INTERNAL, "binned iteration without a bound on the contributions");
   const int frac = pb_frac_bits(a.mass);
   hipStream_t s = g->ctx->stream;
-  const size_t lds1 = (size_t)4 << L.src_bits;
+  const size_t lds1 = ((size_t)4 << L.src_bits) + 16;
   const unsigned g1 = persistent_grid(g->ctx, std::max<uint32_t>(1, L.n1), pb_blocks_per_cu(lds1, PB_SCATTER_THREADS));
   const size_t lds2 = pb_gather_lds_bytes(L.dst_bits);
   const unsigned g2 = binned_grid_gather(g), gs = binned_grid_split(g);
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_SCATTER);
     k_pb_scatter<PB_SCATTER_RECORDS, PB_SCATTER_THREADS><<<g1, PB_SCATTER_THREADS, lds1, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V,
-                                           (int)L.src_bits, b0, b1, ctl, L.values.p);
+                                           (int)L.src_bits, b0, b1, ctl, L.values.p, L.work.p);
     PE_HIP(hipGetLastError());
   }
   {
@@ -723,11 +756,11 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
     if (weighted)
       k_pb_gather<true, PB_GATHER_STEPS_W, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
                                                   (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
-                                                  part_rsum);
+                                                  part_rsum, L.work.p + 2);
     else
       k_pb_gather<false, PB_GATHER_STEPS, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
                                                    (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
-                                                   part_rsum);
+                                                   part_rsum, L.work.p + 2);
     PE_HIP(hipGetLastError());
     k_pb_apply_split<<<gs, 256, 0, s>>>(L.split_bin.p, L.nsplit, (int)L.dst_bits, frac, g->V, L.acc.p, b0, b1, g->od.p,
                                         a, ctl, part_delta + g2, part_rsum + g2);

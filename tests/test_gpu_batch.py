@@ -101,6 +101,29 @@ def test_pass_modes_bitwise(s13):
     assert float(np.abs(res[0] - ref).sum()) <= 1e-6
 
 
+def test_plan_limits_do_not_change_bits(s13):
+    """The pass plan's limits (pe_ppr_batch_opts_t) pick kernels, never arithmetic: the same
+    staggered queries with every narrow kernel off, with only the packed block off, and with
+    the default plan give the same bits."""
+    from plato_amd import _lib as L
+    from plato_amd.batch import PprBatch
+    g, og = s13
+    seeds = _seeds(og, 20, np.random.default_rng(6))
+    OFF = L.PE_BATCH_PLAN_OFF
+    out = []
+    for plan in ((0, 0, 0), (OFF, OFF, OFF), (OFF, 0, 0), (0, OFF, OFF)):
+        b = PprBatch(g, D, tol=1e-7, max_iters=300, plan=plan)
+        for a0, a1, n in [(0, 2, 5), (2, 12, 5), (12, 20, 3)]:
+            for slot in range(a0, a1):
+                b.admit(slot, [seeds[slot]])
+            b.step(n)
+        b.run_until_done(range(20))
+        out.append(b.results(list(range(20))))
+        b.close()
+    for r in out[1:]:
+        assert np.array_equal(r, out[0])
+
+
 def test_packed_slot_refilled_in_place(s13):
     """A column finishes and its slot is refilled before the next pass while another column
     keeps running: the pass after sees the same covered slots as the last packed pass and
