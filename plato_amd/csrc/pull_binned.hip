@@ -206,14 +206,17 @@ static constexpr uint32_t PB_STEP = 512;
 #define PB_GATHER_STEPS_W 2  // the same for weighted graphs (two more float4 per lane and step)
 #endif
 
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
-  uint32_t x = v;
+// Exclusive prefix sum over the wave of per-lane counts below 16: the count's four bits
+// are summed separately, each as the number of lanes below this one whose bit is set
+// (a ballot and the two mbcnt halves) -- no cross-lane data movement through LDS.
+__device__ __forceinline__ uint32_t wave_excl_scan16(uint32_t v) {
+  uint32_t r = 0;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
-    if (lane >= off) x += y;
+  for (int b = 0; b < 4; ++b) {
+    const uint64_t m = __ballot((v >> b) & 1u);
+    r += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
   }
-  return x - v;
+  return r;
 }
 
 template <bool WEIGHTED, int GU, int THREADS>
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
         uint32_t heads = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) heads += raw[k] >> 15;
-        uint32_t idx = base[q] + wave_excl_scan(heads, lane) - 1u;
+        uint32_t idx = base[q] + wave_excl_scan16(heads) - 1u;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           idx += raw[k] >> 15;
