@@ -106,12 +106,18 @@ PB_HD int pb_frac_bits(double mass) {
   return 61 - e;
 }
 
-// FP32 term -> floor(x * 2^F); zero and subnormals of FP32 give their exact floor too.
+// FP32 term -> floor(x * 2^F) for x non-negative and finite with x * 2^F < 2^64; zero
+// and subnormals of FP32 give their exact floor too.  x = m * 2^(k - F) with the 24-bit
+// significand m, so the term is m shifted left by k >= 0 or right by -k (a right shift
+// by 31 already clears any m); this takes about half the instructions of pb_shl64.
 PB_HD uint64_t fixed64_f32(float v, int F) {
-  const uint32_t b = pb_bits32(v);
-  const uint32_t e = (b >> 23) & 0xFFu;
-  const uint32_t m = e ? ((b & 0x7FFFFFu) | 0x800000u) : (b & 0x7FFFFFu);
-  return pb_shl64(m, (e ? (int)e : 1) - 150 + F);
+  const uint32_t b = pb_bits32(v);  // no sign bit
+  const uint32_t e = b >> 23;
+  const uint32_t m = (b & 0x7FFFFFu) | ((e ? 1u : 0u) << 23);
+  const int k = (e ? (int)e : 1) - 150 + F;
+  const int s = k > 0 ? k : 0;
+  const int t = k >= 0 ? 0 : (-k < 31 ? -k : 31);
+  return (uint64_t)(m >> t) << s;
 }
 
 // FP64 term (weighted graphs: the exact product w * x) -> floor(x * 2^F).
