@@ -120,6 +120,22 @@ PB_HD uint64_t fixed64_f32(float v, int F) {
   return (uint64_t)(m >> t) << s;
 }
 
+// The same term in 32 bits: the shifted significand (below 2^24) in bits 0..23 and the
+// left shift still to apply (0..40, since the term is below 2^64) above it.  The binned
+// scatter codes each source once while staging it, so the gather decodes a term with a
+// mask and two shifts.
+PB_HD uint32_t fixed_code_f32(float v, int F) {
+  const uint32_t b = pb_bits32(v);  // no sign bit
+  const uint32_t e = b >> 23;
+  const uint32_t m = (b & 0x7FFFFFu) | ((e ? 1u : 0u) << 23);
+  const int k = (e ? (int)e : 1) - 150 + F;
+  const int s = k > 0 ? k : 0;
+  const int t = k >= 0 ? 0 : (-k < 31 ? -k : 31);
+  return ((uint32_t)s << 24) | (m >> t);
+}
+
+PB_HD uint64_t fixed_code_term(uint32_t c) { return (uint64_t)(c & 0xFFFFFFu) << (c >> 24); }
+
 // FP64 term (weighted graphs: the exact product w * x) -> floor(x * 2^F).
 PB_HD uint64_t fixed64_f64(double v, int F) {
   const uint64_t b = pb_bits64(v);

@@ -128,14 +128,17 @@ __device__ __forceinline__ void pb_load_sources(const pb_rec_group<RB>& g, const
     }
 }
 
-template <int RB, int THREADS>
+// CODED (unweighted iterations): the chunk is staged as 32-bit fixed-point codes
+// (fixed_code_f32 at F = frac), one conversion per source, and the contributions are
+// codes; otherwise (weighted: the gather forms w * x exactly) they are the FP32 values.
+template <int RB, int THREADS, bool CODED>
 __global__ __launch_bounds__(THREADS) void k_pb_scatter(
     const pb_unit1* __restrict__ units, uint32_t nunits, const uint64_t* __restrict__ rec_start,
     const uint32_t* __restrict__ rec_len, const uint16_t* __restrict__ src_upd, uint32_t V, int src_bits,
-    const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl, float* __restrict__ values,
-    uint32_t* __restrict__ queue) {
-  extern __shared__ float pb_scatter_lds[];  // the chunk's 2^src_bits sources, then the unit slot
-  float* xs = pb_scatter_lds;
+    int frac, const float* buf0, const float* buf1, const iter_ctl* __restrict__ ctl,
+    uint32_t* __restrict__ values, uint32_t* __restrict__ queue) {
+  extern __shared__ uint32_t pb_scatter_lds[];  // the chunk's 2^src_bits sources, then the unit slot
+  uint32_t* xs = pb_scatter_lds;
   const float* x = buf0;
   if (ctl) {
     if (ctl->done) return;
@@ -160,10 +163,17 @@ __global__ __launch_bounds__(THREADS) void k_pb_scatter(
     pb_load_sources<RB>(cur, src_upd, lane, sl);
     if ((n & 3u) == 0) {  // base is a multiple of 16 sources, so x + base is 16-byte aligned
       const float4* x4 = reinterpret_cast<const float4*>(x + base);
-      float4* s4 = reinterpret_cast<float4*>(xs);
-      for (uint32_t i = tid; i < (n >> 2); i += THREADS) s4[i] = x4[i];
+      uint4* s4 = reinterpret_cast<uint4*>(xs);
+      for (uint32_t i = tid; i < (n >> 2); i += THREADS) {
+        const float4 v = x4[i];
+        s4[i] = CODED ? make_uint4(fixed_code_f32(v.x, frac), fixed_code_f32(v.y, frac), fixed_code_f32(v.z, frac),
+                                   fixed_code_f32(v.w, frac))
+                      : make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                                   __float_as_uint(v.w));
+      }
     } else {
-      for (uint32_t i = tid; i < n; i += THREADS) xs[i] = x[base + i];
+      for (uint32_t i = tid; i < n; i += THREADS)
+        xs[i] = CODED ? fixed_code_f32(x[base + i], frac) : __float_as_uint(x[base + i]);
     }
     __syncthreads();
     for (; r < u.r1; r += RSTEP) {
@@ -221,7 +231,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan16(uint32_t v) {
 
 template <bool WEIGHTED, int GU, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_pb_gather(
-    const pb_unit2* __restrict__ units, uint32_t nunits, const float* __restrict__ values,
+    const pb_unit2* __restrict__ units, uint32_t nunits, const uint32_t* __restrict__ values,
     const float* __restrict__ w, const uint16_t* __restrict__ dst_local, const uint32_t* __restrict__ step_base,
     uint32_t V, int dst_bits, int frac,
     float* buf0, float* buf1, const uint32_t* __restrict__ od, rank_args a, const iter_ctl* __restrict__ ctl,
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
           }
         }
       }
-      float xv[GU][8];
+      uint32_t xv[GU][8];  // contributions: fixed-point codes, or FP32 bits when WEIGHTED
 #pragma unroll
       for (int q = 0; q < GU; ++q) {
         if (sb + (uint64_t)q * PB_WAVES >= s1) break;
@@ -303,14 +313,14 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
         if (e0 >= u.e0 && e0 + 8 <= u.e1) {
 #pragma unroll
           for (int k = 0; k < 8; ++k)
-            acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)xv[q][k], frac)
-                                          : fixed64_f32(xv[q][k], frac));
+            acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)__uint_as_float(xv[q][k]), frac)
+                                          : fixed_code_term(xv[q][k]));
         } else {  // the unit's first or last group
 #pragma unroll
           for (int k = 0; k < 8; ++k)
             if (e0 + k >= u.e0 && e0 + k < u.e1)
-              acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)xv[q][k], frac)
-                                            : fixed64_f32(xv[q][k], frac));
+              acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)__uint_as_float(xv[q][k]), frac)
+                                            : fixed_code_term(xv[q][k]));
         }
       }
     }
@@ -750,18 +760,25 @@ void binned_iteration(pe_graph_s* g, float* b0, float* b1, bool weighted, const 
   const unsigned g2 = binned_grid_gather(g), gs = binned_grid_split(g);
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_SCATTER);
-    k_pb_scatter<PB_SCATTER_RECORDS, PB_SCATTER_THREADS><<<g1, PB_SCATTER_THREADS, lds1, s>>>(L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V,
-                                           (int)L.src_bits, b0, b1, ctl, L.values.p, L.work.p);
+    uint32_t* vals = reinterpret_cast<uint32_t*>(L.values.p);
+    if (weighted)
+      k_pb_scatter<PB_SCATTER_RECORDS, PB_SCATTER_THREADS, false><<<g1, PB_SCATTER_THREADS, lds1, s>>>(
+          L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V, (int)L.src_bits, frac, b0, b1, ctl, vals,
+          L.work.p);
+    else
+      k_pb_scatter<PB_SCATTER_RECORDS, PB_SCATTER_THREADS, true><<<g1, PB_SCATTER_THREADS, lds1, s>>>(
+          L.units1.p, L.n1, L.rec_start.p, L.rec_len.p, L.src_upd.p, g->V, (int)L.src_bits, frac, b0, b1, ctl, vals,
+          L.work.p);
     PE_HIP(hipGetLastError());
   }
   {
     ktime_scope kt(g->ctx, PE_KT_PULL_BIN_GATHER);
     if (weighted)
-      k_pb_gather<true, PB_GATHER_STEPS_W, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, L.values.p, L.w.p, L.dst_local.p, L.gbase.p, g->V,
+      k_pb_gather<true, PB_GATHER_STEPS_W, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, reinterpret_cast<const uint32_t*>(L.values.p), L.w.p, L.dst_local.p, L.gbase.p, g->V,
                                                   (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                   part_rsum, L.work.p + 2);
     else
-      k_pb_gather<false, PB_GATHER_STEPS, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, L.values.p, nullptr, L.dst_local.p, L.gbase.p, g->V,
+      k_pb_gather<false, PB_GATHER_STEPS, PB_GATHER_THREADS><<<g2, PB_GATHER_THREADS, lds2, s>>>(L.units2.p, L.n2, reinterpret_cast<const uint32_t*>(L.values.p), nullptr, L.dst_local.p, L.gbase.p, g->V,
                                                    (int)L.dst_bits, frac, b0, b1, g->od.p, a, ctl, L.acc.p, part_delta,
                                                    part_rsum, L.work.p + 2);
     PE_HIP(hipGetLastError());

@@ -7,9 +7,10 @@
 //   3. sums: 2^20 random terms accumulated in the two-word form (any order) equal the
 //      __int128 sum of the truncated terms, and from_fixed of the words is within one
 //      rounding of that sum;
-//   4. the single-word form the kernels use: fixed64_f32 == floor(x * 2^F) for every
-//      FP32 value x with x * 2^F < 2^64, at F = 33 (PageRank on 2^27 vertices) and
-//      F = 60 (PPR); fixed64_f64 on 2^22 FP64 products at both;
+//   4. the single-word form the kernels use: fixed64_f32 and the decoded 32-bit code
+//      fixed_code_term(fixed_code_f32) == floor(x * 2^F) for every FP32 value x with
+//      x * 2^F < 2^64, at F = 33 (PageRank on 2^27 vertices) and F = 60 (PPR);
+//      fixed64_f64 on 2^22 FP64 products at both;
 //   5. pb_frac_bits(M) puts M * 2^F in [2^60, 2^61), and 2^20 terms whose values add
 //      up to M sum in one 64-bit word exactly as in __int128, below 2^61.
 #include <cstdio>
@@ -94,6 +95,10 @@ int main() {
         const u128i R = ref_fixed(m, (e ? (int)e : 1) - 150 + F);
         if (pe::fixed64_f32(x, F) != (uint64_t)R || (R >> 64) != 0) {
           if (bad1 < 5) std::printf("fixed64 mismatch F=%d bits=%08x\n", F, bits);
+          ++bad1;
+        }
+        if (pe::fixed_code_term(pe::fixed_code_f32(x, F)) != (uint64_t)R) {
+          if (bad1 < 5) std::printf("fixed code mismatch F=%d bits=%08x\n", F, bits);
           ++bad1;
         }
       }
