@@ -92,6 +92,13 @@ __device__ __forceinline__ void acc_add(unsigned long long* acc, uint64_t v) {
   if (v) atomicAdd(acc, (unsigned long long)v);
 }
 
+// The same for a coded term (fixed_code_f32): zero exactly when its significand field is,
+// since the left shift loses no bits.
+__device__ __forceinline__ void acc_add_code(unsigned long long* acc, uint32_t c) {
+  const uint32_t m = c & 0xFFFFFFu;
+  if (m) atomicAdd(acc, (unsigned long long)m << (c >> 24));
+}
+
 // ---- scatter ----------------------------------------------------------------------------
 // A wave takes RB run records at a time (a record holds at most 256 updates, one pass of
 // 4 x 64 lanes).  The chain per record group -- descriptors, then the sources of its
@@ -312,15 +319,21 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
                                dd[q].w & 0x7FFFu, (dd[q].w >> 16) & 0x7FFFu};
         if (e0 >= u.e0 && e0 + 8 <= u.e1) {
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
-            acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)__uint_as_float(xv[q][k]), frac)
-                                          : fixed_code_term(xv[q][k]));
+          for (int k = 0; k < 8; ++k) {
+            if (WEIGHTED)
+              acc_add(&sacc[d[k]], fixed64_f64((double)wv[k] * (double)__uint_as_float(xv[q][k]), frac));
+            else
+              acc_add_code(&sacc[d[k]], xv[q][k]);
+          }
         } else {  // the unit's first or last group
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
-            if (e0 + k >= u.e0 && e0 + k < u.e1)
-              acc_add(&sacc[d[k]], WEIGHTED ? fixed64_f64((double)wv[k] * (double)__uint_as_float(xv[q][k]), frac)
-                                            : fixed_code_term(xv[q][k]));
+          for (int k = 0; k < 8; ++k) {
+            if (e0 + k < u.e0 || e0 + k >= u.e1) continue;
+            if (WEIGHTED)
+              acc_add(&sacc[d[k]], fixed64_f64((double)wv[k] * (double)__uint_as_float(xv[q][k]), frac));
+            else
+              acc_add_code(&sacc[d[k]], xv[q][k]);
+          }
         }
       }
     }
