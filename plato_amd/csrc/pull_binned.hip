@@ -275,23 +275,23 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
       uint4 dd[GU];
       uint32_t base[GU];
       float4 wa[GU], wb[GU];
+      // Every step's loads are issued without a branch (a step past the unit loads the
+      // unit's last step again, whose entries then fail the range test below and add
+      // nothing), so the compiler counts the loads in flight and waits for each one only.
 #pragma unroll
       for (int q = 0; q < GU; ++q) {
-        const uint64_t st = sb + (uint64_t)q * PB_WAVES;
-        if (st < s1) {
-          dd[q] = reinterpret_cast<const uint4*>(dst_local + st * PB_STEP)[lane];
-          base[q] = step_base[st];
-          if (WEIGHTED) {
-            const float4* wq = reinterpret_cast<const float4*>(w + st * PB_STEP) + 2 * lane;
-            wa[q] = wq[0];
-            wb[q] = wq[1];
-          }
+        const uint64_t st = std::min<uint64_t>(sb + (uint64_t)q * PB_WAVES, s1 - 1);
+        dd[q] = reinterpret_cast<const uint4*>(dst_local + st * PB_STEP)[lane];
+        base[q] = step_base[st];
+        if (WEIGHTED) {
+          const float4* wq = reinterpret_cast<const float4*>(w + st * PB_STEP) + 2 * lane;
+          wa[q] = wq[0];
+          wb[q] = wq[1];
         }
       }
       uint32_t xv[GU][8];  // contributions: fixed-point codes, or FP32 bits when WEIGHTED
 #pragma unroll
       for (int q = 0; q < GU; ++q) {
-        if (sb + (uint64_t)q * PB_WAVES >= s1) break;
         const uint32_t raw[8] = {dd[q].x & 0xFFFFu, dd[q].x >> 16, dd[q].y & 0xFFFFu, dd[q].y >> 16,
                                  dd[q].z & 0xFFFFu, dd[q].z >> 16, dd[q].w & 0xFFFFu, dd[q].w >> 16};
         uint32_t heads = 0;
@@ -306,8 +306,7 @@ __global__ __launch_bounds__(THREADS) void k_pb_gather(
       }
 #pragma unroll
       for (int q = 0; q < GU; ++q) {
-        const uint64_t st = sb + (uint64_t)q * PB_WAVES;
-        if (st >= s1) break;
+        const uint64_t st = sb + (uint64_t)q * PB_WAVES;  // past the unit: every entry fails the range test
         const uint64_t e0 = st * PB_STEP + 8u * (uint32_t)lane;
         float wv[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
         if (WEIGHTED) {
