@@ -18,7 +18,9 @@
 // Sums are exact and order-independent.  Each contribution (non-negative, finite)
 // becomes the integer floor(x * 2^F) (the FP64 products w * x of weighted graphs are
 // formed exactly first), and these integers are added with 64-bit LDS atomics into one
-// word per destination.  F is fixed per iteration from a bound M on the total of all
+// word per destination.  For unweighted iterations the scatter stages each source as
+// a 32-bit code of its term (pb_fixed.hpp), so the gather's per-entry work is a mask
+// and two shifts; weighted iterations carry the FP32 values.  F is fixed per iteration from a bound M on the total of all
 // contributions (pb_frac_bits: M * 2^F <= 2^61): for the rank iterations M is the
 // rank mass, at most V for PageRank (ranks start at 1 and the restart adds
 // (1 - d) per vertex while the damped flow keeps at most d of the mass) and at most 1
@@ -30,10 +32,11 @@
 // Layout (bin order): every CSC entry (v <- u) gets the key
 //   bin = v >> dst_bits | chunk = u >> src_bits | u & smask | v & dmask
 // and the entries are radix-sorted by it.  An update is one (bin, source) pair: the
-// scatter writes x[u] once per update, not once per edge (partition-centric, as in
-// Lakhotia et al.), and every entry carries its destination plus a flag bit marking
-// the first entry of a new update; a prefix count per aligned group of 8 entries lets
-// the gather find each entry's update.  A run is the updates of one (bin, chunk)
+// scatter writes the source's contribution once per update, not once per edge
+// (partition-centric, as in Lakhotia et al.), and every entry carries its destination
+// plus a flag bit marking the first entry of a new update; the update count before
+// each 512-entry step plus a prefix count of the flags lets the gather find each
+// entry's update.  Scatter and gather blocks take their units from a queue.  A run is the updates of one (bin, chunk)
 // pair.  The scatter walks run records (runs cut to at most run_cap updates) grouped
 // by chunk; the gather walks bins, split into units of at most 2^unit_log2 entries
 // (units of a split bin meet in global 64-bit accumulators, applied by a last pass).
