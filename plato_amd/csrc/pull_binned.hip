@@ -720,7 +720,8 @@ void binned_build(pe_graph_s* g, const pe_binned_opts_t* opts) {
     std::stable_sort(u2.begin(), u2.end(),
                      [](const pb_unit2& a, const pb_unit2& b) { return (a.e1 - a.e0) > (b.e1 - b.e0); });
   }
-  PE_REQUIRE(u1.size() < 0xFFFFFFFFull && u2.size() < 0xFFFFFFFFull, PE_ERR_RANGE, "too many binned units");
+  // (the unit queues hand out one ticket per block past the last unit: counts stay far below 2^32)
+  PE_REQUIRE(u1.size() < (1ull << 31) && u2.size() < (1ull << 31), PE_ERR_RANGE, "too many binned units");
   L->n1 = (uint32_t)u1.size();
   L->n2 = (uint32_t)u2.size();
   L->nsplit = (uint32_t)split.size();
