@@ -185,7 +185,8 @@ typedef struct {
   uint32_t src_bits;    /* log2 sources per chunk staged in LDS (<= 15; default 15) */
   uint32_t dst_bits;    /* log2 destinations per bin summed in LDS (<= 14; default 14) */
   uint32_t run_cap;     /* max updates per scatter run record, 64-256 (default 256) */
-  uint32_t unit_log2;   /* log2 max entries per gather unit; bigger bins are split (default 20) */
+  uint32_t unit_log2;   /* log2 max entries per gather unit; bigger bins are split (default: the
+                           largest power of two up to half a CU's share of the entries, >= 10) */
 } pe_binned_opts_t;
 int pe_graph_set_pull_layout(pe_graph_t* g, int layout, const pe_binned_opts_t* opts);
 
