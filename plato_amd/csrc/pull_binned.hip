@@ -72,11 +72,16 @@ static constexpr uint32_t PB_SEG = 1u << 16;      // entries per segment of the 
 
 // Units are taken from a queue (q[0]: next unit, q[1]: blocks finished): the unit sizes
 // vary, and a block that has finished its unit takes the next one.  The last block to
-// finish sets the queue back to zero for the next launch.
+// finish sets the queue back to zero for the next launch.  The unit index is read back
+// as a wave-uniform value (every thread sees the same one), so the compiler treats the
+// loop's exit test as uniform and keeps the loop whole around the block's barriers.
+// A loop body must end with a barrier after any thread-0-only work: the compiler may
+// otherwise merge that work with the next take across the back edge and let the other
+// threads reach the next unit's barrier ahead of thread 0.
 __device__ __forceinline__ uint32_t pb_take_unit(uint32_t* q, uint32_t* s_unit) {
   if (threadIdx.x == 0) *s_unit = atomicAdd(&q[0], 1u);
   __syncthreads();
-  return *s_unit;
+  return __builtin_amdgcn_readfirstlane(*s_unit);
 }
 
 __device__ __forceinline__ void pb_queue_done(uint32_t* q) {
