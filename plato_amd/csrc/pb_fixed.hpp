@@ -3,9 +3,9 @@
 // The kernels use the single-word form: a term x (non-negative, finite) is the integer
 // floor(x * 2^F) in one 64-bit word (FP32 terms travel as the 32-bit code of
 // fixed_code_f32 and are widened by fixed_code_term; fixed64_f32 is the same term
-// computed directly, kept for the host check; weighted terms use fixed64_f64), with F fixed for a whole
-// iteration by pb_frac_bits(M), M a bound on the total of all terms (the rank mass
-// that flows along the edges): every per-destination sum of terms then stays below
+// computed directly, kept for the host check; weighted terms use fixed64_f64), with F
+// fixed for a whole iteration by pb_frac_bits(M), M a bound on the total of all terms
+// (the rank mass that flows along the edges): every per-destination sum of terms then stays below
 // 2^61, so 64-bit integer sums cannot overflow and are exact and order-independent.
 // Each term is truncated at 2^-F (at most one unit of 2^-F per term).
 //
